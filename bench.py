@@ -1,0 +1,169 @@
+"""Throughput benchmark of the HIP progressive path tracer on one MI355X.
+
+A step is one progressive sample over the whole frame of the workload (for
+the default 1080p layered-terrain workload: 1920x1080 paths, max depth 8).
+The scene, its BVH and the path-state buffers are resident in HBM before the
+timed region; the timed region is K steps enqueued back to back on the
+session stream, bracketed by a device synchronize on both sides and timed
+both by the host clock and by HIP events on that stream.
+
+Printed: one JSON line with Mrays/s (every scene query: extension, NEE
+shadow and subsurface-walk queries) as `value`, pixel-samples/s, the
+roofline of the dominant kernel and a CPU baseline (the oracle on the
+host's cores over a bounded sample of the same frame).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "Mrays/s & spp/s at 1080p; preview p50/p99 latency at equal relMSE"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per unit: a node fetch tests two 32-B child boxes, a
+# triangle test reads 36 B of positions, each query reads its 32-B ray and
+# writes a 16-B hit.
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 64, 36, 48
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from denrimrendererkit_amd import synthetic
+    if name == "terrain_1080p":
+        return synthetic.layered_terrain_scene(), 1920, 1080, 8, {
+            "workload": "layered-terrain 1920x1080, 6,168,640 tris (1M fBm terrain + 256 x 20k displaced spheres), "
+                        "6 layered material kinds, 64 emissive tris, 2048x1024 HDRI, max depth 8",
+            "width": 1920, "height": 1080, "max_bounces": 8, "triangles": 6168640}
+    if name == "room_320x180":
+        return synthetic.cornell_room_scene(), 320, 180, 4, {
+            "workload": "room 320x180, 8,334 tris, Lambertian + GGX metal, 1 area light, max depth 4",
+            "width": 320, "height": 180, "max_bounces": 4, "triangles": 8334}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(build, camera, W, H, bounces, target_seconds):
+    """The oracle (C++ restatement, FP32, OpenMP over rows) on this host's
+    cores over a band of rows of the same frame at 1 spp."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    threads = os.cpu_count() or 1
+    osc = O.OracleScene(build)
+    cam = camera.gpu_camera(W, H)
+    rows = 2
+    spent = 0.0
+    while True:
+        y0 = H // 2 - rows // 2
+        st = O.make_settings(build, W, H, max_bounces=bounces, frame_seed=0, tile=(0, y0, W, rows))
+        t0 = time.perf_counter()
+        r = O.render(osc, st, cam, sample_count=1, threads=threads, with_aovs=False)
+        dt = time.perf_counter() - t0
+        spent += dt
+        if dt >= target_seconds * 0.5 or rows >= H or spent > target_seconds * 2:
+            break
+        rows = min(H, max(rows * 2, int(rows * target_seconds / max(dt, 1e-3))))
+    c = r.counters
+    queries = int(c[O.CNT_PRIMARY] + c[O.CNT_BOUNCE] + c[O.CNT_SHADOW])
+    return {"value": round(queries / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows x {W} px x 1 spp of the same frame ({queries} ray queries, {dt:.2f} s); "
+                      f"oracle FP32 restatement of the reference flat-BVH path tracer, OpenMP {threads} threads",
+            "spp_per_s": round(rows * W / dt, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="terrain_1080p")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from denrimrendererkit_amd import DenrimRenderer, RenderQuality, RenderSettings, compile_scene
+
+    t0 = time.perf_counter()
+    scene, W, H, bounces, config = workload(args.workload)
+    build = compile_scene(scene)
+    log(f"scene compiled: {len(build.triangles)} tris in {time.perf_counter() - t0:.1f} s")
+    renderer = DenrimRenderer(0)
+    t1 = time.perf_counter()
+    gscene = renderer.upload(build)
+    info = gscene.info()
+    log(f"uploaded + BVH: {info} in {time.perf_counter() - t1:.1f} s")
+    settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0)
+
+    # Counting pass: ray queries, node fetches and triangle tests per step.
+    csettings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
+                               collects_sdf_traversal_stats=True)
+    cs = renderer.make_session(gscene, csettings, camera=scene.camera, samples_per_launch=1)
+    cs.render(1)
+    c = cs.stats().astype(np.float64)
+    cs.close()
+    queries_per_step = c[0] + c[1] + c[2]
+    log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f}, shadow {c[2]:.0f}), "
+        f"node fetches {c[3]:.0f} + shadow {c[5]:.0f}, tri tests {c[4]:.0f} + shadow {c[6]:.0f}")
+
+    sess = renderer.make_session(gscene, settings, camera=scene.camera, samples_per_launch=1)
+    sess.encode_next_sample(args.warmup)
+    sess.synchronize()
+    sess.reset_stats()
+    sess.set_stage_timing(True)
+    sess.synchronize()
+    w0 = time.perf_counter()
+    sess.encode_next_sample(args.steps)
+    sess.synchronize()
+    wall = time.perf_counter() - w0
+    tm = sess.timing()
+    gpu_s = tm["total_ms"] / 1e3
+    sec = max(wall, gpu_s)
+    mrays = queries_per_step * args.steps / sec / 1e6
+    spp_s = W * H * args.steps / sec
+    beauty = sess.pixels()
+    assert np.isfinite(beauty).all(), "non-finite beauty"
+
+    # Roofline of the dominant kernel (largest share of stage time).
+    st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
+    dom = max(("extend", "shadow", "shade"), key=lambda k: st_ms[k])
+    if dom == "extend":
+        nodes, tris, rays = c[3], c[4], c[0] + c[1]
+    elif dom == "shadow":
+        nodes, tris, rays = c[5], c[6], c[2]
+    else:
+        nodes, tris, rays = c[3], c[4], c[0] + c[1]
+    launches_per_step = max(1, st_n[dom] // max(1, args.steps))
+    bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays) / launches_per_step
+    avg_ms = st_ms[dom] / max(1, st_n[dom])
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    roofline = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+                "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in st_ms.items()}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(build, scene.camera, W, H, bounces, args.cpu_seconds)
+
+    config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"],
+                   "ray_queries_per_step": int(queries_per_step), "rays_per_path": round(queries_per_step / (W * H), 3)})
+    out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "spp_per_s": round(spp_s, 1), "wall_s": round(wall, 4), "gpu_event_s": round(gpu_s, 4),
+           "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,109 @@
+"""Loader of the in-tree gfx950 library and ctypes bindings of its C ABI
+(include/denrim_hip.h). There is no fallback: if the library is missing the
+import of the renderer fails with an error that says how to build it."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdenrim_hip.so")
+
+DR_OK = 0
+STATUS_NAMES = {
+    1: "noDevice", 2: "missingShaderFunction", 3: "invalidScene", 4: "commandBufferFailed",
+    5: "pngExportFailed", 6: "invalidArgument", 7: "outOfMemory",
+}
+
+# Exported symbols declared in include/denrim_hip.h.
+EXPORTS = (
+    "dr_last_error", "dr_hip_device_create", "dr_hip_device_destroy", "dr_hip_scene_upload",
+    "dr_hip_scene_release", "dr_hip_scene_info", "dr_hip_session_create", "dr_hip_session_destroy",
+    "dr_hip_session_render", "dr_hip_session_encode", "dr_hip_session_synchronize",
+    "dr_hip_session_render_tile", "dr_hip_session_reset", "dr_hip_session_update_camera",
+    "dr_hip_session_sample_count", "dr_hip_session_read_output", "dr_hip_session_stats",
+    "dr_hip_session_reset_stats", "dr_hip_session_set_stage_timing", "dr_hip_session_timing",
+)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_void_p), ("triangle_count", C.c_uint32),
+        ("materials", C.c_void_p), ("material_count", C.c_uint32),
+        ("lights", C.c_void_p), ("light_count", C.c_uint32),
+        ("environment_samples", C.c_void_p), ("environment_sample_count", C.c_uint32),
+        ("texture_descriptors", C.c_void_p), ("texture_count", C.c_uint32),
+        ("texture_pixels", C.c_void_p), ("texture_pixel_count", C.c_uint64),
+        ("environment_texture_index_plus_one", C.c_uint32),
+        ("environment_intensity", C.c_float), ("environment_rotation_y", C.c_float),
+        ("environment_max_radiance", C.c_float), ("max_leaf_triangles", C.c_uint32),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("max_bounces", C.c_uint32), ("quality", C.c_uint32),
+        ("sample_radiance_clamp", C.c_float), ("transparent_background", C.c_uint32),
+        ("shows_environment_background", C.c_uint32), ("background_color", C.c_float * 3),
+        ("denoiser_enabled", C.c_uint32), ("collects_stats", C.c_uint32), ("frame_seed", C.c_uint32),
+        ("samples_per_launch", C.c_uint32),
+    ]
+
+
+class GpuCamera(C.Structure):
+    _fields_ = [("origin", C.c_float * 4), ("lower_left", C.c_float * 4), ("horizontal", C.c_float * 4),
+                ("vertical", C.c_float * 4), ("lens", C.c_float * 4)]
+
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libdenrim_hip.so (built by __graft_entry__.build() or
+    `python -m denrimrendererkit_amd._build`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built; run `python -m denrimrendererkit_amd._build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    L.dr_last_error.restype = C.c_char_p
+    L.dr_hip_device_create.argtypes = [i32, C.POINTER(vp)]
+    L.dr_hip_device_destroy.argtypes = [vp]
+    L.dr_hip_scene_upload.argtypes = [vp, C.POINTER(SceneDesc), C.POINTER(vp)]
+    L.dr_hip_scene_release.argtypes = [vp]
+    L.dr_hip_scene_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double)]
+    L.dr_hip_session_create.argtypes = [vp, C.POINTER(Settings), C.POINTER(GpuCamera), C.POINTER(GpuCamera),
+                                        C.POINTER(vp)]
+    L.dr_hip_session_destroy.argtypes = [vp]
+    L.dr_hip_session_render.argtypes = [vp, u32]
+    L.dr_hip_session_encode.argtypes = [vp, u32]
+    L.dr_hip_session_synchronize.argtypes = [vp]
+    L.dr_hip_session_render_tile.argtypes = [vp, u32, u32, u32, u32, i32]
+    L.dr_hip_session_reset.argtypes = [vp]
+    L.dr_hip_session_update_camera.argtypes = [vp, C.POINTER(GpuCamera), C.POINTER(GpuCamera)]
+    L.dr_hip_session_sample_count.argtypes = [vp, C.POINTER(u32)]
+    L.dr_hip_session_read_output.argtypes = [vp, i32, vp, C.c_size_t]
+    L.dr_hip_session_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.c_size_t]
+    L.dr_hip_session_reset_stats.argtypes = [vp]
+    L.dr_hip_session_set_stage_timing.argtypes = [vp, i32]
+    L.dr_hip_session_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_uint64), C.c_size_t]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name != "dr_last_error":
+            fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(status: int, what: str):
+    if status != DR_OK:
+        from .scene import DenrimRendererError
+        msg = load().dr_last_error().decode(errors="replace")
+        raise DenrimRendererError(STATUS_NAMES.get(status, f"status{status}"), f"{what}: {msg}")

@@ -1,0 +1,574 @@
+// dr_api.cpp -- C ABI of the HIP backend (include/denrim_hip.h).
+//
+// Owns the device, the device-resident scene (caller's records plus the
+// traversal BVH) and sessions (path-state buffers, output planes, sample
+// count). Session semantics follow RenderSession (Renderer/RenderSession.swift):
+// samples accumulate as a running mean, resetAccumulation only zeroes the
+// sample count (the next sample has weight 1), tiles advance the count only
+// when they complete a sample.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/denrim_hip.h"
+#include "dr_bvh.h"
+#include "dr_kernels.h"
+#include "dr_scene.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+dr_status fail(dr_status s, const std::string &msg) {
+    g_error = msg;
+    return s;
+}
+
+#define DR_HIP(call, what)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) {                                                                              \
+            dr_status s_ = e_ == hipErrorOutOfMemory ? DR_ERR_OUT_OF_MEMORY : DR_ERR_COMMAND_BUFFER_FAILED; \
+            return fail(s_, std::string(what) + ": " + hipGetErrorString(e_));                               \
+        }                                                                                                    \
+    } while (0)
+
+template <class T> hipError_t devAlloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <class T> hipError_t upload(T **p, const void *src, size_t count) {
+    hipError_t e = devAlloc(p, count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+void devFree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+DrCamera toCam(const dr_gpu_camera &c) {
+    DrCamera d;
+    d.origin = make_float4(c.origin[0], c.origin[1], c.origin[2], c.origin[3]);
+    d.lowerLeft = make_float4(c.lower_left[0], c.lower_left[1], c.lower_left[2], c.lower_left[3]);
+    d.horizontal = make_float4(c.horizontal[0], c.horizontal[1], c.horizontal[2], c.horizontal[3]);
+    d.vertical = make_float4(c.vertical[0], c.vertical[1], c.vertical[2], c.vertical[3]);
+    d.lens = make_float4(c.lens[0], c.lens[1], c.lens[2], c.lens[3]);
+    return d;
+}
+
+}  // namespace
+
+struct dr_device {
+    int ordinal = 0;
+    int cuCount = 0;
+    std::string arch;
+};
+
+struct dr_scene {
+    dr_device *device = nullptr;
+    DrSceneDev dev{};
+    DrNode *nodes = nullptr;
+    float4 *triPos = nullptr;
+    DrTriangle *triangles = nullptr;
+    DrMaterial *materials = nullptr;
+    DrLight *lights = nullptr;
+    float2 *env = nullptr;
+    uint4 *texDesc = nullptr;
+    float4 *texels = nullptr;
+    dr::BvhBuildStats stats;
+    int sessions = 0;
+    bool released = false;
+};
+
+struct dr_session {
+    dr_scene *scene = nullptr;
+    dr_settings settings{};
+    DrCamera camera{}, previousCamera{};
+    uint32_t sampleCount = 0;
+    uint32_t samplesPerLaunch = 1;
+    uint32_t nSlots = 0;
+    uint32_t traceGrid = 0;
+    hipStream_t stream = nullptr;
+    DrPaths paths{};
+    DrOutputs out{};
+    void *pathBlock = nullptr;   // one allocation for all path-state arrays
+    void *outBlock = nullptr;    // one allocation for all output planes
+    hipEvent_t evBegin = nullptr, evEnd = nullptr;
+    std::vector<hipEvent_t> stageEvents;
+    bool stageTiming = false;
+    double stageMs[DR_STAGE_COUNT] = {0};
+    uint64_t stageLaunches[DR_STAGE_COUNT] = {0};
+    double lastMs = 0.0;
+};
+
+extern "C" {
+
+const char *dr_last_error(void) { return g_error.c_str(); }
+
+dr_status dr_hip_device_create(int ordinal, dr_device **out) {
+    if (!out) return fail(DR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= ordinal || ordinal < 0)
+        return fail(DR_ERR_NO_DEVICE, "no HIP device with ordinal " + std::to_string(ordinal));
+    DR_HIP(hipSetDevice(ordinal), "hipSetDevice");
+    hipDeviceProp_t prop;
+    DR_HIP(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0)
+        return fail(DR_ERR_MISSING_SHADER_FUNCTION, "kernels are built for gfx950, device is " + arch);
+    auto *d = new dr_device;
+    d->ordinal = ordinal;
+    d->cuCount = prop.multiProcessorCount;
+    d->arch = arch;
+    *out = d;
+    return DR_OK;
+}
+
+dr_status dr_hip_device_destroy(dr_device *device) {
+    delete device;
+    return DR_OK;
+}
+
+static void freeScene(dr_scene *s) {
+    devFree(s->nodes);
+    devFree(s->triPos);
+    devFree(s->triangles);
+    devFree(s->materials);
+    devFree(s->lights);
+    devFree(s->env);
+    devFree(s->texDesc);
+    devFree(s->texels);
+}
+
+dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scene **out) {
+    if (!device || !d || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    // RenderSession.init checks (Renderer/RenderSession.swift:322-327).
+    if (d->material_count == 0 || !d->materials)
+        return fail(DR_ERR_INVALID_SCENE, "Scene must contain at least one material.");
+    if (d->triangle_count == 0 || !d->triangles) return fail(DR_ERR_INVALID_SCENE, "Scene must contain geometry.");
+    if (d->light_count && !d->lights) return fail(DR_ERR_INVALID_ARGUMENT, "lights is NULL");
+    if (d->environment_sample_count && !d->environment_samples)
+        return fail(DR_ERR_INVALID_ARGUMENT, "environment_samples is NULL");
+    if (d->texture_count && (!d->texture_descriptors || !d->texture_pixels))
+        return fail(DR_ERR_INVALID_ARGUMENT, "texture arrays are NULL");
+    const auto *tris = static_cast<const DrTriangle *>(d->triangles);
+    const auto *texd = static_cast<const uint4 *>(d->texture_descriptors);
+    for (uint32_t i = 0; i < d->texture_count; ++i) {
+        uint64_t end = uint64_t(texd[i].x) + uint64_t(texd[i].y) * texd[i].z;
+        if (end > d->texture_pixel_count) return fail(DR_ERR_INVALID_SCENE, "texture descriptor exceeds texel array");
+    }
+    const auto *lights = static_cast<const DrLight *>(d->lights);
+    for (uint32_t i = 0; i < d->light_count; ++i)
+        if (lights[i].triangleIndex >= d->triangle_count)
+            return fail(DR_ERR_INVALID_SCENE, "light record references an unknown triangle");
+    DR_HIP(hipSetDevice(device->ordinal), "hipSetDevice");
+
+    auto *s = new dr_scene;
+    s->device = device;
+    std::vector<DrNode> nodes;
+    std::vector<float4> triPos;
+    dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos, &s->stats);
+
+    bool opaqueOnly = true;
+    const auto *mats = static_cast<const DrMaterial *>(d->materials);
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        const DrMaterial &m = mats[i];
+        // A shadow ray passes a surface with opacity <= 0.001 or
+        // transmission > 0.001 (PathTrace.metal:4249-4258); base-colour
+        // textures can lower opacity too.
+        if (m.s[M_BASE].w <= 0.001f || std::min(std::max(m.s[M_EMISSION].w, 0.f), 1.f) > 0.001f || m.s[M_PARAMS].z != 0.f)
+            opaqueOnly = false;
+    }
+
+    hipError_t e = hipSuccess;
+    auto chain = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    chain(upload(&s->nodes, nodes.data(), nodes.size()));
+    chain(upload(&s->triPos, triPos.data(), triPos.size()));
+    chain(upload(&s->triangles, d->triangles, d->triangle_count));
+    chain(upload(&s->materials, d->materials, d->material_count));
+    chain(upload(&s->lights, d->lights, d->light_count));
+    chain(upload(&s->env, d->environment_samples, d->environment_sample_count));
+    chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count));
+    chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count));
+    if (e != hipSuccess) {
+        freeScene(s);
+        delete s;
+        return fail(e == hipErrorOutOfMemory ? DR_ERR_OUT_OF_MEMORY : DR_ERR_COMMAND_BUFFER_FAILED,
+                    std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    DrSceneDev &g = s->dev;
+    g.nodes = s->nodes;
+    g.triPos = s->triPos;
+    g.triangles = s->triangles;
+    g.materials = s->materials;
+    g.lights = s->lights;
+    g.env = s->env;
+    g.texDesc = s->texDesc;
+    g.texels = s->texels;
+    g.triangleCount = d->triangle_count;
+    g.materialCount = d->material_count;
+    g.lightCount = d->light_count;
+    g.envCount = d->environment_sample_count;
+    g.texCount = d->texture_count;
+    g.envTexIndexPlusOne = d->environment_texture_index_plus_one;
+    g.envIntensity = d->environment_intensity;
+    g.envRotationY = d->environment_rotation_y;
+    g.envMaxRadiance = d->environment_max_radiance;
+    g.opaqueOnly = opaqueOnly ? 1u : 0u;
+    *out = s;
+    return DR_OK;
+}
+
+dr_status dr_hip_scene_release(dr_scene *scene) {
+    if (!scene) return DR_OK;
+    if (scene->sessions > 0) {
+        scene->released = true;  // freed with its last session
+        return DR_OK;
+    }
+    freeScene(scene);
+    delete scene;
+    return DR_OK;
+}
+
+dr_status dr_hip_scene_info(const dr_scene *s, uint32_t *nodes, uint32_t *leaves, uint32_t *depth, double *secs) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (nodes) *nodes = s->stats.nodeCount;
+    if (leaves) *leaves = s->stats.leafCount;
+    if (depth) *depth = s->stats.maxDepth;
+    if (secs) *secs = s->stats.buildSeconds;
+    return DR_OK;
+}
+
+static void freeSession(dr_session *s) {
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    devFree(s->pathBlock);
+    devFree(s->outBlock);
+    if (s->evBegin) (void)hipEventDestroy(s->evBegin);
+    if (s->evEnd) (void)hipEventDestroy(s->evEnd);
+    for (auto ev : s->stageEvents) (void)hipEventDestroy(ev);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr_gpu_camera *camera,
+                                const dr_gpu_camera *prev, dr_session **out) {
+    if (!scene || !st || !camera || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (st->width == 0 || st->height == 0)  // RenderSession.swift:304-306
+        return fail(DR_ERR_INVALID_SCENE, "Render size must be greater than zero.");
+    if (st->quality != 2)
+        return fail(DR_ERR_MISSING_SHADER_FUNCTION, "the hip backend serves the final (path-traced) quality only");
+    if (st->max_bounces > DR_MAX_BOUNCES) return fail(DR_ERR_INVALID_ARGUMENT, "max_bounces exceeds 64");
+    DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
+    auto *s = new dr_session;
+    s->scene = scene;
+    s->settings = *st;
+    s->camera = toCam(*camera);
+    s->previousCamera = prev ? toCam(*prev) : s->camera;
+    const uint64_t pixels = uint64_t(st->width) * st->height;
+    uint32_t k = st->samples_per_launch;
+    if (k == 0) k = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (2u << 20) / std::max<uint64_t>(pixels, 1))));
+    s->samplesPerLaunch = k;
+    if (pixels * k > 0xFFFFFFFFull / 3) {
+        delete s;
+        return fail(DR_ERR_INVALID_ARGUMENT, "image too large");
+    }
+    s->nSlots = uint32_t(pixels * k);
+    s->traceGrid = uint32_t(std::max(1, scene->device->cuCount)) * 16u;  // 16 x 128 lanes per CU
+    const uint32_t traceLanes = s->traceGrid * DR_TRACE_BLOCK;
+    const uint32_t stride = std::max(s->nSlots, traceLanes);
+    const size_t N = s->nSlots;
+
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
+    if (e == hipSuccess) e = hipEventCreate(&s->evEnd);
+    // Path state: 7 float4 arrays of N, 4 float4 arrays of 3N, 3 u32 queues
+    // (2 x N, 3N), counters, stats, stack overflow.
+    size_t bytes = 0;
+    auto take = [&](size_t n) {
+        size_t off = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return off;
+    };
+    size_t oRayO = take(N * 16), oRayD = take(N * 16), oThr = take(N * 16), oRad = take(N * 16), oMeta = take(N * 16),
+           oHit = take(N * 16), oNeeT = take(N * 16);
+    size_t oShO = take(3 * N * 16), oShD = take(3 * N * 16), oShC = take(3 * N * 16), oShR = take(3 * N * 16);
+    size_t oQ0 = take(N * 4), oQ1 = take(N * 4), oSQ = take(3 * N * 4);
+    size_t oCtr = take(DR_CTR_WORDS * 4), oStats = take(DR_CNT_SLOTS * 8);
+    size_t oOvf = take(size_t(stride) * (DR_MAX_DEPTH - DR_LDS_STACK) * 4);
+    if (e == hipSuccess) e = hipMalloc(&s->pathBlock, bytes);
+    if (e == hipSuccess) e = hipMemset(s->pathBlock, 0, bytes);
+    const size_t plane = pixels * 16;
+    if (e == hipSuccess) e = hipMalloc(&s->outBlock, plane * 8);
+    if (e == hipSuccess) e = hipMemset(s->outBlock, 0, plane * 8);
+    if (e != hipSuccess) {
+        freeSession(s);
+        delete s;
+        return fail(e == hipErrorOutOfMemory ? DR_ERR_OUT_OF_MEMORY : DR_ERR_COMMAND_BUFFER_FAILED,
+                    std::string("session buffers: ") + hipGetErrorString(e));
+    }
+    char *b = static_cast<char *>(s->pathBlock);
+    DrPaths &P = s->paths;
+    P.rayO = reinterpret_cast<float4 *>(b + oRayO);
+    P.rayD = reinterpret_cast<float4 *>(b + oRayD);
+    P.thr = reinterpret_cast<float4 *>(b + oThr);
+    P.rad = reinterpret_cast<float4 *>(b + oRad);
+    P.meta = reinterpret_cast<uint4 *>(b + oMeta);
+    P.hit = reinterpret_cast<float4 *>(b + oHit);
+    P.neeT = reinterpret_cast<float4 *>(b + oNeeT);
+    P.shO = reinterpret_cast<float4 *>(b + oShO);
+    P.shD = reinterpret_cast<float4 *>(b + oShD);
+    P.shC = reinterpret_cast<float4 *>(b + oShC);
+    P.shR = reinterpret_cast<float4 *>(b + oShR);
+    P.queue[0] = reinterpret_cast<uint32_t *>(b + oQ0);
+    P.queue[1] = reinterpret_cast<uint32_t *>(b + oQ1);
+    P.shadowQueue = reinterpret_cast<uint32_t *>(b + oSQ);
+    P.counters = reinterpret_cast<uint32_t *>(b + oCtr);
+    P.stats = reinterpret_cast<unsigned long long *>(b + oStats);
+    P.overflow = reinterpret_cast<uint32_t *>(b + oOvf);
+    P.overflowStride = stride;
+    char *o = static_cast<char *>(s->outBlock);
+    DrOutputs &O = s->out;
+    O.accumulation = reinterpret_cast<float4 *>(o + 0 * plane);
+    O.depth = reinterpret_cast<float4 *>(o + 1 * plane);
+    O.normal = reinterpret_cast<float4 *>(o + 2 * plane);
+    O.albedo = reinterpret_cast<float4 *>(o + 3 * plane);
+    O.materialID = reinterpret_cast<float4 *>(o + 4 * plane);
+    O.objectID = reinterpret_cast<float4 *>(o + 5 * plane);
+    O.motion = reinterpret_cast<float4 *>(o + 6 * plane);
+    O.triangleID = reinterpret_cast<float4 *>(o + 7 * plane);
+    scene->sessions++;
+    *out = s;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_destroy(dr_session *s) {
+    if (!s) return DR_OK;
+    freeSession(s);
+    dr_scene *sc = s->scene;
+    delete s;
+    if (sc && --sc->sessions == 0 && sc->released) {
+        freeScene(sc);
+        delete sc;
+    }
+    return DR_OK;
+}
+
+static DrLaunch makeLaunch(const dr_session *s, uint32_t sampleBegin, uint32_t samples, uint32_t tx, uint32_t ty,
+                           uint32_t tw, uint32_t th) {
+    const dr_settings &st = s->settings;
+    DrLaunch L{};
+    L.width = st.width;
+    L.height = st.height;
+    L.maxBounces = std::max(1u, st.max_bounces);
+    L.frameSeed = st.frame_seed;
+    L.transparentBackground = st.transparent_background ? 1u : 0u;
+    L.showsEnvironmentBackground = st.shows_environment_background ? 1u : 0u;
+    L.denoiserEnabled = st.denoiser_enabled ? 1u : 0u;
+    L.sampleRadianceClamp = st.sample_radiance_clamp < 0.f ? 64.f : st.sample_radiance_clamp;
+    L.backgroundColor[0] = st.background_color[0];
+    L.backgroundColor[1] = st.background_color[1];
+    L.backgroundColor[2] = st.background_color[2];
+    L.backgroundColor[3] = 1.f;
+    L.tileX = tx;
+    L.tileY = ty;
+    L.tileWidth = tw;
+    L.tileHeight = th;
+    L.sampleBegin = sampleBegin;
+    L.samplesInBatch = samples;
+    L.statsEnabled = st.collects_stats ? 1u : 0u;
+    return L;
+}
+
+static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
+    const uint32_t nSlots = L.tileWidth * L.tileHeight * L.samplesInBatch;
+    auto run = [&](DrStage stage, uint32_t b) -> hipError_t {
+        size_t ev = s->stageEvents.size();
+        if (s->stageTiming) {
+            hipEvent_t a, z;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&z) != hipSuccess) return hipErrorOutOfMemory;
+            s->stageEvents.push_back(a);
+            s->stageEvents.push_back(z);
+            (void)hipEventRecord(a, s->stream);
+        }
+        hipError_t e = dr::launchStage(stage, b, s->scene->dev, L, s->camera, s->previousCamera, s->paths, s->out,
+                                       nSlots, s->traceGrid, s->stream);
+        if (s->stageTiming) (void)hipEventRecord(s->stageEvents[ev + 1], s->stream);
+        s->stageLaunches[stage]++;
+        return e;
+    };
+    DR_HIP(run(DR_STAGE_RAYGEN, 0), "raygen launch");
+    for (uint32_t b = 0; b < L.maxBounces; ++b) {
+        DR_HIP(run(DR_STAGE_EXTEND, b), "extend launch");
+        DR_HIP(run(DR_STAGE_SHADE, b), "shade launch");
+        DR_HIP(run(DR_STAGE_SHADOW, b), "shadow launch");
+        DR_HIP(run(DR_STAGE_RESOLVE, b), "resolve launch");
+    }
+    DR_HIP(run(DR_STAGE_FINALIZE, 0), "finalize launch");
+    return DR_OK;
+}
+
+// Stage order of events pushed by enqueueBatch, to attribute timings.
+static void collectStageTimes(dr_session *s) {
+    if (!s->stageTiming || s->stageEvents.empty()) return;
+    (void)hipStreamSynchronize(s->stream);
+    // Events come in (begin, end) pairs in launch order; replay the order.
+    size_t i = 0;
+    const uint32_t mb = std::max(1u, s->settings.max_bounces);
+    while (i + 1 < s->stageEvents.size()) {
+        auto add = [&](DrStage st) {
+            float ms = 0.f;
+            if (i + 1 < s->stageEvents.size() && hipEventElapsedTime(&ms, s->stageEvents[i], s->stageEvents[i + 1]) == hipSuccess)
+                s->stageMs[st] += ms;
+            i += 2;
+        };
+        add(DR_STAGE_RAYGEN);
+        for (uint32_t b = 0; b < mb; ++b) {
+            add(DR_STAGE_EXTEND);
+            add(DR_STAGE_SHADE);
+            add(DR_STAGE_SHADOW);
+            add(DR_STAGE_RESOLVE);
+        }
+        add(DR_STAGE_FINALIZE);
+    }
+    for (auto ev : s->stageEvents) (void)hipEventDestroy(ev);
+    s->stageEvents.clear();
+}
+
+static dr_status encodeSamples(dr_session *s, uint32_t samples) {
+    DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
+    DR_HIP(hipEventRecord(s->evBegin, s->stream), "event");
+    uint32_t left = samples;
+    while (left > 0) {
+        uint32_t k = std::min(left, s->samplesPerLaunch);
+        DrLaunch L = makeLaunch(s, s->sampleCount, k, 0, 0, s->settings.width, s->settings.height);
+        dr_status r = enqueueBatch(s, L);
+        if (r != DR_OK) return r;
+        s->sampleCount += k;
+        left -= k;
+    }
+    DR_HIP(hipEventRecord(s->evEnd, s->stream), "event");
+    return DR_OK;
+}
+
+dr_status dr_hip_session_encode(dr_session *s, uint32_t samples) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    return encodeSamples(s, samples);
+}
+
+dr_status dr_hip_session_synchronize(dr_session *s) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->evBegin, s->evEnd) == hipSuccess) s->lastMs = ms;
+    collectStageTimes(s);
+    return DR_OK;
+}
+
+dr_status dr_hip_session_render(dr_session *s, uint32_t samples) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    uint32_t before = s->sampleCount;
+    dr_status r = encodeSamples(s, samples);
+    if (r == DR_OK) r = dr_hip_session_synchronize(s);
+    if (r != DR_OK) s->sampleCount = before;  // RenderSession.swift:844-847
+    return r;
+}
+
+dr_status dr_hip_session_render_tile(dr_session *s, uint32_t x, uint32_t y, uint32_t w, uint32_t h, int completes) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    // Clamp like RenderSession.encodeNextSample (RenderSession.swift:900-907).
+    const uint32_t W = s->settings.width, H = s->settings.height;
+    uint32_t tx = std::min(x, W), ty = std::min(y, H);
+    uint32_t tw = std::min(w, W - tx), th = std::min(h, H - ty);
+    if (tw == 0 || th == 0) return DR_OK;
+    DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
+    DR_HIP(hipEventRecord(s->evBegin, s->stream), "event");
+    DrLaunch L = makeLaunch(s, s->sampleCount, 1, tx, ty, tw, th);
+    dr_status r = enqueueBatch(s, L);
+    if (r != DR_OK) return r;
+    DR_HIP(hipEventRecord(s->evEnd, s->stream), "event");
+    r = dr_hip_session_synchronize(s);
+    if (r == DR_OK && completes) s->sampleCount += 1;
+    return r;
+}
+
+dr_status dr_hip_session_reset(dr_session *s) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    s->sampleCount = 0;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_update_camera(dr_session *s, const dr_gpu_camera *camera, const dr_gpu_camera *prev) {
+    if (!s || !camera) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    DrCamera old = s->camera;
+    s->camera = toCam(*camera);
+    s->previousCamera = prev ? toCam(*prev) : old;
+    s->sampleCount = 0;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_sample_count(const dr_session *s, uint32_t *out) {
+    if (!s || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = s->sampleCount;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_read_output(dr_session *s, dr_output kind, float *rgba, size_t floats) {
+    if (!s || !rgba) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t pixels = size_t(s->settings.width) * s->settings.height;
+    if (floats < pixels * 4) return fail(DR_ERR_INVALID_ARGUMENT, "output buffer too small");
+    if (int(kind) < 0 || int(kind) > 7) return fail(DR_ERR_INVALID_ARGUMENT, "unknown output kind");
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    const float4 *planes[8] = {s->out.accumulation, s->out.depth, s->out.normal, s->out.albedo,
+                               s->out.materialID, s->out.objectID, s->out.motion, s->out.triangleID};
+    DR_HIP(hipMemcpy(rgba, planes[int(kind)], pixels * 16, hipMemcpyDeviceToHost), "readback");
+    return DR_OK;
+}
+
+dr_status dr_hip_session_stats(dr_session *s, uint64_t *counters, size_t count) {
+    if (!s || !counters) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    unsigned long long tmp[DR_CNT_SLOTS];
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    DR_HIP(hipMemcpy(tmp, s->paths.stats, sizeof(tmp), hipMemcpyDeviceToHost), "stats readback");
+    for (size_t i = 0; i < count && i < DR_CNT_SLOTS; ++i) counters[i] = tmp[i];
+    return DR_OK;
+}
+
+dr_status dr_hip_session_reset_stats(dr_session *s) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    DR_HIP(hipMemset(s->paths.stats, 0, DR_CNT_SLOTS * 8), "stats reset");
+    for (int i = 0; i < DR_STAGE_COUNT; ++i) {
+        s->stageMs[i] = 0.0;
+        s->stageLaunches[i] = 0;
+    }
+    return DR_OK;
+}
+
+dr_status dr_hip_session_set_stage_timing(dr_session *s, int enabled) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    s->stageTiming = enabled != 0;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_timing(dr_session *s, double *total_ms, double *stage_ms, uint64_t *launches, size_t stages) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (total_ms) *total_ms = s->lastMs;
+    for (size_t i = 0; i < stages && i < DR_STAGE_COUNT; ++i) {
+        if (stage_ms) stage_ms[i] = s->stageMs[i];
+        if (launches) launches[i] = s->stageLaunches[i];
+    }
+    return DR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+// dr_bvh.cpp -- binned-SAH BVH build and conversion to the traversal layout.
+//
+// The reference builds a count-median BVH with 48-B nodes and a separate
+// index array (Acceleration/BVHBuilder.swift:4-135, FlatBVH.swift:13-33).
+// Closest-hit results with the lowest-index tie-break do not depend on the
+// tree, so this backend builds its own: binned SAH (32 bins, 3 axes), leaves
+// of at most `maxLeaf` triangles, written as two-box nodes in depth-first
+// order with leaf triangles stored contiguously in leaf order.
+#include "dr_bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+
+namespace dr {
+namespace {
+
+constexpr int kBins = 32;
+constexpr float kNodeCost = 1.0f;
+constexpr float kTriCost = 1.0f;
+
+struct Box {
+    float mn[3], mx[3];
+    void reset() {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::numeric_limits<float>::infinity();
+            mx[a] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box &b) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], b.mn[a]);
+            mx[a] = std::max(mx[a], b.mx[a]);
+        }
+    }
+    void grow(const float *p) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], p[a]);
+            mx[a] = std::max(mx[a], p[a]);
+        }
+    }
+    float area() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        if (!(dx >= 0.f) || !(dy >= 0.f) || !(dz >= 0.f)) return 0.f;
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Prim {
+    Box box;
+    float c[3];
+};
+
+struct TNode {
+    Box box;
+    int32_t left = -1, right = -1;
+    uint32_t first = 0, count = 0;
+};
+
+struct Builder {
+    const std::vector<Prim> &prims;
+    std::vector<uint32_t> &idx;
+    std::vector<TNode> &nodes;
+    std::atomic<uint32_t> next{0};
+    uint32_t maxLeaf;
+    std::atomic<uint32_t> maxDepth{0};
+
+    Builder(const std::vector<Prim> &p, std::vector<uint32_t> &i, std::vector<TNode> &n, uint32_t ml)
+        : prims(p), idx(i), nodes(n), maxLeaf(ml) {}
+
+    uint32_t alloc() { return next.fetch_add(1); }
+
+    void build(uint32_t nodeIndex, uint32_t lo, uint32_t hi, uint32_t depth) {
+        uint32_t d = maxDepth.load();
+        while (depth > d && !maxDepth.compare_exchange_weak(d, depth)) {
+        }
+        TNode &node = nodes[nodeIndex];
+        Box bounds, cb;
+        bounds.reset();
+        cb.reset();
+        for (uint32_t i = lo; i < hi; ++i) {
+            const Prim &p = prims[idx[i]];
+            bounds.grow(p.box);
+            cb.grow(p.c);
+        }
+        node.box = bounds;
+        const uint32_t count = hi - lo;
+        auto makeLeaf = [&]() {
+            node.first = lo;
+            node.count = count;
+            node.left = node.right = -1;
+        };
+        if (count <= 1 || depth + 2 >= DR_MAX_DEPTH) {
+            makeLeaf();
+            return;
+        }
+        // Binned SAH over centroid bounds.
+        int bestAxis = -1;
+        int bestSplit = -1;
+        float bestCost = std::numeric_limits<float>::infinity();
+        for (int a = 0; a < 3; ++a) {
+            float ext = cb.mx[a] - cb.mn[a];
+            if (!(ext > 0.f)) continue;
+            float scale = float(kBins) / ext;
+            Box bb[kBins];
+            uint32_t bc[kBins] = {0};
+            for (int b = 0; b < kBins; ++b) bb[b].reset();
+            for (uint32_t i = lo; i < hi; ++i) {
+                const Prim &p = prims[idx[i]];
+                int b = std::min(kBins - 1, int((p.c[a] - cb.mn[a]) * scale));
+                bc[b]++;
+                bb[b].grow(p.box);
+            }
+            float rightArea[kBins];
+            uint32_t rightCount[kBins];
+            Box acc;
+            acc.reset();
+            uint32_t n = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                acc.grow(bb[b]);
+                n += bc[b];
+                rightArea[b] = acc.area();
+                rightCount[b] = n;
+            }
+            acc.reset();
+            n = 0;
+            for (int b = 0; b < kBins - 1; ++b) {
+                acc.grow(bb[b]);
+                n += bc[b];
+                if (n == 0 || rightCount[b + 1] == 0) continue;
+                float cost = acc.area() * float(n) + rightArea[b + 1] * float(rightCount[b + 1]);
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = a;
+                    bestSplit = b;
+                }
+            }
+        }
+        float parentArea = std::max(bounds.area(), 1e-30f);
+        float leafCost = kTriCost * float(count);
+        float splitCost = kNodeCost + kTriCost * bestCost / parentArea;
+        if (count <= maxLeaf && (bestAxis < 0 || leafCost <= splitCost)) {
+            makeLeaf();
+            return;
+        }
+        uint32_t mid;
+        if (bestAxis < 0) {
+            mid = lo + count / 2;  // all centroids coincide: split by position in the list
+        } else {
+            float ext = cb.mx[bestAxis] - cb.mn[bestAxis];
+            float scale = float(kBins) / ext;
+            auto it = std::partition(idx.begin() + lo, idx.begin() + hi, [&](uint32_t k) {
+                const Prim &p = prims[k];
+                int b = std::min(kBins - 1, int((p.c[bestAxis] - cb.mn[bestAxis]) * scale));
+                return b <= bestSplit;
+            });
+            mid = uint32_t(it - idx.begin());
+            if (mid == lo || mid == hi) mid = lo + count / 2;
+        }
+        uint32_t l = alloc(), r = alloc();
+        node.left = int32_t(l);
+        node.right = int32_t(r);
+        node.count = 0;
+        if (count > 65536 && depth < 6) {
+            auto f = std::async(std::launch::async, [=]() { build(l, lo, mid, depth + 1); });
+            build(r, mid, hi, depth + 1);
+            f.get();
+        } else {
+            build(l, lo, mid, depth + 1);
+            build(r, mid, hi, depth + 1);
+        }
+    }
+};
+
+float widenDown(float v, float margin) {
+    double w = double(v) - double(margin);
+    float f = float(w);
+    if (double(f) > w) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+    return std::nextafter(f, -std::numeric_limits<float>::infinity());
+}
+float widenUp(float v, float margin) {
+    double w = double(v) + double(margin);
+    float f = float(w);
+    if (double(f) < w) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return std::nextafter(f, std::numeric_limits<float>::infinity());
+}
+
+void childBox(const Box &b, float out[6]) {
+    float ext = std::max(b.mx[0] - b.mn[0], std::max(b.mx[1] - b.mn[1], b.mx[2] - b.mn[2]));
+    for (int a = 0; a < 3; ++a) {
+        float mag = std::max(std::fabs(b.mn[a]), std::fabs(b.mx[a]));
+        float margin = 1e-6f * (mag + ext) + 1e-30f;
+        out[a] = widenDown(b.mn[a], margin);
+        out[3 + a] = widenUp(b.mx[a], margin);
+    }
+}
+
+struct Flattener {
+    const std::vector<TNode> &tn;
+    const std::vector<uint32_t> &idx;
+    const DrTriangle *tris;
+    std::vector<DrNode> &out;
+    std::vector<float4> &pos;
+    uint32_t leaves = 0;
+    double sah = 0.0;
+    double rootArea = 1.0;
+
+    void emitLeafTris(const TNode &n, uint32_t &first) {
+        first = uint32_t(pos.size() / 3);
+        for (uint32_t k = 0; k < n.count; ++k) {
+            uint32_t orig = idx[n.first + k];
+            const DrTriangle &t = tris[orig];
+            float4 v0 = t.v0, v1 = t.v1, v2 = t.v2;
+            uint32_t bits = orig;
+            float fb;
+            std::memcpy(&fb, &bits, 4);
+            pos.push_back(make_float4(v0.x, v0.y, v0.z, fb));
+            pos.push_back(make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.f));
+            pos.push_back(make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.f));
+        }
+        leaves++;
+        sah += double(n.box.area()) / rootArea * kTriCost * n.count;
+    }
+
+    // Returns the index of the emitted node.
+    uint32_t emit(int32_t ni) {
+        const TNode &n = tn[ni];
+        uint32_t me = uint32_t(out.size());
+        out.push_back(DrNode{});
+        sah += double(n.box.area()) / rootArea * kNodeCost;
+        int32_t ch[2] = {n.left, n.right};
+        uint32_t ref[2], cnt[2];
+        float bx[2][6];
+        for (int c = 0; c < 2; ++c) {
+            const TNode &cn = tn[ch[c]];
+            childBox(cn.box, bx[c]);
+            if (cn.count > 0) {
+                emitLeafTris(cn, ref[c]);
+                cnt[c] = cn.count;
+            } else {
+                ref[c] = emit(ch[c]);
+                cnt[c] = 0;
+            }
+        }
+        DrNode &o = out[me];
+        o.a = make_float4(bx[0][0], bx[0][1], bx[0][2], bx[0][3]);
+        o.b = make_float4(bx[0][4], bx[0][5], bx[1][0], bx[1][1]);
+        o.c = make_float4(bx[1][2], bx[1][3], bx[1][4], bx[1][5]);
+        o.d = make_uint4(ref[0], ref[1], cnt[0], cnt[1]);
+        return me;
+    }
+};
+
+}  // namespace
+
+void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode> &nodes,
+              std::vector<float4> &triPos, BvhBuildStats *stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    nodes.clear();
+    triPos.clear();
+    if (count == 0) {
+        if (stats) *stats = BvhBuildStats{};
+        return;
+    }
+    std::vector<Prim> prims(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const DrTriangle &t = tris[i];
+        const float *v[3] = {&t.v0.x, &t.v1.x, &t.v2.x};
+        Prim &p = prims[i];
+        p.box.reset();
+        for (int k = 0; k < 3; ++k) p.box.grow(v[k]);
+        for (int a = 0; a < 3; ++a) p.c[a] = 0.5f * (p.box.mn[a] + p.box.mx[a]);
+    }
+    std::vector<uint32_t> idx(count);
+    for (uint32_t i = 0; i < count; ++i) idx[i] = i;
+    std::vector<TNode> tn(size_t(2) * count + 1);
+    Builder b(prims, idx, tn, std::max(1u, maxLeaf));
+    uint32_t root = b.alloc();
+    b.build(root, 0, count, 0);
+
+    triPos.reserve(size_t(count) * 3);
+    nodes.reserve(count);
+    Flattener f{tn, idx, tris, nodes, triPos};
+    f.rootArea = std::max(double(tn[root].box.area()), 1e-30);
+    if (tn[root].count > 0) {
+        // Whole scene in one leaf: a root whose second child is empty.
+        DrNode r{};
+        float bx[6];
+        childBox(tn[root].box, bx);
+        uint32_t first;
+        nodes.push_back(r);
+        f.emitLeafTris(tn[root], first);
+        DrNode &o = nodes[0];
+        o.a = make_float4(bx[0], bx[1], bx[2], bx[3]);
+        o.b = make_float4(bx[4], bx[5], 0.f, 0.f);
+        o.c = make_float4(0.f, 0.f, 0.f, 0.f);
+        o.d = make_uint4(first, DR_EMPTY_CHILD, tn[root].count, 0);
+    } else {
+        f.emit(int32_t(root));
+    }
+    if (stats) {
+        stats->nodeCount = uint32_t(nodes.size());
+        stats->leafCount = f.leaves;
+        stats->maxDepth = b.maxDepth.load();
+        stats->sahCost = f.sah;
+        stats->buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+}  // namespace dr

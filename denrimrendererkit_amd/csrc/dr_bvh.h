@@ -1,0 +1,34 @@
+// dr_bvh.h -- host-side builder of the traversal BVH.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+#include "dr_scene.h"
+
+namespace dr {
+
+struct BvhBuildStats {
+    uint32_t nodeCount = 0;
+    uint32_t leafCount = 0;
+    uint32_t maxDepth = 0;
+    double sahCost = 0.0;
+    double buildSeconds = 0.0;
+};
+
+// Builds a binned-SAH binary BVH over the triangles' world-space vertices
+// (reference GPUTriangle records) and writes the traversal layout:
+//   nodes  -- DrNode per inner node, depth-first pre-order, root at 0;
+//   triPos -- 3 float4 per triangle in leaf order: {v0, bits(orig index)},
+//             {v1 - v0, 0}, {v2 - v0, 0}.
+// Child boxes are widened outward by a few ulps of their magnitude so the
+// fused-multiply-add slab test can never cull a box that the exact test
+// would keep; that keeps the closest hit identical to a brute-force search.
+// Depth is capped at DR_MAX_DEPTH so the traversal stack can never overflow.
+void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode> &nodes,
+              std::vector<float4> &triPos, BvhBuildStats *stats);
+
+}  // namespace dr
+
+#define DR_MAX_DEPTH 64
+#define DR_EMPTY_CHILD 0xffffffffu

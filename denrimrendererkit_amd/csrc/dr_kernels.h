@@ -1,0 +1,67 @@
+// dr_kernels.h -- path-state buffers and the per-sample launch sequence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dr_scene.h"
+
+#define DR_TRACE_BLOCK 128
+#define DR_SHADE_BLOCK 128
+#define DR_MAX_BOUNCES 64
+// Traversal stack entries kept in LDS per lane; deeper entries spill to a
+// per-lane global area of (DR_MAX_DEPTH - DR_LDS_STACK) words.
+#define DR_LDS_STACK 24
+
+// Device counter words: queue lengths per bounce (index b = queue consumed
+// by bounce b), work-stealing cursors of extend and shadow per bounce, and
+// shadow-queue lengths per bounce. Zeroed by k_raygen at the start of each
+// sample batch, so no counter is reset while a kernel reads it.
+#define DR_CTR_QUEUE 0
+#define DR_CTR_EXTEND (DR_CTR_QUEUE + DR_MAX_BOUNCES + 1)
+#define DR_CTR_SHADOWQ (DR_CTR_EXTEND + DR_MAX_BOUNCES)
+#define DR_CTR_SHADOW (DR_CTR_SHADOWQ + DR_MAX_BOUNCES)
+#define DR_CTR_WORDS (DR_CTR_SHADOW + DR_MAX_BOUNCES)
+
+struct DrPaths {
+    float4 *rayO;   // origin.xyz, previousBSDFPDF
+    float4 *rayD;   // direction.xyz, -
+    float4 *thr;    // throughput.xyz, outputAlpha
+    float4 *rad;    // radiance.xyz, -
+    uint4 *meta;    // pixel, sampleIndex, rng state, flags
+    float4 *hit;    // t, u, v, bits(triangle id)
+    float4 *neeT;   // throughput at the NEE vertex
+    float4 *shO;    // 3 per slot: shadow origin.xyz, max distance
+    float4 *shD;    // 3 per slot: direction.xyz, pdf denominator
+    float4 *shC;    // 3 per slot: unshadowed contribution.xyz, valid
+    float4 *shR;    // 3 per slot: transmittance.xyz, visible
+    uint32_t *queue[2];
+    uint32_t *shadowQueue;
+    uint32_t *counters;  // DR_CTR_WORDS
+    unsigned long long *stats;  // DR_CNT_SLOTS, accumulated over launches
+    uint32_t *overflow;  // traversal stack spill, overflowStride lanes
+    uint32_t overflowStride;
+};
+
+struct DrOutputs {
+    float4 *accumulation;
+    float4 *depth, *normal, *albedo, *materialID, *objectID, *motion, *triangleID;
+};
+
+enum DrStage {
+    DR_STAGE_RAYGEN = 0,
+    DR_STAGE_EXTEND = 1,
+    DR_STAGE_SHADE = 2,
+    DR_STAGE_SHADOW = 3,
+    DR_STAGE_RESOLVE = 4,
+    DR_STAGE_FINALIZE = 5,
+    DR_STAGE_COUNT = 6
+};
+
+namespace dr {
+// Enqueues one stage of a sample batch on `st`. One batch is RAYGEN, then
+// for b in [0, maxBounces): EXTEND, SHADE, SHADOW, RESOLVE; then FINALIZE.
+hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
+                       const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
+                       uint32_t traceGrid, hipStream_t st);
+}

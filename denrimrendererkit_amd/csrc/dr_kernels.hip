@@ -1,0 +1,815 @@
+// dr_kernels.hip -- wavefront progressive path tracer for gfx950.
+//
+// One progressive sample of pathTraceKernel (PathTrace.metal:5167-5586) is
+// split into stages that run over compacted queues of path slots:
+//
+//   raygen   -- slot -> (pixel, sample); RNG seed and camera ray (:5212-5225)
+//   extend   -- closest-hit traversal for every live path (persistent,
+//               wave-granular work stealing)
+//   shade    -- one bounce of the reference loop body (:5279-5517): miss and
+//               emission terms, pass-through, volume / subsurface / dielectric
+//               continuation, NEE sample generation, lobe sampling, RR. Live
+//               paths are appended to the next queue with a wave ballot and
+//               one atomic per wave; NEE shadow rays go to a shadow queue.
+//   shadow   -- occlusion / transmittance of each NEE ray (any-hit when no
+//               material can pass light, else the reference's 8-step chain)
+//   resolve  -- sums a path's NEE terms in the reference order and adds the
+//               clamped total (:5394-5422)
+//   finalize -- running-mean accumulation in sample order (:5520-5524) and
+//               miss AOVs (:5578-5585)
+//
+// Path state is structure-of-arrays of float4/uint4 (one dwordx4 per lane,
+// coalesced). The per-path RNG stream is keyed by (frameSeed, pixel,
+// sampleIndex) only, so a path's result does not depend on which slot,
+// queue position or batch it ran in.
+#include "dr_kernels.h"
+#include "dr_shade.h"
+#include "dr_traverse.h"
+
+#define DR_FLAG_PRIMARY 1u
+#define DR_FLAG_NEE 2u
+
+__device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+// Wave-aggregated append: returns the position for this lane (valid only if
+// keep). All lanes of the wave that are still active must call it.
+__device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t *counter) {
+    uint64_t mask = __ballot(keep);
+    uint32_t total = __popcll(mask);
+    uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    uint32_t base = 0;
+    if (total && laneId() == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    uint32_t below = __popcll(mask & ((1ull << laneId()) - 1ull));
+    return base + below;
+}
+
+__device__ __forceinline__ void waveAddStat(unsigned long long *stats, int slot, uint32_t v) {
+    // Counting runs only (statsEnabled); one relaxed 64-bit atomic per lane.
+    if (v) atomicAdd(&stats[slot], (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------- raygen --
+__global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        for (int k = 0; k < DR_CTR_WORDS; ++k) P.counters[k] = 0u;
+        P.counters[DR_CTR_QUEUE + 0] = nSlots;
+    }
+    if (i >= nSlots) return;
+    uint32_t tilePixels = L.tileWidth * L.tileHeight;
+    uint32_t sLocal = i / tilePixels;
+    uint32_t p = i - sLocal * tilePixels;
+    uint32_t px = L.tileX + p % L.tileWidth;
+    uint32_t py = L.tileY + p / L.tileWidth;
+    uint32_t sampleIndex = L.sampleBegin + sLocal;
+    uint32_t state = L.frameSeed ^ dr_hash(px + py * L.width) ^ dr_hash(sampleIndex + 1u);
+    float jx = randomFloat(state);
+    float jy = randomFloat(state);
+    float lx = randomFloat(state);
+    float ly = randomFloat(state);
+    vec3 o, d;
+    makeCameraRay(cam, L.width, L.height, px, py, jx, jy, lx, ly, o, d);
+    P.rayO[i] = make_float4(o.x, o.y, o.z, 0.0f);  // w: previousBSDFPDF
+    P.rayD[i] = make_float4(d.x, d.y, d.z, 0.0f);
+    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: outputAlpha
+    P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.meta[i] = make_uint4(px + py * L.width, sampleIndex, state, 0u);
+    P.queue[0][i] = i;
+}
+
+// ---------------------------------------------------------------- extend --
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
+    const uint32_t *queue = P.queue[bounce & 1u];
+    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    uint32_t *steal = &P.counters[DR_CTR_EXTEND + bounce];
+    uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    uint32_t nv = 0, nt = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (laneId() == 0) base = atomicAdd(steal, 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        uint32_t qi = base + laneId();
+        if (qi < count) {
+            uint32_t slot = queue[qi];
+            float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+            DrRayPre r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+            DrHitRec h{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
+            dr_traverse<false>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+            P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        }
+    }
+    if (statsOn) {
+        waveAddStat(P.stats, DR_CNT_NODES, nv);
+        waveAddStat(P.stats, DR_CNT_TRIS, nt);
+    }
+}
+
+// Fills the surface record of a hit (attribute half of intersectTriangle,
+// PathTrace.metal:1841-1850, and the Hit fields set at :3372-3384).
+struct Surf {
+    vec3 position, normal, tangent, bitangent;
+    vec2 uv;
+    float t;
+    uint32_t materialID, objectID, primitiveID;
+    bool frontFacing;
+};
+__device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, float4 h) {
+    Surf s;
+    uint32_t id = __float_as_uint(h.w);
+    const DrTriangle &tri = S.triangles[id];
+    float t = h.x, u = h.y, v = h.z;
+    float w = 1.0f - u - v;
+    vec3 n = normalize(w * xyz(tri.n0) + u * xyz(tri.n1) + v * xyz(tri.n2));
+    s.frontFacing = dot(n, d) <= 0.0f;
+    if (dot(n, d) > 0.0f) n = -n;
+    s.t = t;
+    s.position = o + t * d;
+    s.normal = n;
+    s.uv = vec2{w * tri.uv0.x + u * tri.uv1.x + v * tri.uv2.x, w * tri.uv0.y + u * tri.uv1.y + v * tri.uv2.y};
+    s.tangent = normalize(xyz(tri.tangent));
+    s.bitangent = normalize(xyz(tri.bitangent));
+    s.materialID = tri.materialID;
+    s.objectID = tri.objectID;
+    s.primitiveID = id;
+    return s;
+}
+
+__device__ __forceinline__ Mat materialFor(const DrSceneDev &S, uint32_t materialID) {  // :1096-1137 (triangles)
+    const DrMaterial *p = &S.materials[min(materialID, S.materialCount - 1u)];
+    return Mat{p, p->s[M_BASE]};
+}
+
+__device__ __forceinline__ void applyNormalMap(const DrSceneDev &S, Surf &h, const Mat &m, vec3 rayDir) {  // :4021-4054
+    uint32_t ti = (uint32_t)ms<M_PARAMS>(m).w;
+    if (ti == 0u) return;
+    vec3 mapped = xyz(sampleTexture(S, ti, h.uv, make_float4(0.5f, 0.5f, 1.0f, 1.0f)));
+    mapped = normalize(mapped * 2.0f - v3s(1.0f));
+    vec3 t = h.tangent - h.normal * dot(h.tangent, h.normal);
+    if (length(t) <= 1e-5f) {
+        vec3 helper = fabsf(h.normal.y) < 0.999f ? v3(0, 1, 0) : v3(1, 0, 0);
+        t = cross(helper, h.normal);
+    }
+    t = normalize(t);
+    vec3 b = h.bitangent - h.normal * dot(h.bitangent, h.normal);
+    if (length(b) <= 1e-5f) b = cross(h.normal, t);
+    b = normalize(b);
+    h.normal = normalize(mapped.x * t + mapped.y * b + mapped.z * h.normal);
+    if (dot(h.normal, rayDir) > 0.0f) h.normal = -h.normal;
+}
+__device__ __forceinline__ void applyBaseColorTexture(const DrSceneDev &S, Mat &m, const Surf &h) {  // :4005-4019
+    uint32_t ti = (uint32_t)ms<M_PARAMS>(m).z;
+    if (ti == 0u) return;
+    float4 t = sampleTexture(S, ti, h.uv, make_float4(1.0f, 1.0f, 1.0f, 1.0f));
+    m.base = make_float4(m.base.x * t.x, m.base.y * t.y, m.base.z * t.z, m.base.w * t.w);
+}
+
+__device__ __forceinline__ float lightPDFForHit(const DrSceneDev &S, const Surf &h, vec3 incoming) {  // :1580-1613
+    if (h.primitiveID >= S.triangleCount) return 0.0f;
+    uint32_t lp1 = S.triangles[h.primitiveID].padding2;
+    if (lp1 == 0u) return 0.0f;
+    uint32_t index = lp1 - 1u;
+    if (index >= S.lightCount) return 0.0f;
+    const DrLight &l = S.lights[index];
+    if (l.triangleIndex != h.primitiveID || l.area <= 0.0f) return 0.0f;
+    float cosLight = fmaxf(0.0f, dot(xyz(l.normal), -incoming));
+    if (cosLight <= 0.0f) return 0.0f;
+    return lightSelectionPDF(S.lights, index) * lightSolidAnglePDF(h.t * h.t, cosLight, l.area);
+}
+
+// Subsurface helpers (PathTrace.metal:3490-3523).
+__device__ __forceinline__ float sampleFreeFlight(vec3 sigmaT, float ux, float uy) {
+    float sigma = sigmaT.z;
+    if (ux < 0.33333333333f) sigma = sigmaT.x;
+    else if (ux < 0.66666666667f) sigma = sigmaT.y;
+    return -logf(fmaxf(1.0f - uy, 1e-6f)) / fmaxf(sigma, 1e-4f);
+}
+__device__ __forceinline__ float collisionPDF(vec3 sigmaT, float d) {
+    vec3 tr = vexp((-sigmaT) * d);
+    return fmaxf((sigmaT.x * tr.x + sigmaT.y * tr.y + sigmaT.z * tr.z) * 0.33333333333f, 1e-6f);
+}
+__device__ __forceinline__ float boundaryPDF(vec3 sigmaT, float d) {
+    vec3 tr = vexp((-sigmaT) * d);
+    return fmaxf((tr.x + tr.y + tr.z) * 0.33333333333f, 1e-6f);
+}
+
+// Closest hit used by the nested queries of the subsurface walk.
+__device__ __forceinline__ bool traceClosest(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, DrHitRec &h,
+                                             uint32_t *nv, uint32_t *nt) {
+    DrRayPre r = dr_prep(o, d);
+    h = DrHitRec{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
+    return dr_traverse<false>(S, r, h, stk, nv, nt);
+}
+
+__device__ __noinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, const Surf &entry, const Mat &m, vec3 &ro,
+                                                          vec3 &rd, vec3 &T, float &prevPdf, uint32_t &state,
+                                                          DrStack &stk, uint32_t *cnt) {  // :3525-3621
+    float subsurface = materialSubsurface(m);
+    if (subsurface <= 0.0f || !entry.frontFacing) return false;
+    float e0 = randomFloat(state);
+    float e1 = randomFloat(state);
+    vec3 entryDir = orientHemisphere(cosineHemisphere(e0, e1), -entry.normal);
+    vec3 mo = entry.position + entryDir * 0.001f;
+    vec3 md = entryDir;
+    vec3 sigmaT = v3s(1.0f) / materialSubsurfaceRadius(m);
+    vec3 albedo = materialSubsurfaceColor(m);
+    float g = materialSubsurfaceAnisotropy(m);
+    for (uint32_t step = 0u; step < 8u; ++step) {
+        cnt[DR_CNT_BOUNCE]++;
+        DrHitRec hr;
+        if (!traceClosest(S, mo, md, stk, hr, &cnt[DR_CNT_NODES], &cnt[DR_CNT_TRIS])) return false;
+        Surf b = surfaceAt(S, mo, md, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
+        float f0 = randomFloat(state);
+        float f1 = randomFloat(state);
+        float ff = sampleFreeFlight(sigmaT, f0, f1);
+        if (ff < b.t) {
+            mo += md * ff;
+            float h0 = randomFloat(state);
+            float h1 = randomFloat(state);
+            md = sampleHenyeyGreenstein(h0, h1, md, g);
+            vec3 tr = vexp((-sigmaT) * ff);
+            T *= albedo * sigmaT * tr / collisionPDF(sigmaT, ff);
+            if (maxComponent(T) <= 1e-4f) return false;
+            continue;
+        }
+        vec3 exitN = b.frontFacing ? b.normal : -b.normal;
+        exitN = normalize(exitN);
+        float x0 = randomFloat(state);
+        float x1 = randomFloat(state);
+        vec3 local = cosineHemisphere(x0, x1);
+        rd = orientHemisphere(local, exitN);
+        ro = b.position + exitN * 0.001f;
+        vec3 tr = vexp((-sigmaT) * b.t);
+        T *= tr / boundaryPDF(sigmaT, b.t);
+        prevPdf = fmaxf(dot(exitN, rd), 0.0f) * 0.31830988618f;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool continueTransmissionVolumeRandomWalk(const Surf &b, const Mat &m, vec3 &ro, vec3 &rd,
+                                                                     vec3 &T, float &prevPdf,
+                                                                     uint32_t &state) {  // :3685-3721
+    if (b.frontFacing || !materialHasVolumeScattering(m)) return false;
+    vec3 sigmaT = materialVolumeSigmaT(m);
+    vec3 sigmaS = materialVolumeSigmaS(m);
+    if (maxComponent(sigmaT) <= 1e-5f || maxComponent(sigmaS) <= 1e-5f) return false;
+    float f0 = randomFloat(state);
+    float f1 = randomFloat(state);
+    float ff = sampleFreeFlight(sigmaT, f0, f1);
+    if (ff >= b.t) {
+        T *= materialVolumeTransmittance(m, b.t) / boundaryPDF(sigmaT, b.t);
+        return false;
+    }
+    vec3 tr = vexp((-sigmaT) * ff);
+    T *= sigmaS * tr / collisionPDF(sigmaT, ff);
+    ro += rd * ff;
+    float h0 = randomFloat(state);
+    float h1 = randomFloat(state);
+    rd = sampleHenyeyGreenstein(h0, h1, rd, clampf(ms<M_VOL_PARAMS>(m).y, -0.95f, 0.95f));
+    ro += rd * 0.001f;
+    prevPdf = 0.0f;
+    return maxComponent(T) > 1e-4f;
+}
+
+__device__ __noinline__ bool continueTransmissiveSurface(const Surf &h, const Mat &m, vec3 &ro, vec3 &rd, vec3 &T,
+                                                         float &prevPdf, uint32_t &state) {  // :1723-1799
+    float transmission = materialTransmission(m);
+    if (transmission <= 0.0f) return false;
+    float gate = randomFloat(state);
+    if (gate > transmission) return false;
+    vec3 incoming = rd;
+    vec3 view = -incoming;
+    float roughness = materialTransmissionRoughness(m);
+    float ior = materialTransmissionIOR(m);
+    vec3 micro = h.normal;
+    if (roughness > 0.01f) {
+        float a = randomFloat(state);
+        float b = randomFloat(state);
+        micro = sampleGGXHalfVector(a, b, h.normal, roughness);
+        if (dot(micro, incoming) > 0.0f) micro = -micro;
+    }
+    float etaI = h.frontFacing ? 1.0f : ior;
+    float etaT = h.frontFacing ? ior : 1.0f;
+    float eta = etaI / etaT;
+    float cosThetaI = fmaxf(dot(view, micro), 0.0f);
+    float fresnel = dielectricFresnel(cosThetaI, etaI, etaT);
+    if (materialThinWalled(m)) {
+        fresnel = dielectricFresnel(cosThetaI, 1.0f, ior);
+        float rp = clampf(fresnel, 0.02f, 0.98f);
+        float choice = randomFloat(state);
+        if (choice < rp) {
+            rd = normalize(reflect(incoming, micro));
+            ro = h.position + micro * 0.001f;
+            T *= fresnel / fmaxf(rp, 1e-5f);
+        } else {
+            rd = incoming;
+            ro = h.position + rd * 0.001f;
+            float grazing = 1.0f - fabsf(dot(view, h.normal));
+            float tint = mixf(0.35f, 0.95f, grazing * grazing);
+            float w = (1.0f - fresnel) / fmaxf(1.0f - rp, 1e-5f);
+            T *= materialTransmissionTint(m, tint) * w;
+        }
+        prevPdf = 0.0f;
+        return true;
+    }
+    vec3 refracted = refract(incoming, micro, eta);
+    bool tir = dot(refracted, refracted) <= 1e-7f;
+    float rp = tir ? 1.0f : clampf(fresnel, 0.02f, 0.98f);
+    float choice = randomFloat(state);
+    if (choice < rp) {
+        rd = normalize(reflect(incoming, micro));
+        ro = h.position + micro * 0.001f;
+        T *= tir ? 1.0f : fresnel / fmaxf(rp, 1e-5f);
+    } else {
+        rd = normalize(refracted);
+        ro = h.position + rd * 0.001f;
+        float grazing = 1.0f - fabsf(dot(view, h.normal));
+        float tint = mixf(0.35f, 0.95f, grazing * grazing);
+        float w = (1.0f - fresnel) / fmaxf(1.0f - rp, 1e-5f);
+        vec3 absorption = materialTransmissionAbsorption(m, h.t, !h.frontFacing && !materialHasVolumeScattering(m));
+        T *= materialTransmissionTint(m, tint) * absorption * w;
+    }
+    prevPdf = 0.0f;
+    return true;
+}
+
+__device__ __forceinline__ void writeAOV(float4 *plane, uint32_t pix, float4 v) { plane[pix] = v; }
+
+// ----------------------------------------------------------------- shade --
+__global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
+                                                         DrPaths P, DrOutputs O, uint32_t bounce) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
+    const uint32_t *queue = P.queue[bounce & 1u];
+    uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
+    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= count) return;  // whole waves past the end leave together
+    uint32_t gl = qi;
+    DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + gl, P.overflowStride};
+    uint32_t cnt[DR_CNT_SLOTS] = {0};
+
+    const uint32_t slot = queue[qi];
+    float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
+    uint4 meta = P.meta[slot];
+    float4 h4 = P.hit[slot];
+    vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
+    vec3 T = v3(t4.x, t4.y, t4.z), radiance = v3(l4.x, l4.y, l4.z);
+    float outputAlpha = t4.w;
+    float prevPdf = o4.w;
+    uint32_t state = meta.z;
+    uint32_t flags = meta.w & DR_FLAG_PRIMARY;
+    const bool hasPrimary = (flags & DR_FLAG_PRIMARY) != 0u;
+    const bool lastSample = meta.y == L.sampleBegin + L.samplesInBatch - 1u;
+    const uint32_t pix = meta.x;
+    const float clampValue = L.sampleRadianceClamp;
+    bool alive = false;
+    cnt[bounce == 0u ? DR_CNT_PRIMARY : DR_CNT_BOUNCE]++;
+
+    float4 nee0 = make_float4(0, 0, 0, 0), nee1 = nee0, nee2 = nee0;  // pre.xyz, valid
+    float4 neeO[3], neeD[3];
+    bool neeValid[3] = {false, false, false};
+
+    const uint32_t id = __float_as_uint(h4.w);
+    if (id == 0xffffffffu) {
+        // Miss (:5279-5305).
+        if (L.transparentBackground != 0u && !hasPrimary) {
+            outputAlpha = 0.0f;
+        } else if (L.showsEnvironmentBackground == 0u && !hasPrimary) {
+            outputAlpha = 1.0f;
+            radiance += vmax(v3(L.backgroundColor[0], L.backgroundColor[1], L.backgroundColor[2]), v3s(0.0f));
+        } else {
+            float w = 1.0f;
+            if (bounce > 0u && prevPdf > 0.0f) w = powerHeuristic(prevPdf, environmentPDF(S, rd));
+            vec3 c = T * sampleEnvironment(S, rd) * w;
+            radiance += clampSampleContribution(c, clampValue);
+        }
+    } else {
+        Surf h = surfaceAt(S, ro, rd, h4);
+        Mat m = materialFor(S, h.materialID);
+        applyNormalMap(S, h, m, rd);
+        applyBaseColorTexture(S, m, h);
+        if (m.base.w <= 0.001f) {
+            // Pass-through: consumes this bounce (:5318-5321).
+            ro = h.position + rd * 0.001f;
+            alive = true;
+        } else {
+            if (!hasPrimary) {
+                flags |= DR_FLAG_PRIMARY;
+                if (lastSample && L.denoiserEnabled == 0u) {
+                    vec3 en = h.normal * 0.5f + v3s(0.5f);
+                    float mid = (float)(h.materialID + 1u);
+                    float oid = (float)(h.objectID + 1u);
+                    writeAOV(O.depth, pix, make_float4(h.t, h.t, h.t, 1.0f));
+                    writeAOV(O.normal, pix, make_float4(en.x, en.y, en.z, 1.0f));
+                    writeAOV(O.albedo, pix, m.base);
+                    writeAOV(O.materialID, pix, make_float4(mid, mid, mid, 1.0f));
+                    writeAOV(O.objectID, pix, make_float4(oid, oid, oid, 1.0f));
+                    vec2 cur, prv;
+                    if (projectToScreen(cam, h.position, L.width, L.height, cur) &&
+                        projectToScreen(prevCam, h.position, L.width, L.height, prv)) {
+                        writeAOV(O.motion, pix, make_float4(prv.x - cur.x, prv.y - cur.y, 0.0f, 1.0f));
+                    } else {
+                        writeAOV(O.motion, pix, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+                    }
+                    float tid = __uint_as_float(h.primitiveID + 1u);
+                    writeAOV(O.triangleID, pix, make_float4(tid, tid, tid, tid));
+                }
+            }
+            bool handled = false;
+            if (continueTransmissionVolumeRandomWalk(h, m, ro, rd, T, prevPdf, state)) {
+                handled = true;
+                alive = !terminatePathWithRussianRoulette(bounce, T, state);
+            }
+            if (!handled) {
+                float subsurface = materialSubsurface(m);
+                if (subsurface > 0.0f) {
+                    float gate = randomFloat(state);
+                    if (gate < subsurface &&
+                        continueSubsurfaceRandomWalk(S, h, m, ro, rd, T, prevPdf, state, stk, cnt)) {
+                        handled = true;
+                        alive = !terminatePathWithRussianRoulette(bounce, T, state);
+                    }
+                }
+            }
+            if (!handled && continueTransmissiveSurface(h, m, ro, rd, T, prevPdf, state)) {
+                handled = true;
+                alive = !terminatePathWithRussianRoulette(bounce, T, state);
+            }
+            if (!handled && maxComponent(xyz(ms<M_EMISSION>(m))) > 0.0f) {
+                // Emitter (:5376-5392): add and stop.
+                float w = 1.0f;
+                if (bounce > 0u && prevPdf > 0.0f) w = powerHeuristic(prevPdf, lightPDFForHit(S, h, rd));
+                vec3 c = T * xyz(ms<M_EMISSION>(m)) * w;
+                radiance += clampSampleContribution(c, clampValue);
+                handled = true;
+            }
+            if (!handled) {
+                vec3 view = -rd;
+                // NEE sample generation (sampleDirectLighting, :4407-4585); the
+                // visibility term is applied later by k_shadow/k_resolve.
+                if (S.lightCount > 0u) {
+                    float r0 = randomFloat(state);
+                    uint32_t index = selectLightIndex(S.lights, S.lightCount, r0);
+                    const DrLight &light = S.lights[index];
+                    float selectionPDF = lightSelectionPDF(S.lights, index);
+                    uint32_t lti = light.triangleIndex;
+                    if (lti < S.triangleCount && selectionPDF > 0.0f) {
+                        const DrTriangle &lt = S.triangles[lti];
+                        const DrMaterial &lm = S.materials[min(light.materialIndex, S.materialCount - 1u)];
+                        vec3 emission = xyz(lm.s[M_EMISSION]);
+                        float area = light.area;
+                        if (area > 0.0f) {
+                            float ux = randomFloat(state);
+                            float uy = randomFloat(state);
+                            float su0 = sqrtf(ux);
+                            float b0 = 1.0f - su0, b1 = su0 * (1.0f - uy), b2 = su0 * uy;
+                            vec3 lp = b0 * xyz(lt.v0) + b1 * xyz(lt.v1) + b2 * xyz(lt.v2);
+                            vec3 toLight = lp - h.position;
+                            float d2 = dot(toLight, toLight);
+                            float dist = sqrtf(d2);
+                            vec3 ldir = toLight / dist;
+                            float cosS = fmaxf(0.0f, dot(h.normal, ldir));
+                            float cosL = fmaxf(0.0f, dot(xyz(light.normal), -ldir));
+                            if (cosS > 0.0f && cosL > 0.0f) {
+                                float lightPDF = selectionPDF * lightSolidAnglePDF(d2, cosL, area);
+                                float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, ldir);
+                                float mis = powerHeuristic(lightPDF, bsdfPDF);
+                                float G = cosS * cosL * area / fmaxf(d2, 1e-6f);
+                                vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, ldir);
+                                vec3 pre = brdf * emission * G * mis;
+                                vec3 so = h.position + h.normal * 0.001f;
+                                neeO[0] = make_float4(so.x, so.y, so.z, dist - 0.002f);
+                                neeD[0] = make_float4(ldir.x, ldir.y, ldir.z, selectionPDF);
+                                nee0 = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                                neeValid[0] = true;
+                            }
+                        }
+                    }
+                }
+                for (uint32_t e = 0; e < 2u; ++e) {
+                    float px = randomFloat(state);
+                    float py = randomFloat(state);
+                    float cdf = randomFloat(state);
+                    float envPDF;
+                    vec3 dir = sampleEnvironmentDirection(S, px, py, cdf, envPDF);
+                    float cosS = fmaxf(0.0f, dot(h.normal, dir));
+                    if (envPDF > 0.0f && cosS > 0.0f) {
+                        float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, dir);
+                        float mis = powerHeuristic(envPDF, bsdfPDF);
+                        vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, dir);
+                        vec3 env = sampleEnvironment(S, dir);
+                        vec3 pre = brdf * env * cosS * mis;
+                        vec3 so = h.position + h.normal * 0.001f;
+                        neeO[1 + e] = make_float4(so.x, so.y, so.z, 1.0e20f);
+                        neeD[1 + e] = make_float4(dir.x, dir.y, dir.z, envPDF * 2.0f);
+                        float4 rec = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                        if (e == 0) nee1 = rec; else nee2 = rec;
+                        neeValid[1 + e] = true;
+                    }
+                }
+                if (neeValid[0] || neeValid[1] || neeValid[2]) {
+                    flags |= DR_FLAG_NEE;
+                    P.neeT[slot] = make_float4(T.x, T.y, T.z, 0.0f);
+                }
+                // Lobe sampling (:5423-5513).
+                float roughness = clampf(ms<M_PARAMS>(m).x, 0.02f, 1.0f);
+                float anisotropy = materialSpecularAnisotropy(m);
+                vec3 f0 = materialF0(m);
+                float clearcoat = materialClearcoat(m);
+                float ccr = materialClearcoatRoughness(m);
+                float pd, ps, pc;
+                materialLobeProbabilities(m, pd, ps, pc);
+                float d0 = randomFloat(state);
+                float d1 = randomFloat(state);
+                vec3 diffuseDir = orientHemisphere(cosineHemisphere(d0, d1), h.normal);
+                vec3 next;
+                float lobe = randomFloat(state);
+                if (lobe < pc) {
+                    float a = randomFloat(state);
+                    float b = randomFloat(state);
+                    vec3 hv = sampleGGXVisibleNormal(a, b, h.normal, h.tangent, h.bitangent, view, ccr, 0.0f);
+                    next = normalize(reflect(rd, hv));
+                    if (dot(next, h.normal) <= 0.0f) {
+                        next = diffuseDir;
+                        T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
+                    } else {
+                        float nDotL = fmaxf(dot(h.normal, next), 1e-5f);
+                        float vDotH = fmaxf(dot(view, hv), 1e-5f);
+                        T *= clearcoat * fresnelSchlickThinFilm(vDotH, materialClearcoatF0Color(m), m) *
+                             geometrySmithG1GGX(nDotL, ccr) / fmaxf(pc, 1e-5f);
+                    }
+                } else if (lobe < pc + ps) {
+                    float a = randomFloat(state);
+                    float b = randomFloat(state);
+                    vec3 hv = sampleGGXVisibleNormal(a, b, h.normal, h.tangent, h.bitangent, view, roughness, anisotropy);
+                    next = normalize(reflect(rd, hv));
+                    if (dot(next, h.normal) <= 0.0f) {
+                        next = diffuseDir;
+                        T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
+                    } else {
+                        float vDotH = fmaxf(dot(view, hv), 1e-5f);
+                        vec3 fresnel = fresnelSchlickThinFilm(vDotH, f0, m);
+                        vec3 cca = materialClearcoatAttenuationForDirection(m, h.normal, view, next);
+                        T *= cca * fresnel *
+                             geometrySmithG1GGXAnisotropic(next, h.normal, h.tangent, h.bitangent, roughness, anisotropy) /
+                             fmaxf(ps, 1e-5f);
+                    }
+                } else {
+                    next = diffuseDir;
+                    T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
+                }
+                prevPdf = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, next);
+                ro = h.position + h.normal * 0.001f;
+                rd = next;
+                alive = !terminatePathWithRussianRoulette(bounce, T, state);
+            }
+        }
+    }
+    alive = alive && (bounce + 1u < L.maxBounces);
+
+    P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
+    P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+    P.thr[slot] = make_float4(T.x, T.y, T.z, outputAlpha);
+    P.rad[slot] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+    P.meta[slot] = make_uint4(meta.x, meta.y, state, flags);
+
+    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+    if (alive) queueOut[pos] = slot;
+    for (int j = 0; j < 3; ++j) {
+        bool v = neeValid[j];
+        uint32_t sp = waveAppend(v, &P.counters[DR_CTR_SHADOWQ + bounce]);
+        if (v) {
+            uint32_t k = slot * 3u + (uint32_t)j;
+            P.shO[k] = neeO[j];
+            P.shD[k] = neeD[j];
+            P.shC[k] = j == 0 ? nee0 : (j == 1 ? nee1 : nee2);
+            P.shadowQueue[sp] = k;
+        } else {
+            P.shC[slot * 3u + (uint32_t)j].w = 0.0f;
+        }
+    }
+    if (L.statsEnabled) {
+        waveAddStat(P.stats, DR_CNT_PRIMARY, cnt[DR_CNT_PRIMARY]);
+        waveAddStat(P.stats, DR_CNT_BOUNCE, cnt[DR_CNT_BOUNCE]);
+        waveAddStat(P.stats, DR_CNT_NODES, cnt[DR_CNT_NODES]);
+        waveAddStat(P.stats, DR_CNT_TRIS, cnt[DR_CNT_TRIS]);
+    }
+}
+
+// ---------------------------------------------------------------- shadow --
+// shadowOccluded (PathTrace.metal:4196-4274) for one NEE ray; writes the
+// transmittance (0 if occluded) into shC[k].w's companion record shR[k].
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
+    const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce];
+    uint32_t *steal = &P.counters[DR_CTR_SHADOW + bounce];
+    uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    uint32_t nv = 0, nt = 0, nq = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (laneId() == 0) base = atomicAdd(steal, 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        uint32_t qi = base + laneId();
+        if (qi >= count) continue;
+        uint32_t k = P.shadowQueue[qi];
+        float4 o4 = P.shO[k], d4 = P.shD[k];
+        vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+        float maxDistance = o4.w;
+        vec3 tr = v3s(1.0f);
+        bool occluded;
+        if (S.opaqueOnly) {
+            ++nq;
+            DrRayPre r = dr_prep(o, d);
+            DrHitRec h{maxDistance, 0.0f, 0.0f, 0u};  // id 0: the limit is exclusive
+            occluded = dr_traverse<true>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+        } else {
+            occluded = true;
+            float traveled = 0.0f;
+            for (uint32_t step = 0; step < 8u; ++step) {
+                ++nq;
+                DrRayPre r = dr_prep(o, d);
+                DrHitRec h{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
+                bool hit = dr_traverse<false>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+                if (!hit || traveled + h.t >= maxDistance) {
+                    occluded = false;
+                    break;
+                }
+                Surf sh = surfaceAt(S, o, d, make_float4(h.t, h.u, h.v, __uint_as_float(h.id)));
+                Mat m = materialFor(S, sh.materialID);
+                if (m.base.w <= 0.001f) {
+                    traveled += sh.t;
+                    o = sh.position + d * 0.001f;
+                    continue;
+                }
+                float transmission = materialTransmission(m);
+                if (transmission <= 0.001f) break;
+                bool inside = !sh.frontFacing && !materialThinWalled(m);
+                vec3 medium = materialHasVolumeScattering(m) && inside ? materialVolumeTransmittance(m, sh.t)
+                                                                       : materialTransmissionAbsorption(m, sh.t, inside);
+                tr = tr * materialTransmissionTint(m, 0.35f) * medium * transmission;
+                if (maxComponent(tr) < 0.02f) break;
+                traveled += sh.t;
+                o = sh.position + d * 0.001f;
+            }
+        }
+        P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
+    }
+    if (statsOn) {
+        waveAddStat(P.stats, DR_CNT_SHADOW, nq);
+        waveAddStat(P.stats, DR_CNT_SHADOW_NODES, nv);
+        waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, nt);
+    }
+}
+
+// --------------------------------------------------------------- resolve --
+__global__ void k_resolve(DrLaunch L, DrPaths P, uint32_t bounce) {
+    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= count) return;
+    uint32_t slot = P.queue[bounce & 1u][qi];
+    uint4 meta = P.meta[slot];
+    if ((meta.w & DR_FLAG_NEE) == 0u) return;
+    // direct += pre * T / denom, light sample then the two HDRI samples
+    // (:4508, :4578-4579), then radiance += clamp(throughput * direct).
+    vec3 direct = v3s(0.0f);
+    for (uint32_t j = 0; j < 3u; ++j) {
+        uint32_t k = slot * 3u + j;
+        float4 c = P.shC[k];
+        if (c.w == 0.0f) continue;
+        float4 r = P.shR[k];
+        if (r.w == 0.0f) continue;
+        float denom = P.shD[k].w;
+        direct += v3(c.x, c.y, c.z) * v3(r.x, r.y, r.z) / denom;
+    }
+    float4 t = P.neeT[slot];
+    float4 l = P.rad[slot];
+    vec3 add = clampSampleContribution(v3(t.x, t.y, t.z) * direct, L.sampleRadianceClamp);
+    P.rad[slot] = make_float4(l.x + add.x, l.y + add.y, l.z + add.z, l.w);
+    P.meta[slot] = make_uint4(meta.x, meta.y, meta.z, meta.w & ~DR_FLAG_NEE);
+}
+
+// ---------------------------------------------------------------- finish --
+// Primary AOV ray used when a denoiser is enabled (tracePrimaryAOV,
+// PathTrace.metal:4093-4160).
+__device__ bool tracePrimaryAOV(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, Surf &out, Mat &outM) {
+    for (uint32_t pass = 0; pass < 32u; ++pass) {
+        DrHitRec hr;
+        if (!traceClosest(S, o, d, stk, hr, nullptr, nullptr)) return false;
+        Surf h = surfaceAt(S, o, d, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
+        Mat m = materialFor(S, h.materialID);
+        applyNormalMap(S, h, m, d);
+        applyBaseColorTexture(S, m, h);
+        if (m.base.w <= 0.001f) {
+            o = h.position + d * 0.001f;
+            continue;
+        }
+        out = h;
+        outM = m;
+        return true;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
+                                                            DrPaths P, DrOutputs O) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t tilePixels = L.tileWidth * L.tileHeight;
+    if (p >= tilePixels) return;
+    uint32_t px = L.tileX + p % L.tileWidth;
+    uint32_t py = L.tileY + p / L.tileWidth;
+    uint32_t pix = px + py * L.width;
+    float4 acc = O.accumulation[pix];
+    uint32_t lastFlags = 0;
+    for (uint32_t s = 0; s < L.samplesInBatch; ++s) {
+        uint32_t slot = s * tilePixels + p;
+        float4 l = P.rad[slot];
+        float alpha = P.thr[slot].w;
+        float w = 1.0f / ((float)(L.sampleBegin + s) + 1.0f);
+        acc.x = mixf(acc.x, l.x, w);
+        acc.y = mixf(acc.y, l.y, w);
+        acc.z = mixf(acc.z, l.z, w);
+        acc.w = mixf(acc.w, alpha, w);
+        lastFlags = P.meta[slot].w;
+    }
+    O.accumulation[pix] = acc;
+    bool has = (lastFlags & DR_FLAG_PRIMARY) != 0u;
+    if (L.denoiserEnabled != 0u) {
+        DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + p, P.overflowStride};
+        vec3 o, d;
+        makeCameraRay(cam, L.width, L.height, px, py, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
+        Surf h;
+        Mat m;
+        has = tracePrimaryAOV(S, o, d, stk, h, m);
+        if (has) {
+            vec3 en = h.normal * 0.5f + v3s(0.5f);
+            float mid = (float)(h.materialID + 1u), oid = (float)(h.objectID + 1u);
+            O.depth[pix] = make_float4(h.t, h.t, h.t, 1.0f);
+            O.normal[pix] = make_float4(en.x, en.y, en.z, 1.0f);
+            O.albedo[pix] = m.base;
+            O.materialID[pix] = make_float4(mid, mid, mid, 1.0f);
+            O.objectID[pix] = make_float4(oid, oid, oid, 1.0f);
+            vec2 cur, prv;
+            if (projectToScreen(cam, h.position, L.width, L.height, cur) &&
+                projectToScreen(prevCam, h.position, L.width, L.height, prv))
+                O.motion[pix] = make_float4(prv.x - cur.x, prv.y - cur.y, 0.0f, 1.0f);
+            else
+                O.motion[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float tid = __uint_as_float(h.primitiveID + 1u);
+            O.triangleID[pix] = make_float4(tid, tid, tid, tid);
+        }
+    }
+    if (!has) {
+        O.depth[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.normal[pix] = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
+        O.albedo[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.materialID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.objectID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.motion[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.triangleID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// --------------------------------------------------------------- launches --
+namespace dr {
+
+hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
+                       const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
+                       uint32_t traceGrid, hipStream_t st) {
+    const uint32_t bThreads = 256;
+    switch (stage) {
+    case DR_STAGE_RAYGEN:
+        hipLaunchKernelGGL(k_raygen, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, cam, P, nSlots);
+        break;
+    case DR_STAGE_EXTEND:
+        hipLaunchKernelGGL(k_extend, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        break;
+    case DR_STAGE_SHADE:
+        hipLaunchKernelGGL(k_shade, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK), 0, st,
+                           S, L, cam, prevCam, P, O, bounce);
+        break;
+    case DR_STAGE_SHADOW:
+        hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        break;
+    case DR_STAGE_RESOLVE:
+        hipLaunchKernelGGL(k_resolve, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, P, bounce);
+        break;
+    case DR_STAGE_FINALIZE: {
+        uint32_t tilePixels = L.tileWidth * L.tileHeight;
+        hipLaunchKernelGGL(k_finalize, dim3((tilePixels + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK),
+                           0, st, S, L, cam, prevCam, P, O);
+        break;
+    }
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dr

@@ -1,0 +1,102 @@
+// dr_scene.h -- device-resident scene and per-launch settings.
+//
+// The scene is held in HBM in two forms:
+//   * the caller's flat records in the reference layouts (GPUTriangle 192 B,
+//     GPUMaterial 272 B, GPULightRecord 32 B, GPUEnvironmentSample 8 B,
+//     texture descriptors and RGBA32F texels; Metal/GPUTypes.swift), read
+//     only at shading time;
+//   * a traversal form built by this library: a binary BVH whose nodes hold
+//     both child boxes (64 B per node, one 4 x dwordx4 fetch tests two
+//     boxes) and a position-only triangle stream in leaf order, 48 B per
+//     triangle {v0 | original index, v1 - v0, v2 - v0}, so a leaf's triangles
+//     are contiguous and the traversal never touches the 192-B records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// GPUTriangle (PathTrace.metal:14-30).
+struct DrTriangle {
+    float4 v0, v1, v2, n0, n1, n2, uv0, uv1, uv2, tangent, bitangent;
+    uint32_t materialID, objectID, primitiveID, padding2;
+};
+static_assert(sizeof(DrTriangle) == 192, "GPUTriangle layout");
+
+// GPUMaterial (PathTrace.metal:32-50): 17 float4 slots.
+enum DrMatSlot {
+    M_BASE = 0, M_EMISSION, M_PARAMS, M_PARAMS2, M_SPECULAR, M_SHEEN, M_TRANS, M_PARAMS3,
+    M_CLEARCOAT, M_CC_ATTEN, M_TRANS_ABS, M_THINFILM, M_SSS_COLOR, M_SSS_RADIUS, M_SSS_PARAMS,
+    M_VOL_SCATTER, M_VOL_PARAMS, M_SLOTS
+};
+struct DrMaterial {
+    float4 s[M_SLOTS];
+};
+static_assert(sizeof(DrMaterial) == 272, "GPUMaterial layout");
+
+// GPULightRecord (PathTrace.metal:62-68).
+struct DrLight {
+    uint32_t triangleIndex, materialIndex;
+    float area, selectionCDF;
+    float4 normal;
+};
+static_assert(sizeof(DrLight) == 32, "GPULightRecord layout");
+
+// Traversal node: both children's boxes plus child references.
+//   a = (c0.min.xyz, c0.max.x)  b = (c0.max.yz, c1.min.xy)
+//   c = (c1.min.z, c1.max.xyz)  d = (child0, child1, count0, count1)
+// count > 0: the child is a leaf, child = first triangle in leaf order;
+// count == 0: the child is an inner node index.
+struct DrNode {
+    float4 a, b, c;
+    uint4 d;
+};
+static_assert(sizeof(DrNode) == 64, "node layout");
+
+struct DrSceneDev {
+    const DrNode *nodes;
+    const float4 *triPos;        // 3 float4 per triangle in leaf order
+    const DrTriangle *triangles; // caller's records, original order
+    const DrMaterial *materials;
+    const DrLight *lights;
+    const float2 *env;           // (cdf, pdf) per environment texel
+    const uint4 *texDesc;        // (offset, width, height, mode)
+    const float4 *texels;
+    uint32_t triangleCount, materialCount, lightCount, envCount, texCount;
+    uint32_t envTexIndexPlusOne;
+    float envIntensity, envRotationY, envMaxRadiance;
+    uint32_t opaqueOnly;         // 1: no material can let a shadow ray through
+};
+
+// Per-launch settings: the fields of GPURenderConstants
+// (PathTrace.metal:150-189) that the triangle path reads.
+struct DrLaunch {
+    uint32_t width, height;
+    uint32_t maxBounces;
+    uint32_t frameSeed;
+    uint32_t transparentBackground, showsEnvironmentBackground, denoiserEnabled;
+    float sampleRadianceClamp;
+    float backgroundColor[4];
+    uint32_t tileX, tileY, tileWidth, tileHeight;
+    uint32_t sampleBegin;      // sample index of the first sample in this batch
+    uint32_t samplesInBatch;   // samples per pixel in flight together
+    uint32_t statsEnabled;
+};
+
+struct DrCamera {
+    float4 origin, lowerLeft, horizontal, vertical, lens;
+};
+
+// Traversal and query counters (the reference's SDF counter slots 10-12,
+// PathTrace.metal:201-203, plus node fetches and triangle tests for the
+// roofline: extension and subsurface queries in NODES/TRIS, NEE shadow
+// queries in SHADOW_NODES/SHADOW_TRIS).
+enum {
+    DR_CNT_PRIMARY = 0,
+    DR_CNT_BOUNCE = 1,
+    DR_CNT_SHADOW = 2,
+    DR_CNT_NODES = 3,
+    DR_CNT_TRIS = 4,
+    DR_CNT_SHADOW_NODES = 5,
+    DR_CNT_SHADOW_TRIS = 6,
+    DR_CNT_SLOTS = 8
+};

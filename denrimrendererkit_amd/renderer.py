@@ -1,0 +1,220 @@
+"""DenrimRenderer / RenderSession over the HIP backend's C ABI.
+
+Mirrors the reference's public session API (Renderer/DenrimRenderer.swift:52-292,
+Renderer/RenderSession.swift:582-1058) with Python spelling:
+
+=====================================  ==========================================
+reference                              here
+=====================================  ==========================================
+DenrimRenderer(device:)                DenrimRenderer(device_ordinal=0)
+makeSession(scene:settings:...)        renderer.make_session(scene, settings)
+session.render(samples:)               session.render(samples)
+session.renderNextSample()             session.render_next_sample()
+session.encodeNextSample(into:)        session.encode_next_sample(); session.synchronize()
+session.renderNextTile(_:completes..)  session.render_next_tile(x, y, w, h, completes_sample)
+session.resetAccumulation()            session.reset_accumulation()
+session.updateCamera(_:previous...)    session.update_camera(camera, previous_camera)
+session.sampleCount                    session.sample_count
+session.pixels(for:)                   session.pixels(output) -> (H, W, 4) float32
+session.sdfTraversalStats()            session.stats()
+=====================================  ==========================================
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, compile_scene
+from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
+
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None or a.size == 0 else a.ctypes.data
+
+
+def gpu_camera_struct(g: np.ndarray) -> _lib.GpuCamera:
+    c = _lib.GpuCamera()
+    for name, field in (("origin", "origin"), ("lowerLeft", "lower_left"), ("horizontal", "horizontal"),
+                        ("vertical", "vertical"), ("lens", "lens")):
+        arr = getattr(c, field)
+        for k in range(4):
+            arr[k] = float(g[name][k])
+    return c
+
+
+class DenrimRenderer:
+    """DenrimRenderer.init(device:) (DenrimRenderer.swift:52-63)."""
+
+    def __init__(self, device_ordinal: int = 0):
+        L = _lib.load()
+        h = C.c_void_p()
+        _lib.check(L.dr_hip_device_create(device_ordinal, C.byref(h)), "device")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib.load().dr_hip_device_destroy(self._h)
+            self._h = None
+
+    def upload(self, build: AccelerationBuild, max_leaf: int = 0) -> "GpuScene":
+        return GpuScene(self, build, max_leaf)
+
+    def make_session(self, scene, settings: RenderSettings,
+                     acceleration_mode: RenderAccelerationMode = RenderAccelerationMode.HIP,
+                     camera: Optional[Camera] = None, samples_per_launch: int = 0) -> "RenderSession":
+        """makeSession(scene:settings:accelerationMode:) (DenrimRenderer.swift:253-292).
+        `scene` is a RenderScene (compiled here) or an already uploaded GpuScene."""
+        if acceleration_mode not in (RenderAccelerationMode.HIP, RenderAccelerationMode.AUTOMATIC,
+                                     RenderAccelerationMode.FLAT_BVH):
+            raise DenrimRendererError("missingShaderFunction", f"{acceleration_mode.value} is not available")
+        if settings.width <= 0 or settings.height <= 0:
+            raise DenrimRendererError.invalid_scene("Render size must be greater than zero.")
+        if isinstance(scene, RenderScene):
+            gpu_scene = self.upload(compile_scene(scene))
+            cam = camera or scene.camera
+        else:
+            gpu_scene = scene
+            if camera is None:
+                raise DenrimRendererError("invalidArgument", "a camera is required with a pre-uploaded scene")
+            cam = camera
+        return RenderSession(gpu_scene, settings, cam, samples_per_launch)
+
+
+class GpuScene:
+    """A scene resident in HBM (records + traversal BVH); shareable by sessions."""
+
+    def __init__(self, renderer: DenrimRenderer, build: AccelerationBuild, max_leaf: int = 0):
+        self.renderer = renderer
+        self.build = build
+        keep = [np.ascontiguousarray(x) for x in (build.triangles, build.materials, build.lights,
+                                                   build.environment_samples, build.texture_descriptors,
+                                                   build.texture_pixels)]
+        t, m, li, e, td, tp = keep
+        d = _lib.SceneDesc(_ptr(t), len(t), _ptr(m), len(m), _ptr(li), len(li), _ptr(e), len(e), _ptr(td), len(td),
+                           _ptr(tp), len(tp), build.environment_texture_index_plus_one, build.environment_intensity,
+                           build.environment_rotation_y, build.environment_max_radiance, max_leaf)
+        h = C.c_void_p()
+        _lib.check(_lib.load().dr_hip_scene_upload(renderer._h, C.byref(d), C.byref(h)), "scene upload")
+        self._h = h
+
+    def info(self) -> dict:
+        n, l, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        s = C.c_double()
+        _lib.check(_lib.load().dr_hip_scene_info(self._h, C.byref(n), C.byref(l), C.byref(d), C.byref(s)), "info")
+        return {"nodes": n.value, "leaves": l.value, "max_depth": d.value, "build_seconds": s.value}
+
+    def release(self):
+        if getattr(self, "_h", None):
+            _lib.load().dr_hip_scene_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.release()
+
+
+class RenderSession:
+    def __init__(self, scene: GpuScene, settings: RenderSettings, camera: Camera, samples_per_launch: int = 0):
+        self.scene = scene
+        self.settings = settings
+        self.camera = camera
+        self.previous_camera = settings.previous_camera or camera
+        s = _lib.Settings()
+        s.width, s.height = settings.width, settings.height
+        s.max_bounces = max(1, settings.max_bounces)
+        s.quality = settings.quality.value
+        s.sample_radiance_clamp = settings.resolved_sample_radiance_clamp
+        s.transparent_background = int(settings.transparent_background)
+        s.shows_environment_background = int(settings.shows_environment_background)
+        for k in range(3):
+            s.background_color[k] = float(settings.background_color[k])
+        s.denoiser_enabled = int(bool(settings.denoise))
+        s.collects_stats = int(settings.collects_sdf_traversal_stats)
+        s.frame_seed = settings.frame_seed & 0xFFFFFFFF
+        s.samples_per_launch = samples_per_launch
+        self._settings = s
+        cam = gpu_camera_struct(camera.gpu_camera(settings.width, settings.height))
+        prev = gpu_camera_struct(self.previous_camera.gpu_camera(settings.width, settings.height))
+        h = C.c_void_p()
+        _lib.check(_lib.load().dr_hip_session_create(scene._h, C.byref(s), C.byref(cam), C.byref(prev), C.byref(h)),
+                   "session")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().dr_hip_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- rendering -----------------------------------------------------------
+    def render(self, samples: int) -> None:
+        if samples < 0:  # RenderSession.swift:1011-1013
+            raise DenrimRendererError.invalid_scene("Sample count must not be negative.")
+        _lib.check(_lib.load().dr_hip_session_render(self._h, samples), "render")
+
+    def render_next_sample(self) -> None:
+        self.render(1)
+
+    def encode_next_sample(self, samples: int = 1) -> None:
+        _lib.check(_lib.load().dr_hip_session_encode(self._h, samples), "encode")
+
+    def synchronize(self) -> None:
+        _lib.check(_lib.load().dr_hip_session_synchronize(self._h), "synchronize")
+
+    def render_next_tile(self, x: int, y: int, width: int, height: int, completes_sample: bool) -> None:
+        _lib.check(_lib.load().dr_hip_session_render_tile(self._h, x, y, width, height, int(completes_sample)), "tile")
+
+    def reset_accumulation(self) -> None:
+        _lib.check(_lib.load().dr_hip_session_reset(self._h), "reset")
+
+    def update_camera(self, camera: Camera, previous_camera: Optional[Camera] = None) -> None:
+        old = self.camera
+        self.camera = camera
+        self.previous_camera = previous_camera or old
+        W, H = self.settings.width, self.settings.height
+        c = gpu_camera_struct(camera.gpu_camera(W, H))
+        p = gpu_camera_struct(self.previous_camera.gpu_camera(W, H))
+        _lib.check(_lib.load().dr_hip_session_update_camera(self._h, C.byref(c), C.byref(p)), "camera")
+
+    @property
+    def sample_count(self) -> int:
+        v = C.c_uint32()
+        _lib.check(_lib.load().dr_hip_session_sample_count(self._h, C.byref(v)), "sample count")
+        return v.value
+
+    # -- outputs -------------------------------------------------------------
+    def pixels(self, output: RenderOutput = RenderOutput.BEAUTY) -> np.ndarray:
+        W, H = self.settings.width, self.settings.height
+        out = np.zeros((H, W, 4), np.float32)
+        _lib.check(_lib.load().dr_hip_session_read_output(self._h, int(output), _ptr(out), out.size), "read output")
+        return out
+
+    def triangle_ids(self) -> np.ndarray:
+        """Internal parity plane: global triangle index + 1 (0 = no hit)."""
+        return self.pixels(RenderOutput.TRIANGLE_ID)[..., 0].view(np.uint32).copy()
+
+    def stats(self) -> np.ndarray:
+        out = (C.c_uint64 * 8)()
+        _lib.check(_lib.load().dr_hip_session_stats(self._h, out, 8), "stats")
+        return np.array(list(out), dtype=np.uint64)
+
+    def reset_stats(self) -> None:
+        _lib.check(_lib.load().dr_hip_session_reset_stats(self._h), "reset stats")
+
+    def set_stage_timing(self, enabled: bool) -> None:
+        _lib.check(_lib.load().dr_hip_session_set_stage_timing(self._h, int(enabled)), "stage timing")
+
+    def timing(self) -> dict:
+        total = C.c_double()
+        ms = (C.c_double * 6)()
+        n = (C.c_uint64 * 6)()
+        _lib.check(_lib.load().dr_hip_session_timing(self._h, C.byref(total), ms, n, 6), "timing")
+        return {"total_ms": total.value, "stage_ms": dict(zip(STAGES, list(ms))),
+                "stage_launches": dict(zip(STAGES, [int(x) for x in n]))}

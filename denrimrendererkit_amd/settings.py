@@ -1,0 +1,65 @@
+"""Render settings and output kinds (Renderer/RenderSettings.swift:5-115,
+Renderer/RenderOutput.swift:4-25, RenderSession.swift:8-17)."""
+from __future__ import annotations
+
+import enum
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+from .scene import Camera
+
+
+class RenderQuality(enum.Enum):
+    """RenderSettings.swift:5-27."""
+    PREVIEW = 0
+    INTERACTIVE = 1
+    FINAL = 2
+
+    @property
+    def default_sample_radiance_clamp(self) -> float:
+        return {RenderQuality.PREVIEW: 10.0, RenderQuality.INTERACTIVE: 24.0, RenderQuality.FINAL: 64.0}[self]
+
+
+class RenderOutput(enum.IntEnum):
+    """RenderOutput.swift:4-25, in the reference's case order; TRIANGLE_ID
+    is this backend's internal parity output (the global materialized
+    triangle index + 1, stored as uint32 bits in every channel)."""
+    BEAUTY = 0
+    DEPTH = 1
+    NORMAL = 2
+    ALBEDO = 3
+    MATERIAL_ID = 4
+    OBJECT_ID = 5
+    MOTION_VECTOR = 6
+    TRIANGLE_ID = 7
+
+
+class RenderAccelerationMode(enum.Enum):
+    """RenderSession.swift:8-17 plus the new ``hip`` backend."""
+    AUTOMATIC = "automatic"
+    FLAT_BVH = "flat-bvh"
+    METAL_RAY_TRACING = "metal-ray-tracing"
+    HIP = "hip"
+
+
+@dataclass
+class RenderSettings:
+    """RenderSettings.swift:29-101 (defaults at :75-87)."""
+    width: int = 512
+    height: int = 512
+    max_bounces: int = 4
+    quality: RenderQuality = RenderQuality.PREVIEW
+    previous_camera: Optional[Camera] = None
+    transparent_background: bool = False
+    shows_environment_background: bool = True
+    background_color: Sequence[float] = (0.0, 0.0, 0.0)
+    denoise: bool = False
+    sample_radiance_clamp: Optional[float] = None
+    collects_sdf_traversal_stats: bool = False
+    frame_seed: int = 0x1234ABCD  # RenderSession.swift:917
+
+    @property
+    def resolved_sample_radiance_clamp(self) -> float:
+        if self.sample_radiance_clamp is not None:
+            return float(self.sample_radiance_clamp)
+        return self.quality.default_sample_radiance_clamp

@@ -1,0 +1,191 @@
+/*
+ * denrim_hip.h -- C ABI of the MI355X (gfx950) backend of the progressive
+ * path tracer.
+ *
+ * This is the boundary a Swift `hip` acceleration mode binds through a
+ * module map. Every entry point replaces one call of the
+ * reference's public session API or its internal acceleration plug point;
+ * the reference interface each one stands in for is cited beside it (paths
+ * relative to Sources/DenrimRendererKit/ of markusmoenig/DenrimRendererKit).
+ *
+ * Conventions
+ *   - Plain C types only; all arrays are caller-owned host memory that the
+ *     call copies (nothing is retained after it returns).
+ *   - Scene records use the reference's GPU layouts byte for byte
+ *     (Metal/GPUTypes.swift): GPUTriangle 192 B, GPUMaterial 272 B,
+ *     GPULightRecord 32 B, GPUEnvironmentSample 8 B, GPUTextureDescriptor
+ *     16 B, texels RGBA32F, GPUCamera 80 B. They are exactly the arrays of
+ *     AccelerationBuild (Acceleration/AccelerationBuild.swift:4-44).
+ *   - Every function returns a dr_status; on failure a thread-local message
+ *     is available from dr_last_error().
+ *   - A session is single-caller, like RenderSession (no internal locks).
+ */
+#ifndef DENRIM_HIP_H
+#define DENRIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* DenrimRendererError cases (Renderer/DenrimRenderer.swift:5-26) plus the
+ * argument and memory errors a C caller can hit. */
+typedef enum {
+    DR_OK = 0,
+    DR_ERR_NO_DEVICE = 1,              /* .noMetalDevice */
+    DR_ERR_MISSING_SHADER_FUNCTION = 2,/* .missingShaderFunction */
+    DR_ERR_INVALID_SCENE = 3,          /* .invalidScene */
+    DR_ERR_COMMAND_BUFFER_FAILED = 4,  /* .commandBufferFailed */
+    DR_ERR_PNG_EXPORT_FAILED = 5,      /* .pngExportFailed */
+    DR_ERR_INVALID_ARGUMENT = 6,
+    DR_ERR_OUT_OF_MEMORY = 7
+} dr_status;
+
+/* RenderOutput (Renderer/RenderOutput.swift:4-25) in case order, plus the
+ * internal triangle-ID plane used for parity checks (the global materialized
+ * triangle index + 1, stored as uint32 bits in all four channels). */
+typedef enum {
+    DR_OUTPUT_BEAUTY = 0,
+    DR_OUTPUT_DEPTH = 1,
+    DR_OUTPUT_NORMAL = 2,
+    DR_OUTPUT_ALBEDO = 3,
+    DR_OUTPUT_MATERIAL_ID = 4,
+    DR_OUTPUT_OBJECT_ID = 5,
+    DR_OUTPUT_MOTION_VECTOR = 6,
+    DR_OUTPUT_TRIANGLE_ID = 7
+} dr_output;
+
+typedef struct dr_device dr_device;
+typedef struct dr_scene dr_scene;
+typedef struct dr_session dr_session;
+
+/* GPUCamera (Metal/GPUTypes.swift:4-10): origin (w = 1 orthographic),
+ * lowerLeft, horizontal, vertical, lens (x = aperture radius, y = focus). */
+typedef struct {
+    float origin[4];
+    float lower_left[4];
+    float horizontal[4];
+    float vertical[4];
+    float lens[4];
+} dr_gpu_camera;
+
+/* The arrays of AccelerationBuild for the triangle path
+ * (Acceleration/AccelerationBuild.swift:4-44) plus the environment fields
+ * the session copies into GPURenderConstants (Renderer/RenderSession.swift:908-947). */
+typedef struct {
+    const void *triangles;            uint32_t triangle_count;           /* GPUTriangle[] */
+    const void *materials;            uint32_t material_count;           /* GPUMaterial[] */
+    const void *lights;               uint32_t light_count;              /* GPULightRecord[] */
+    const void *environment_samples;  uint32_t environment_sample_count; /* GPUEnvironmentSample[] */
+    const void *texture_descriptors;  uint32_t texture_count;            /* GPUTextureDescriptor[] */
+    const void *texture_pixels;       uint64_t texture_pixel_count;      /* SIMD4<Float>[] */
+    uint32_t environment_texture_index_plus_one;
+    float environment_intensity;
+    float environment_rotation_y;
+    float environment_max_radiance;
+    uint32_t max_leaf_triangles;      /* 0 = default (4) */
+} dr_scene_desc;
+
+/* RenderSettings (Renderer/RenderSettings.swift:29-101) as the session
+ * resolves it, plus the sample seed and batching knob of this backend. */
+typedef struct {
+    uint32_t width, height;
+    uint32_t max_bounces;             /* clamped to >= 1 like RenderSession.swift:916 */
+    uint32_t quality;                 /* 0 preview, 1 interactive, 2 final; only final (the path tracer) is served */
+    float sample_radiance_clamp;      /* < 0: the quality default (10 / 24 / 64) */
+    uint32_t transparent_background;
+    uint32_t shows_environment_background;
+    float background_color[3];
+    uint32_t denoiser_enabled;        /* AOVs from the pixel-centre ray (PathTrace.metal:5531-5557) */
+    uint32_t collects_stats;          /* collectsSDFTraversalStats: query / node / triangle counters */
+    uint32_t frame_seed;              /* 0x1234ABCD in the reference (RenderSession.swift:917) */
+    uint32_t samples_per_launch;      /* samples of one pixel in flight together; 0 = auto */
+} dr_settings;
+
+/* Counters read by dr_hip_session_stats (the reference's primary, bounce
+ * and shadow scene-query counters, PathTrace.metal:201-203, plus traversal
+ * work). */
+typedef enum {
+    DR_STAT_PRIMARY_QUERIES = 0,
+    DR_STAT_BOUNCE_QUERIES = 1,
+    DR_STAT_SHADOW_QUERIES = 2,
+    DR_STAT_NODE_VISITS = 3,          /* extension + subsurface queries: node fetches (2 boxes each) */
+    DR_STAT_TRIANGLE_TESTS = 4,
+    DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
+    DR_STAT_SHADOW_TRIANGLE_TESTS = 6,
+    DR_STAT_COUNT = 8
+} dr_stat;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *dr_last_error(void);
+
+/* DenrimRenderer.init(device:) (Renderer/DenrimRenderer.swift:52-63):
+ * binds HIP device `ordinal` and checks that the gfx950 kernels load. */
+dr_status dr_hip_device_create(int ordinal, dr_device **out);
+dr_status dr_hip_device_destroy(dr_device *device);
+
+/* AccelerationBackend.build + buffer upload
+ * (Acceleration/AccelerationBackend.swift:7-67, Renderer/RenderSession.swift:371-520):
+ * copies the records into HBM and builds this backend's traversal BVH. */
+dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *desc, dr_scene **out);
+dr_status dr_hip_scene_release(dr_scene *scene);
+/* Build facts: node count, leaf count, max depth, build seconds (as float). */
+dr_status dr_hip_scene_info(const dr_scene *scene, uint32_t *node_count, uint32_t *leaf_count,
+                            uint32_t *max_depth, double *build_seconds);
+
+/* DenrimRenderer.makeSession(scene:settings:accelerationMode:)
+ * (Renderer/DenrimRenderer.swift:253-292) for accelerationMode "hip".
+ * `previous_camera` may be NULL (then the current camera). */
+dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *settings, const dr_gpu_camera *camera,
+                                const dr_gpu_camera *previous_camera, dr_session **out);
+dr_status dr_hip_session_destroy(dr_session *session);
+
+/* RenderSession.render(samples:) (Renderer/RenderSession.swift:1010-1017):
+ * renders `samples` more progressive samples and waits for them. */
+dr_status dr_hip_session_render(dr_session *session, uint32_t samples);
+
+/* RenderSession.encodeNextSample(into:) (Renderer/RenderSession.swift:854-860):
+ * enqueues `samples` samples on the session stream without waiting. */
+dr_status dr_hip_session_encode(dr_session *session, uint32_t samples);
+/* commandBuffer.waitUntilCompleted() (Renderer/RenderSession.swift:841-847). */
+dr_status dr_hip_session_synchronize(dr_session *session);
+
+/* RenderSession.renderNextTile(_:completesSample:) (Renderer/RenderSession.swift:876-889). */
+dr_status dr_hip_session_render_tile(dr_session *session, uint32_t x, uint32_t y, uint32_t width, uint32_t height,
+                                     int completes_sample);
+
+/* RenderSession.resetAccumulation() (Renderer/RenderSession.swift:582-588). */
+dr_status dr_hip_session_reset(dr_session *session);
+
+/* RenderSession.updateCamera(_:previousCamera:) (Renderer/RenderSession.swift:595-608);
+ * `previous_camera` NULL means the camera active before the call. */
+dr_status dr_hip_session_update_camera(dr_session *session, const dr_gpu_camera *camera,
+                                       const dr_gpu_camera *previous_camera);
+
+/* RenderSession.sampleCount (Renderer/RenderSession.swift:227). */
+dr_status dr_hip_session_sample_count(const dr_session *session, uint32_t *out);
+
+/* RenderSession.pixels(for:) (Renderer/RenderSession.swift:1037-1039):
+ * RGBA32F, row-major, row 0 at the top; `floats` must be >= width*height*4. */
+dr_status dr_hip_session_read_output(dr_session *session, dr_output kind, float *rgba, size_t floats);
+
+/* RenderSession.sdfTraversalStats() (Renderer/RenderSession.swift:807-831):
+ * counters accumulated since creation or the last reset_stats. */
+dr_status dr_hip_session_stats(dr_session *session, uint64_t *counters, size_t count);
+dr_status dr_hip_session_reset_stats(dr_session *session);
+
+/* GPU time of the session's work, measured with HIP events on the session
+ * stream: `total_ms` of the last render/encode call; per-stage sums since the
+ * last reset_stats (raygen, extend, shade, shadow, resolve, finalize) when
+ * stage timing is enabled. */
+dr_status dr_hip_session_set_stage_timing(dr_session *session, int enabled);
+dr_status dr_hip_session_timing(dr_session *session, double *total_ms, double *stage_ms, uint64_t *stage_launches,
+                                size_t stages);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DENRIM_HIP_H */

@@ -1,0 +1,130 @@
+/*
+ * oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a plain C++ restatement of the
+ * reference progressive path tracer (pathTraceKernel and its callees in
+ * Sources/DenrimRendererKit/Resources/Shaders/PathTrace.metal, plus the
+ * count-median BVH of Acceleration/BVHBuilder.swift). Only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg may load it, and only as the
+ * checker; the product never links it.
+ *
+ * Struct layouts below restate Sources/DenrimRendererKit/Metal/GPUTypes.swift
+ * (GPUTriangle :12, GPUMaterial :30, GPULightRecord, GPUEnvironmentSample,
+ * GPUTextureDescriptor, GPUCamera :4) byte for byte, because the oracle
+ * consumes the same flat arrays as the reference kernel.
+ */
+#ifndef DENRIM_ORACLE_H
+#define DENRIM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y, z, w; } or_float4;
+typedef struct { uint32_t x, y, z, w; } or_uint4;
+
+typedef struct {
+    or_float4 origin, lowerLeft, horizontal, vertical, lens;
+} or_camera;                                  /* 80 B */
+
+typedef struct {
+    or_float4 v0, v1, v2, n0, n1, n2, uv0, uv1, uv2, tangent, bitangent;
+    uint32_t materialID, objectID, primitiveID, padding2;
+} or_triangle;                                /* 192 B */
+
+typedef struct {
+    or_float4 baseColor, emission, parameters, parameters2, specularColor,
+        sheenColor, transmissionColor, parameters3, clearcoatColor,
+        clearcoatAttenuation, transmissionAbsorption, thinFilm,
+        subsurfaceColor, subsurfaceRadius, subsurfaceParameters,
+        volumeScattering, volumeParameters;
+} or_material;                                /* 272 B */
+
+typedef struct {
+    uint32_t triangleIndex, materialIndex;
+    float area, selectionCDF;
+    or_float4 normal;
+} or_light;                                   /* 32 B */
+
+typedef struct { float cdf, pdf; } or_env_sample;   /* 8 B */
+typedef struct { uint32_t offset, width, height, mode; } or_texture_desc; /* 16 B */
+
+/* The reference's per-dispatch constants that the triangle path reads
+ * (GPURenderConstants, PathTrace.metal:150-189). */
+typedef struct {
+    uint32_t width, height;
+    uint32_t maxBounces;
+    uint32_t frameSeed;
+    uint32_t transparentBackground;
+    uint32_t showsEnvironmentBackground;
+    uint32_t environmentTextureIndexPlusOne;
+    uint32_t denoiserEnabled;
+    float environmentIntensity;
+    float environmentRotationY;
+    float environmentMaxRadiance;
+    float sampleRadianceClamp;
+    float backgroundColor[4];
+    uint32_t tileX, tileY, tileWidth, tileHeight;
+} or_settings;
+
+typedef struct {
+    const or_triangle *triangles; uint32_t triangleCount;
+    const or_material *materials; uint32_t materialCount;
+    const or_light *lights; uint32_t lightCount;
+    const or_env_sample *envSamples; uint32_t envSampleCount;
+    const or_texture_desc *textureDescs; uint32_t textureCount;
+    const or_float4 *texturePixels; uint64_t texturePixelCount;
+} or_scene_desc;
+
+typedef struct or_scene or_scene;
+
+/* Counter slots written by or_render (traversal work, for the roofline). */
+enum {
+    OR_CNT_PRIMARY_QUERIES = 0,
+    OR_CNT_BOUNCE_QUERIES = 1,
+    OR_CNT_SHADOW_QUERIES = 2,
+    OR_CNT_NODE_VISITS = 3,
+    OR_CNT_TRIANGLE_TESTS = 4,
+    OR_CNT_COUNT = 8
+};
+
+/* Scene: copies the arrays and builds the count-median BVH. */
+int or_scene_create(const or_scene_desc *desc, uint32_t maxLeaf, or_scene **out);
+void or_scene_destroy(or_scene *scene);
+uint32_t or_scene_node_count(const or_scene *scene);
+/* Flat BVH export in the reference layout: nodes as 12 x u32 (48 B), indices u32. */
+int or_scene_export_bvh(const or_scene *scene, void *nodes48, uint32_t *primitiveIndices);
+
+/*
+ * Render `sampleCount` progressive samples starting at `sampleBegin` into the
+ * caller's buffers, following pathTraceKernel (PathTrace.metal:5167-5586).
+ *   precision 32: accumulation/aovs are float[W*H*4]; 64: double[W*H*4].
+ *   traversal 0: flat BVH; 1: brute force over all triangles.
+ *   aovs: 7 planes of W*H*4 (depth, normal, albedo, materialID, objectID,
+ *         motion, triangleID) in the caller's precision; may be NULL.
+ *   counters: OR_CNT_COUNT u64 (may be NULL), accumulated.
+ *   threads: OpenMP threads (0 = default).
+ */
+int or_render(const or_scene *scene, const or_settings *settings,
+              const or_camera *camera, const or_camera *previousCamera,
+              uint32_t sampleBegin, uint32_t sampleCount, int precision,
+              int traversal, void *accumulation, void *aovs,
+              uint64_t *counters, int threads);
+
+/* Closest-hit queries for arbitrary rays (fp32 path): rays are 8 floats
+ * (ox,oy,oz,_,dx,dy,dz,_); out: t (float), triangle index (u32, ~0 = miss). */
+int or_intersect(const or_scene *scene, const float *rays, uint32_t count,
+                 int traversal, float *tOut, uint32_t *idOut);
+
+/* RNG restatement (PathTrace.metal:264-276). */
+uint32_t or_hash(uint32_t value);
+float or_random_float(uint32_t *state);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
