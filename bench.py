@@ -112,7 +112,8 @@ def main():
     c = cs.stats().astype(np.float64)
     cs.close()
     queries_per_step = c[0] + c[1] + c[2]
-    log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f}, shadow {c[2]:.0f}), "
+    log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
+        f"shadow {c[2]:.0f}), "
         f"node fetches {c[3]:.0f} + shadow {c[5]:.0f}, tri tests {c[4]:.0f} + shadow {c[6]:.0f}")
 
     sess = renderer.make_session(gscene, settings, camera=scene.camera, samples_per_launch=1)

@@ -80,6 +80,7 @@ struct dr_scene {
     DrMaterial *materials = nullptr;
     DrLight *lights = nullptr;
     float2 *env = nullptr;
+    uint32_t *envGuide = nullptr;
     uint4 *texDesc = nullptr;
     float4 *texels = nullptr;
     dr::BvhBuildStats stats;
@@ -144,6 +145,7 @@ static void freeScene(dr_scene *s) {
     devFree(s->materials);
     devFree(s->lights);
     devFree(s->env);
+    devFree(s->envGuide);
     devFree(s->texDesc);
     devFree(s->texels);
 }
@@ -189,6 +191,24 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
             opaqueOnly = false;
     }
 
+    // Guide table of the HDRI CDF (G a power of two, about 8 texels per slot).
+    std::vector<uint32_t> guide;
+    uint32_t guideSize = 0;
+    if (d->environment_sample_count > 1) {
+        const uint32_t n = d->environment_sample_count;
+        guideSize = 1;
+        while (guideSize < (n >> 3) && guideSize < (1u << 20)) guideSize <<= 1;
+        const auto *env = static_cast<const float2 *>(d->environment_samples);
+        guide.resize(size_t(guideSize) + 1);
+        uint32_t i = 0;
+        for (uint32_t k = 0; k <= guideSize; ++k) {
+            float threshold = float(k) / float(guideSize);
+            while (i < n - 1 && !(threshold <= env[i].x)) ++i;
+            guide[k] = i;
+        }
+        guide[guideSize] = n - 1;
+    }
+
     hipError_t e = hipSuccess;
     auto chain = [&](hipError_t x) {
         if (e == hipSuccess) e = x;
@@ -199,6 +219,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     chain(upload(&s->materials, d->materials, d->material_count));
     chain(upload(&s->lights, d->lights, d->light_count));
     chain(upload(&s->env, d->environment_samples, d->environment_sample_count));
+    chain(upload(&s->envGuide, guide.data(), guide.size()));
     chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count));
     chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count));
     if (e != hipSuccess) {
@@ -214,6 +235,8 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     g.materials = s->materials;
     g.lights = s->lights;
     g.env = s->env;
+    g.envGuide = s->envGuide;
+    g.envGuideSize = guideSize;
     g.texDesc = s->texDesc;
     g.texels = s->texels;
     g.triangleCount = d->triangle_count;

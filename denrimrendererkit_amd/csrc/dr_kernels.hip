@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths
             float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
             DrRayPre r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
             DrHitRec h{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
-            dr_traverse<false>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+            dr_traverse<false>(S, r, h, stk, nv, nt);
             P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
         }
     }
@@ -198,15 +198,16 @@ __device__ __forceinline__ float boundaryPDF(vec3 sigmaT, float d) {
 
 // Closest hit used by the nested queries of the subsurface walk.
 __device__ __forceinline__ bool traceClosest(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, DrHitRec &h,
-                                             uint32_t *nv, uint32_t *nt) {
+                                             uint32_t &nv, uint32_t &nt) {
     DrRayPre r = dr_prep(o, d);
     h = DrHitRec{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
     return dr_traverse<false>(S, r, h, stk, nv, nt);
 }
 
-__device__ __noinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, const Surf &entry, const Mat &m, vec3 &ro,
+__device__ __forceinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, const Surf &entry, const Mat &m, vec3 &ro,
                                                           vec3 &rd, vec3 &T, float &prevPdf, uint32_t &state,
-                                                          DrStack &stk, uint32_t *cnt) {  // :3525-3621
+                                                          DrStack &stk, uint32_t &cBounce, uint32_t &cNodes,
+                                                          uint32_t &cTris, uint32_t &cSss) {  // :3525-3621
     float subsurface = materialSubsurface(m);
     if (subsurface <= 0.0f || !entry.frontFacing) return false;
     float e0 = randomFloat(state);
@@ -218,9 +219,10 @@ __device__ __noinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, c
     vec3 albedo = materialSubsurfaceColor(m);
     float g = materialSubsurfaceAnisotropy(m);
     for (uint32_t step = 0u; step < 8u; ++step) {
-        cnt[DR_CNT_BOUNCE]++;
+        cBounce++;
+        cSss++;
         DrHitRec hr;
-        if (!traceClosest(S, mo, md, stk, hr, &cnt[DR_CNT_NODES], &cnt[DR_CNT_TRIS])) return false;
+        if (!traceClosest(S, mo, md, stk, hr, cNodes, cTris)) return false;
         Surf b = surfaceAt(S, mo, md, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
         float f0 = randomFloat(state);
         float f1 = randomFloat(state);
@@ -275,7 +277,7 @@ __device__ __forceinline__ bool continueTransmissionVolumeRandomWalk(const Surf 
     return maxComponent(T) > 1e-4f;
 }
 
-__device__ __noinline__ bool continueTransmissiveSurface(const Surf &h, const Mat &m, vec3 &ro, vec3 &rd, vec3 &T,
+__device__ __forceinline__ bool continueTransmissiveSurface(const Surf &h, const Mat &m, vec3 &ro, vec3 &rd, vec3 &T,
                                                          float &prevPdf, uint32_t &state) {  // :1723-1799
     float transmission = materialTransmission(m);
     if (transmission <= 0.0f) return false;
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     if (qi >= count) return;  // whole waves past the end leave together
     uint32_t gl = qi;
     DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + gl, P.overflowStride};
-    uint32_t cnt[DR_CNT_SLOTS] = {0};
+    uint32_t cPrimary = 0u, cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
 
     const uint32_t slot = queue[qi];
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
@@ -367,11 +369,12 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     const uint32_t pix = meta.x;
     const float clampValue = L.sampleRadianceClamp;
     bool alive = false;
-    cnt[bounce == 0u ? DR_CNT_PRIMARY : DR_CNT_BOUNCE]++;
+    if (bounce == 0u) cPrimary++;
+    else cBounce++;
 
-    float4 nee0 = make_float4(0, 0, 0, 0), nee1 = nee0, nee2 = nee0;  // pre.xyz, valid
-    float4 neeO[3], neeD[3];
-    bool neeValid[3] = {false, false, false};
+    // NEE shadow records are written straight to the slot's three records;
+    // neeMask holds which of them are valid.
+    uint32_t neeMask = 0u;
 
     const uint32_t id = __float_as_uint(h4.w);
     if (id == 0xffffffffu) {
@@ -429,7 +432,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                 if (subsurface > 0.0f) {
                     float gate = randomFloat(state);
                     if (gate < subsurface &&
-                        continueSubsurfaceRandomWalk(S, h, m, ro, rd, T, prevPdf, state, stk, cnt)) {
+                        continueSubsurfaceRandomWalk(S, h, m, ro, rd, T, prevPdf, state, stk, cBounce, cNodes, cTris, cSss)) {
                         handled = true;
                         alive = !terminatePathWithRussianRoulette(bounce, T, state);
                     }
@@ -482,14 +485,15 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                                 vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, ldir);
                                 vec3 pre = brdf * emission * G * mis;
                                 vec3 so = h.position + h.normal * 0.001f;
-                                neeO[0] = make_float4(so.x, so.y, so.z, dist - 0.002f);
-                                neeD[0] = make_float4(ldir.x, ldir.y, ldir.z, selectionPDF);
-                                nee0 = make_float4(pre.x, pre.y, pre.z, 1.0f);
-                                neeValid[0] = true;
+                                P.shO[slot * 3u] = make_float4(so.x, so.y, so.z, dist - 0.002f);
+                                P.shD[slot * 3u] = make_float4(ldir.x, ldir.y, ldir.z, selectionPDF);
+                                P.shC[slot * 3u] = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                                neeMask |= 1u;
                             }
                         }
                     }
                 }
+#pragma unroll
                 for (uint32_t e = 0; e < 2u; ++e) {
                     float px = randomFloat(state);
                     float py = randomFloat(state);
@@ -504,14 +508,14 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                         vec3 env = sampleEnvironment(S, dir);
                         vec3 pre = brdf * env * cosS * mis;
                         vec3 so = h.position + h.normal * 0.001f;
-                        neeO[1 + e] = make_float4(so.x, so.y, so.z, 1.0e20f);
-                        neeD[1 + e] = make_float4(dir.x, dir.y, dir.z, envPDF * 2.0f);
-                        float4 rec = make_float4(pre.x, pre.y, pre.z, 1.0f);
-                        if (e == 0) nee1 = rec; else nee2 = rec;
-                        neeValid[1 + e] = true;
+                        uint32_t k = slot * 3u + 1u + e;
+                        P.shO[k] = make_float4(so.x, so.y, so.z, 1.0e20f);
+                        P.shD[k] = make_float4(dir.x, dir.y, dir.z, envPDF * 2.0f);
+                        P.shC[k] = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                        neeMask |= 2u << e;
                     }
                 }
-                if (neeValid[0] || neeValid[1] || neeValid[2]) {
+                if (neeMask != 0u) {
                     flags |= DR_FLAG_NEE;
                     P.neeT[slot] = make_float4(T.x, T.y, T.z, 0.0f);
                 }
@@ -579,24 +583,19 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
 
     uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
     if (alive) queueOut[pos] = slot;
-    for (int j = 0; j < 3; ++j) {
-        bool v = neeValid[j];
+#pragma unroll
+    for (uint32_t j = 0; j < 3u; ++j) {
+        bool v = (neeMask >> j) & 1u;
         uint32_t sp = waveAppend(v, &P.counters[DR_CTR_SHADOWQ + bounce]);
-        if (v) {
-            uint32_t k = slot * 3u + (uint32_t)j;
-            P.shO[k] = neeO[j];
-            P.shD[k] = neeD[j];
-            P.shC[k] = j == 0 ? nee0 : (j == 1 ? nee1 : nee2);
-            P.shadowQueue[sp] = k;
-        } else {
-            P.shC[slot * 3u + (uint32_t)j].w = 0.0f;
-        }
+        if (v) P.shadowQueue[sp] = slot * 3u + j;
+        else P.shC[slot * 3u + j].w = 0.0f;
     }
     if (L.statsEnabled) {
-        waveAddStat(P.stats, DR_CNT_PRIMARY, cnt[DR_CNT_PRIMARY]);
-        waveAddStat(P.stats, DR_CNT_BOUNCE, cnt[DR_CNT_BOUNCE]);
-        waveAddStat(P.stats, DR_CNT_NODES, cnt[DR_CNT_NODES]);
-        waveAddStat(P.stats, DR_CNT_TRIS, cnt[DR_CNT_TRIS]);
+        waveAddStat(P.stats, DR_CNT_PRIMARY, cPrimary);
+        waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
+        waveAddStat(P.stats, DR_CNT_NODES, cNodes);
+        waveAddStat(P.stats, DR_CNT_TRIS, cTris);
+        waveAddStat(P.stats, DR_CNT_SSS, cSss);
     }
 }
 
@@ -627,15 +626,20 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
             ++nq;
             DrRayPre r = dr_prep(o, d);
             DrHitRec h{maxDistance, 0.0f, 0.0f, 0u};  // id 0: the limit is exclusive
-            occluded = dr_traverse<true>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+            occluded = dr_traverse<true>(S, r, h, stk, nv, nt);
         } else {
             occluded = true;
             float traveled = 0.0f;
             for (uint32_t step = 0; step < 8u; ++step) {
                 ++nq;
                 DrRayPre r = dr_prep(o, d);
-                DrHitRec h{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
-                bool hit = dr_traverse<false>(S, r, h, stk, statsOn ? &nv : nullptr, statsOn ? &nt : nullptr);
+                // Hits at or beyond maxDistance - traveled end the chain, so
+                // the search is limited there: exactly (exclusive limit) on
+                // the first step, with a relative margin afterwards where the
+                // reference test traveled + t >= maxDistance decides.
+                float limit = step == 0u ? maxDistance : (maxDistance - traveled) * 1.000001f + 1e-30f;
+                DrHitRec h{limit, 0.0f, 0.0f, step == 0u ? 0u : 0xffffffffu};
+                bool hit = dr_traverse<false>(S, r, h, stk, nv, nt);
                 if (!hit || traveled + h.t >= maxDistance) {
                     occluded = false;
                     break;
@@ -700,7 +704,8 @@ __global__ void k_resolve(DrLaunch L, DrPaths P, uint32_t bounce) {
 __device__ bool tracePrimaryAOV(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, Surf &out, Mat &outM) {
     for (uint32_t pass = 0; pass < 32u; ++pass) {
         DrHitRec hr;
-        if (!traceClosest(S, o, d, stk, hr, nullptr, nullptr)) return false;
+        uint32_t nv = 0u, nt = 0u;
+        if (!traceClosest(S, o, d, stk, hr, nv, nt)) return false;
         Surf h = surfaceAt(S, o, d, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
         Mat m = materialFor(S, h.materialID);
         applyNormalMap(S, h, m, d);

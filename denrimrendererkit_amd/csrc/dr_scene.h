@@ -59,9 +59,10 @@ struct DrSceneDev {
     const DrMaterial *materials;
     const DrLight *lights;
     const float2 *env;           // (cdf, pdf) per environment texel
+    const uint32_t *envGuide;    // envGuideSize + 1 entries: first texel with cdf >= k / envGuideSize
     const uint4 *texDesc;        // (offset, width, height, mode)
     const float4 *texels;
-    uint32_t triangleCount, materialCount, lightCount, envCount, texCount;
+    uint32_t triangleCount, materialCount, lightCount, envCount, texCount, envGuideSize;
     uint32_t envTexIndexPlusOne;
     float envIntensity, envRotationY, envMaxRadiance;
     uint32_t opaqueOnly;         // 1: no material can let a shadow ray through
@@ -98,5 +99,6 @@ enum {
     DR_CNT_TRIS = 4,
     DR_CNT_SHADOW_NODES = 5,
     DR_CNT_SHADOW_TRIS = 6,
+    DR_CNT_SSS = 7,  // subsurface-walk queries (also counted in BOUNCE)
     DR_CNT_SLOTS = 8
 };

@@ -251,7 +251,7 @@ __device__ __forceinline__ vec3 materialClearcoatAttenuationForDirection(const M
     vec3 fa = vmax(v3s(0.0f), v3s(1.0f) - ccF * clearcoat);
     return fa * materialClearcoatDepthAttenuation(m, n, v, l);
 }
-__device__ __noinline__ vec3 evaluateMaterialBRDF(const Mat &m, vec3 n, vec3 t, vec3 b, vec3 v, vec3 l) {  // :1330-1390
+__device__ __forceinline__ vec3 evaluateMaterialBRDF(const Mat &m, vec3 n, vec3 t, vec3 b, vec3 v, vec3 l) {  // :1330-1390
     const float inversePi = 0.31830988618f;
     float roughness = clampf(ms<M_PARAMS>(m).x, 0.02f, 1.0f);
     float metallic = matMetallic(m);
@@ -310,7 +310,7 @@ __device__ __forceinline__ float ggxReflectionPDF(vec3 n, vec3 t, vec3 b, vec3 v
     return distributionGGXAnisotropic(h, n, f.t, f.b, roughness, anisotropy) *
            geometrySmithG1GGXAnisotropic(v, n, f.t, f.b, roughness, anisotropy) / fmaxf(4.0f * nDotV, 1e-5f);
 }
-__device__ __noinline__ float materialBSDFPDF(const Mat &m, vec3 n, vec3 t, vec3 b, vec3 v, vec3 l) {  // :1450-1495
+__device__ __forceinline__ float materialBSDFPDF(const Mat &m, vec3 n, vec3 t, vec3 b, vec3 v, vec3 l) {  // :1450-1495
     const float inversePi = 0.31830988618f;
     float nDotL = fmaxf(dot(n, l), 0.0f), nDotV = fmaxf(dot(n, v), 0.0f);
     if (nDotL <= 0.0f || nDotV <= 0.0f) return 0.0f;
@@ -353,7 +353,7 @@ __device__ __forceinline__ vec3 sampleGGXHalfVector(float ux, float uy, vec3 n, 
     float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
     return orientHemisphere(vec3{cosf(phi) * sinT, sinf(phi) * sinT, cosT}, n);
 }
-__device__ __noinline__ vec3 sampleGGXVisibleNormal(float ux, float uy, vec3 n, vec3 t, vec3 b, vec3 v, float roughness,
+__device__ __forceinline__ vec3 sampleGGXVisibleNormal(float ux, float uy, vec3 n, vec3 t, vec3 b, vec3 v, float roughness,
                                                     float anisotropy) {  // :1625-1664
     Frame f = makeTangentFrame(n, t, b);
     vec2 alpha = anisotropicGGXAlpha(roughness, anisotropy);
@@ -419,7 +419,7 @@ __device__ __forceinline__ vec3 fallbackSky(vec3 d) {  // :3765-3768
     float sky = 0.5f * (d.y + 1.0f);
     return mixv(vec3{0.02f, 0.025f, 0.035f}, vec3{0.25f, 0.32f, 0.45f}, sky);
 }
-__device__ __noinline__ vec3 sampleEnvironment(const DrSceneDev &S, vec3 d) {  // :3770-3803
+__device__ __forceinline__ vec3 sampleEnvironment(const DrSceneDev &S, vec3 d) {  // :3770-3803
     float intensity = fmaxf(S.envIntensity, 0.0f);
     if (S.envTexIndexPlusOne == 0u) return fallbackSky(d) * intensity;
     float s = sinf(S.envRotationY), c = cosf(S.envRotationY);
@@ -459,11 +459,21 @@ __device__ __forceinline__ vec3 sampleEnvironmentDirection(const DrSceneDev &S, 
     uint4 td = S.texDesc[S.envTexIndexPlusOne - 1u];
     uint32_t width = td.y, height = td.z;
     if (width == 0u || height == 0u) return v3(0, 1, 0);
-    // CDF binary search (:3929-3950).
+    // CDF inversion (:3929-3950): the lower bound of the target in the CDF.
+    // A guide table over target space brackets it -- guide[k] is the first
+    // texel whose cdf >= k/G, so the answer lies in [guide[k], guide[k+1]]
+    // for k = floor(target * G) -- and the reference's binary search runs
+    // only inside that bracket. G is a power of two, so target * G and k / G
+    // are exact and the index is the one the full search returns.
     uint32_t index = 0u;
     if (S.envCount > 1u) {
         float target = clampf(cdfSample, 0.0f, 0.99999994f);
         uint32_t lo = 0u, hi = S.envCount - 1u;
+        if (S.envGuideSize > 0u) {
+            uint32_t k = min((uint32_t)(target * (float)S.envGuideSize), S.envGuideSize - 1u);
+            lo = S.envGuide[k];
+            hi = S.envGuide[k + 1u];
+        }
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1u;
             if (target <= S.env[mid].x) hi = mid; else lo = mid + 1u;

@@ -73,17 +73,28 @@ __device__ __forceinline__ float dr_box(const DrRayPre &r, float x0, float y0, f
 
 // Moeller-Trumbore with precomputed edges (e1 = v1 - v0, e2 = v2 - v0 are
 // computed on the host with the same float subtraction the reference does).
+// The IEEE reciprocal is the expensive step, so the sign tests run first on
+// the numerators: u = inv * un is negative exactly when un and det differ in
+// sign and the product does not underflow to -0; only rejections that are
+// certain (|un| > 2^-100 |det|) are taken early, everything else falls
+// through to the reference arithmetic, so the accepted set and the values of
+// u, v, t are unchanged.
 __device__ __forceinline__ bool dr_tri(const DrRayPre &r, float4 A, float4 B, float4 C, float &t, float &u, float &v) {
     vec3 v0 = v3(A.x, A.y, A.z), e1 = v3(B.x, B.y, B.z), e2 = v3(C.x, C.y, C.z);
     vec3 p = cross(r.d, e2);
     float det = dot(e1, p);
     if (fabsf(det) < 1e-7f) return false;
-    float inv = 1.0f / det;
     vec3 s = r.o - v0;
-    float uu = inv * dot(s, p);
-    if (uu < 0.0f || uu > 1.0f) return false;
+    float un = dot(s, p);
+    const float tiny = 7.8886091e-31f * fabsf(det);  // 2^-100 |det|
+    if ((un < 0.0f) != (det < 0.0f) && fabsf(un) > tiny) return false;
     vec3 q = cross(s, e1);
-    float vv = inv * dot(r.d, q);
+    float vn = dot(r.d, q);
+    if ((vn < 0.0f) != (det < 0.0f) && fabsf(vn) > tiny) return false;
+    float inv = 1.0f / det;
+    float uu = inv * un;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    float vv = inv * vn;
     if (vv < 0.0f || uu + vv > 1.0f) return false;
     float tt = inv * dot(e2, q);
     if (tt <= 0.0005f) return false;
@@ -102,9 +113,10 @@ struct DrHitRec {
 // On entry best.t is the search limit; best.id must be 0xffffffff for a
 // closest-hit query and the result is the lowest-index triangle among those
 // with minimal t < limit.
+// Node fetches and triangle tests are added to nodeVisits / triTests.
 template <bool ANY>
 __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre &r, DrHitRec &best, DrStack &stk,
-                                            uint32_t *nodeVisits, uint32_t *triTests) {
+                                            uint32_t &nodeVisits, uint32_t &triTests) {
     if (S.triangleCount == 0) return false;
     uint32_t sp = 0;
     uint32_t cur = 0;
@@ -138,8 +150,8 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
                         best.id = id;
                         found = true;
                         if (ANY) {
-                            if (nodeVisits) *nodeVisits += nv;
-                            if (triTests) *triTests += nt;
+                            nodeVisits += nv;
+                            triTests += nt;
                             return true;
                         }
                     }
@@ -162,7 +174,7 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
             cur = stk.get(--sp);
         }
     }
-    if (nodeVisits) *nodeVisits += nv;
-    if (triTests) *triTests += nt;
+    nodeVisits += nv;
+    triTests += nt;
     return found;
 }

@@ -115,6 +115,7 @@ typedef enum {
     DR_STAT_TRIANGLE_TESTS = 4,
     DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
     DR_STAT_SHADOW_TRIANGLE_TESTS = 6,
+    DR_STAT_SUBSURFACE_QUERIES = 7,   /* random-walk queries, also in BOUNCE_QUERIES */
     DR_STAT_COUNT = 8
 } dr_stat;
 
