@@ -23,6 +23,7 @@ EXPORTS = (
     "dr_hip_session_render_tile", "dr_hip_session_reset", "dr_hip_session_update_camera",
     "dr_hip_session_sample_count", "dr_hip_session_read_output", "dr_hip_session_stats",
     "dr_hip_session_reset_stats", "dr_hip_session_set_stage_timing", "dr_hip_session_timing",
+    "dr_sdf_build_sparse", "dr_sdf_volume_info", "dr_sdf_volume_export", "dr_sdf_volume_free",
 )
 
 
@@ -37,7 +38,23 @@ class SceneDesc(C.Structure):
         ("environment_texture_index_plus_one", C.c_uint32),
         ("environment_intensity", C.c_float), ("environment_rotation_y", C.c_float),
         ("environment_max_radiance", C.c_float), ("max_leaf_triangles", C.c_uint32),
+        ("volumes", C.c_void_p), ("volume_count", C.c_uint32),
+        ("volume_bricks", C.c_void_p), ("volume_brick_count", C.c_uint32),
+        ("volume_brick_samples", C.c_void_p), ("volume_brick_sample_count", C.c_uint32),
+        ("volume_brick_grids", C.c_void_p), ("volume_brick_grid_count", C.c_uint32),
+        ("volume_brick_grid_indices", C.c_void_p), ("volume_brick_grid_index_count", C.c_uint32),
     ]
+
+
+class SdfPrimitive(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("material", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float),
+                ("axis", C.c_float * 3), ("minor_radius", C.c_float)]
+
+
+class SdfBuildSettings(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 3), ("brick_size", C.c_uint32), ("apron", C.c_uint32),
+                ("narrow_band", C.c_float), ("smooth_radius", C.c_float), ("bounds_min", C.c_float * 3),
+                ("bounds_max", C.c_float * 3)]
 
 
 class Settings(C.Structure):
@@ -94,6 +111,10 @@ def load():
     L.dr_hip_session_set_stage_timing.argtypes = [vp, i32]
     L.dr_hip_session_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_uint64), C.c_size_t]
+    L.dr_sdf_build_sparse.argtypes = [C.POINTER(SdfPrimitive), u32, C.POINTER(SdfBuildSettings), C.POINTER(vp)]
+    L.dr_sdf_volume_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.dr_sdf_volume_export.argtypes = [vp] + [vp] * 8
+    L.dr_sdf_volume_free.argtypes = [vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

@@ -1,0 +1,487 @@
+// dr_sdf.h -- sparse signed-distance brick tracing on gfx950.
+//
+// Follows the reference's sparse-volume path step for step:
+//   closestSparseVolumeBrickDDAHit  PathTrace.metal:2881-3088 (3-D DDA over
+//                                   the brick grid with 4^3 macro-cell skips)
+//   intersectVolumeBrickPrepared    :2608-2807 (back-track, sphere trace <= 192
+//                                   steps, 8 bisections, cubic polish)
+//   trilinear / Catmull-Rom gradients and volumeBrickNormal :2096-2469
+// Storage is this backend's: distances as FP16 (widened to FP32 on load, so
+// tracing is FP32), one u32 material word per sample (A | B << 8 |
+// round(blend * 65535) << 16), and a 64-B brick header. With FP16-exact
+// input the values seen here equal the reference-layout FP32 values, and the
+// arithmetic (only + - * / floor round sqrt) is IEEE, so hits are bit-exact
+// with an FP32 evaluation of the reference over the same field.
+#pragma once
+
+#include "dr_math.h"
+#include "dr_scene.h"
+
+// Compact brick header (64 B).
+struct DrBrick {
+    float4 smin;  // sampleBoundsMin.xyz, min distance
+    float4 smax;  // sampleBoundsMax.xyz, max distance
+    float4 cmin;  // core localBoundsMin.xyz, bits(dims.x | dims.y << 8 | dims.z << 16)
+    float4 cmax;  // core localBoundsMax.xyz, bits(sample offset)
+};
+static_assert(sizeof(DrBrick) == 64, "brick header");
+
+// GPUVolumeDescriptor (PathTrace.metal:74-90).
+struct DrVolume {
+    float4 worldBoundsMin, worldBoundsMax, localBoundsMin, localBoundsMax;
+    uint4 dimensions, metadata, materialProgram;
+    float4 w2l[4];
+    float4 nrm[4];
+};
+static_assert(sizeof(DrVolume) == 240, "GPUVolumeDescriptor layout");
+
+// GPUVolumeBrickGrid (PathTrace.metal:143-148).
+struct DrBrickGrid {
+    uint4 dimensionsAndIndexOffset, brickSizeAndVolume, macroDimensionsAndIndexOffset, macroSizeAndReserved;
+};
+
+struct DrSdfDev {
+    const DrVolume *volumes;
+    const DrBrick *bricks;
+    const uint16_t *dist;       // FP16 bits
+    const uint32_t *material;   // A | B << 8 | q16 << 16
+    const DrBrickGrid *grids;
+    const uint32_t *gridIndices;
+    uint32_t volumeCount, brickCount, sampleCount, gridCount, gridIndexCount;
+    uint32_t renderQuality;     // 2 = final
+};
+
+// SDF counter slots in the session stats (reference slots 2..9 at 8..15).
+enum {
+    DR_SC_GRID_CELLS = 8, DR_SC_MACRO_SKIPS, DR_SC_BRICK_TESTS, DR_SC_BRICK_INVALID, DR_SC_RANGE_CULLS,
+    DR_SC_MARCHES, DR_SC_MARCH_STEPS, DR_SC_BRICK_HITS
+};
+
+struct DrSdfCounters {
+    uint32_t gridCells, macroSkips, brickTests, invalid, rangeCulls, marches, marchSteps, hits;
+};
+
+struct DrVolHit {
+    float t;
+    vec3 normal;
+    uint32_t material;  // packed material word
+    uint32_t volume;
+    bool frontFacing;
+};
+
+__device__ __forceinline__ float dr_half(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+
+__device__ __forceinline__ vec3 mulCols(const float4 *c, vec3 v, float w) {
+    return vec3{c[0].x * v.x + c[1].x * v.y + c[2].x * v.z + c[3].x * w,
+                c[0].y * v.x + c[1].y * v.y + c[2].y * v.z + c[3].y * w,
+                c[0].z * v.x + c[1].z * v.y + c[2].z * v.z + c[3].z * w};
+}
+
+__device__ __forceinline__ bool sdfAABB(vec3 o, vec3 d, vec3 bmin, vec3 bmax, float closestT, float &tNear,
+                                        float &tFar) {  // intersectAABB :1865-1881
+    vec3 inv = 1.0f / d;
+    vec3 t0 = (bmin - o) * inv;
+    vec3 t1 = (bmax - o) * inv;
+    vec3 tn = vmin(t0, t1), tf = vmax(t0, t1);
+    tNear = fmaxf(fmaxf(tn.x, tn.y), fmaxf(tn.z, 0.0005f));
+    tFar = fminf(fminf(tf.x, tf.y), fminf(tf.z, closestT));
+    return tNear <= tFar;
+}
+
+struct DrBrickCtx {
+    uint32_t dim[3];
+    uint32_t offset, strideY, strideZ;
+    vec3 smin, smax, extent;
+    bool valid;
+};
+
+__device__ __forceinline__ DrBrickCtx brickCtx(const DrSdfDev &V, const DrBrick &b) {  // :2074-2094
+    DrBrickCtx c;
+    uint32_t packed = __float_as_uint(b.cmin.w);
+    c.dim[0] = packed & 0xffu;
+    c.dim[1] = (packed >> 8) & 0xffu;
+    c.dim[2] = (packed >> 16) & 0xffu;
+    c.offset = __float_as_uint(b.cmax.w);
+    c.strideY = c.dim[0];
+    c.strideZ = c.dim[0] * c.dim[1];
+    uint32_t n = c.dim[0] * c.dim[1] * c.dim[2];
+    c.smin = v3(b.smin.x, b.smin.y, b.smin.z);
+    c.smax = v3(b.smax.x, b.smax.y, b.smax.z);
+    c.extent = vmax(c.smax - c.smin, v3s(1e-6f));
+    bool range = c.offset < V.sampleCount && n <= V.sampleCount - c.offset;
+    c.valid = c.dim[0] >= 2u && c.dim[1] >= 2u && c.dim[2] >= 2u && range;
+    return c;
+}
+
+__device__ __forceinline__ float sdAt(const DrSdfDev &V, const DrBrickCtx &c, uint32_t x, uint32_t y, uint32_t z) {
+    return dr_half(V.dist[c.offset + x + y * c.strideY + z * c.strideZ]);
+}
+
+__device__ __forceinline__ void brickGrid(const DrBrickCtx &c, vec3 p, uint32_t b[3], uint32_t n[3], vec3 &f) {
+    vec3 uvw = vclamp((p - c.smin) / c.extent, v3s(0.0f), v3s(1.0f));
+    float g[3] = {uvw.x * (float)(c.dim[0] - 1u), uvw.y * (float)(c.dim[1] - 1u), uvw.z * (float)(c.dim[2] - 1u)};
+    float fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b[a] = (uint32_t)floorf(g[a]);
+        n[a] = min(b[a] + 1u, c.dim[a] - 1u);
+        fr[a] = g[a] - (float)b[a];
+    }
+    f = v3(fr[0], fr[1], fr[2]);
+}
+
+__device__ __forceinline__ float brickDistance(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2096-2127
+    if (!c.valid) return __builtin_inff();
+    uint32_t b[3], n[3];
+    vec3 f;
+    brickGrid(c, p, b, n, f);
+    float c00 = mixf(sdAt(V, c, b[0], b[1], b[2]), sdAt(V, c, n[0], b[1], b[2]), f.x);
+    float c10 = mixf(sdAt(V, c, b[0], n[1], b[2]), sdAt(V, c, n[0], n[1], b[2]), f.x);
+    float c01 = mixf(sdAt(V, c, b[0], b[1], n[2]), sdAt(V, c, n[0], b[1], n[2]), f.x);
+    float c11 = mixf(sdAt(V, c, b[0], n[1], n[2]), sdAt(V, c, n[0], n[1], n[2]), f.x);
+    float c0 = mixf(c00, c10, f.y);
+    float c1 = mixf(c01, c11, f.y);
+    return mixf(c0, c1, f.z);
+}
+
+struct DrDG {
+    float distance;
+    vec3 gradient;
+};
+
+__device__ __forceinline__ DrDG brickDistanceGradient(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2161-2219
+    DrDG r{__builtin_inff(), v3s(0.0f)};
+    if (!c.valid) return r;
+    uint32_t b[3], n[3];
+    vec3 f;
+    brickGrid(c, p, b, n, f);
+    float d000 = sdAt(V, c, b[0], b[1], b[2]), d100 = sdAt(V, c, n[0], b[1], b[2]);
+    float d010 = sdAt(V, c, b[0], n[1], b[2]), d110 = sdAt(V, c, n[0], n[1], b[2]);
+    float d001 = sdAt(V, c, b[0], b[1], n[2]), d101 = sdAt(V, c, n[0], b[1], n[2]);
+    float d011 = sdAt(V, c, b[0], n[1], n[2]), d111 = sdAt(V, c, n[0], n[1], n[2]);
+    float c00 = mixf(d000, d100, f.x), c10 = mixf(d010, d110, f.x);
+    float c01 = mixf(d001, d101, f.x), c11 = mixf(d011, d111, f.x);
+    float c0 = mixf(c00, c10, f.y), c1 = mixf(c01, c11, f.y);
+    r.distance = mixf(c0, c1, f.z);
+    float dx = mixf(mixf(d100 - d000, d110 - d010, f.y), mixf(d101 - d001, d111 - d011, f.y), f.z);
+    float dy = mixf(mixf(d010 - d000, d110 - d100, f.x), mixf(d011 - d001, d111 - d101, f.x), f.z);
+    float dz = c1 - c0;
+    vec3 g2l = v3((float)(c.dim[0] - 1u), (float)(c.dim[1] - 1u), (float)(c.dim[2] - 1u)) / c.extent;
+    r.gradient = v3(dx, dy, dz) * g2l;
+    return r;
+}
+
+__device__ __forceinline__ float catmullRom(float p0, float p1, float p2, float p3, float t) {  // :2141-2150
+    float t2 = t * t;
+    float t3 = t2 * t;
+    return 0.5f * (2.0f * p1 + (-p0 + p2) * t + (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3) * t2 +
+                   (-p0 + 3.0f * p1 - 3.0f * p2 + p3) * t3);
+}
+__device__ __forceinline__ float catmullRomD(float p0, float p1, float p2, float p3, float t) {  // :2152-2159
+    float t2 = t * t;
+    return 0.5f * ((-p0 + p2) + 2.0f * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3) * t +
+                   3.0f * (-p0 + 3.0f * p1 - 3.0f * p2 + p3) * t2);
+}
+
+__device__ __noinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2257-2314
+    DrDG inv{__builtin_inff(), v3s(0.0f)};
+    if (!c.valid) return inv;
+    vec3 uvw = vclamp((p - c.smin) / c.extent, v3s(0.0f), v3s(1.0f));
+    float g[3] = {uvw.x * (float)(c.dim[0] - 1u), uvw.y * (float)(c.dim[1] - 1u), uvw.z * (float)(c.dim[2] - 1u)};
+    uint32_t b[3] = {(uint32_t)floorf(g[0]), (uint32_t)floorf(g[1]), (uint32_t)floorf(g[2])};
+    if (b[0] < 1u || b[1] < 1u || b[2] < 1u || b[0] + 2u >= c.dim[0] || b[1] + 2u >= c.dim[1] || b[2] + 2u >= c.dim[2])
+        return brickDistanceGradient(V, c, p);
+    vec3 f = v3(g[0] - (float)b[0], g[1] - (float)b[1], g[2] - (float)b[2]);
+    float yr[4], dyr[4], dxyr[4];
+#pragma unroll
+    for (uint32_t z = 0; z < 4; ++z) {
+        float xr[4], dxr[4];
+#pragma unroll
+        for (uint32_t y = 0; y < 4; ++y) {
+            uint32_t yy = b[1] + y - 1u, zz = b[2] + z - 1u;
+            float p0 = sdAt(V, c, b[0] - 1u, yy, zz), p1 = sdAt(V, c, b[0], yy, zz);
+            float p2 = sdAt(V, c, b[0] + 1u, yy, zz), p3 = sdAt(V, c, b[0] + 2u, yy, zz);
+            xr[y] = catmullRom(p0, p1, p2, p3, f.x);
+            dxr[y] = catmullRomD(p0, p1, p2, p3, f.x);
+        }
+        yr[z] = catmullRom(xr[0], xr[1], xr[2], xr[3], f.y);
+        dyr[z] = catmullRomD(xr[0], xr[1], xr[2], xr[3], f.y);
+        dxyr[z] = catmullRom(dxr[0], dxr[1], dxr[2], dxr[3], f.y);
+    }
+    DrDG r;
+    r.distance = catmullRom(yr[0], yr[1], yr[2], yr[3], f.z);
+    float dx = catmullRom(dxyr[0], dxyr[1], dxyr[2], dxyr[3], f.z);
+    float dy = catmullRom(dyr[0], dyr[1], dyr[2], dyr[3], f.z);
+    float dz = catmullRomD(yr[0], yr[1], yr[2], yr[3], f.z);
+    vec3 g2l = v3((float)(c.dim[0] - 1u), (float)(c.dim[1] - 1u), (float)(c.dim[2] - 1u)) / c.extent;
+    r.gradient = v3(dx, dy, dz) * g2l;
+    return r;
+}
+
+__device__ __forceinline__ float brickDistanceAt(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {
+    return brickDistance(V, c, p);
+}
+
+__device__ __noinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &vol, const DrBrickCtx &c, vec3 p,
+                                         bool cubic, bool coarse) {  // :2420-2469
+    if (!c.valid) return v3(0, 1, 0);
+    vec3 ln = cubic ? brickDistanceGradientCubic(V, c, p).gradient : brickDistanceGradient(V, c, p).gradient;
+    if (coarse) {
+        vec3 ext = vmax(xyzf(vol.localBoundsMax) - xyzf(vol.localBoundsMin), v3s(1e-6f));
+        vec3 dims = v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z);
+        vec3 sp = (ext / vmax(dims - v3s(1.0f), v3s(1.0f))) * 1.5f;
+        vec3 mn = c.smin, mx = c.smax;
+        vec3 x0 = p, x1 = p;
+        x0.x = fmaxf(p.x - sp.x, mn.x);
+        x1.x = fminf(p.x + sp.x, mx.x);
+        float dx = (brickDistance(V, c, x1) - brickDistance(V, c, x0)) / fmaxf(x1.x - x0.x, 1e-6f);
+        vec3 y0 = p, y1 = p;
+        y0.y = fmaxf(p.y - sp.y, mn.y);
+        y1.y = fminf(p.y + sp.y, mx.y);
+        float dy = (brickDistance(V, c, y1) - brickDistance(V, c, y0)) / fmaxf(y1.y - y0.y, 1e-6f);
+        vec3 z0 = p, z1 = p;
+        z0.z = fmaxf(p.z - sp.z, mn.z);
+        z1.z = fminf(p.z + sp.z, mx.z);
+        float dz = (brickDistance(V, c, z1) - brickDistance(V, c, z0)) / fmaxf(z1.z - z0.z, 1e-6f);
+        vec3 cn = v3(dx, dy, dz);
+        if (dot(cn, cn) > 1e-10f) ln = normalize(mixv(normalize(ln), normalize(cn), 0.35f));
+    }
+    if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
+    return normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
+}
+
+__device__ __noinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd, const DrVolume &vol,
+                                            const DrBrick &brick, float closestT, vec3 lo, vec3 ld, float lupw,
+                                            float cellSize, DrSdfCounters &sc, DrVolHit &hit) {  // :2608-2807
+    float tNear, tFar;
+    if (!sdfAABB(lo, ld, v3(brick.cmin.x, brick.cmin.y, brick.cmin.z), v3(brick.cmax.x, brick.cmax.y, brick.cmax.z),
+                 closestT, tNear, tFar))
+        return false;
+    DrBrickCtx ctx = brickCtx(V, brick);
+    if (!ctx.valid) {
+        sc.invalid++;
+        return false;
+    }
+    if (brick.smin.w > 0.0f || brick.smax.w < 0.0f) {
+        sc.rangeCulls++;
+        return false;
+    }
+    sc.marches++;
+    float minimumStep = fmaxf(cellSize * 0.10f / lupw, 0.0005f);
+    float prevT = tNear;
+    float prevD = brickDistance(V, ctx, lo + ld * prevT);
+    if (!isfinite(prevD)) return false;
+    float originalEntryT = prevT;
+    if (prevD <= 0.0f) {
+        float backStep = fmaxf(cellSize / lupw * 0.5f, minimumStep);
+        float searchT = prevT;
+        for (uint32_t bt = 0; bt < 12u; ++bt) {
+            searchT = fmaxf(searchT - backStep, 0.0005f);
+            float sd = brickDistance(V, ctx, lo + ld * searchT);
+            if (!isfinite(sd)) break;
+            if (sd > 0.0f) {
+                prevT = searchT;
+                prevD = sd;
+                break;
+            }
+            if (searchT <= 0.0005f) break;
+        }
+    }
+    float t = fmaxf(originalEntryT, prevT + fmaxf(fabsf(prevD) / lupw * 0.9f, minimumStep));
+    for (uint32_t step = 0; step < 192u && t <= tFar && t < closestT; ++step) {
+        sc.marchSteps++;
+        float d = brickDistance(V, ctx, lo + ld * t);
+        if (!isfinite(d)) return false;
+        bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
+        if (crossed) {
+            float lowT = prevT, highT = t, lowD = prevD;
+            float refinedT = t;
+            for (uint32_t k = 0; k < 8u; ++k) {
+                float midT = 0.5f * (lowT + highT);
+                float md = brickDistance(V, ctx, lo + ld * midT);
+                if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
+                    lowT = midT;
+                    lowD = md;
+                } else {
+                    highT = midT;
+                }
+                refinedT = 0.5f * (lowT + highT);
+            }
+            bool cubicPolish = cellSize <= 0.03f;
+            bool coarse = V.renderQuality > 0u && cellSize > 0.03f;
+            vec3 rp = lo + ld * refinedT;
+            if (cubicPolish) {
+                uint32_t polishCount = V.renderQuality > 1u ? 2u : 1u;
+                for (uint32_t k = 0; k < polishCount; ++k) {
+                    DrDG pg = brickDistanceGradientCubic(V, ctx, rp);
+                    float slope = dot(pg.gradient, ld);
+                    if (!isfinite(pg.distance) || fabsf(slope) <= 1e-6f) break;
+                    float projected = refinedT - pg.distance / slope;
+                    if (projected <= lowT || projected >= highT) break;
+                    refinedT = projected;
+                    rp = lo + ld * refinedT;
+                }
+            }
+            vec3 n = brickNormal(V, vol, ctx, rp, cubicPolish, coarse);
+            bool ff = dot(n, rd) <= 0.0f;
+            if (!ff) n = -n;
+            // Nearest sample's material (sampleVolumeBrickMaterial, :2350-2356).
+            vec3 uvw = vclamp((rp - ctx.smin) / ctx.extent, v3s(0.0f), v3s(1.0f));
+            uint32_t nx = min((uint32_t)rintf(uvw.x * (float)(ctx.dim[0] - 1u)), ctx.dim[0] - 1u);
+            uint32_t ny = min((uint32_t)rintf(uvw.y * (float)(ctx.dim[1] - 1u)), ctx.dim[1] - 1u);
+            uint32_t nz = min((uint32_t)rintf(uvw.z * (float)(ctx.dim[2] - 1u)), ctx.dim[2] - 1u);
+            hit.t = refinedT;
+            hit.normal = n;
+            hit.material = V.material[ctx.offset + nx + ny * ctx.strideY + nz * ctx.strideZ];
+            hit.frontFacing = ff;
+            sc.hits++;
+            return true;
+        }
+        prevT = t;
+        prevD = d;
+        t += fmaxf(fabsf(d) / lupw * 0.8f, minimumStep);
+    }
+    return false;
+}
+
+// Closest sparse-volume hit below closestT (closestVolumeHit, brick-grid path).
+__device__ __noinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
+                                              DrVolHit &out) {
+    bool any = false;
+    float best = closestT;
+    if (V.gridCount == 0u || V.gridIndexCount == 0u) return false;
+    for (uint32_t vi = 0; vi < V.volumeCount; ++vi) {
+        const DrVolume &vol = V.volumes[vi];
+        if (vol.metadata.w != 1u) continue;
+        const DrBrickGrid *grid = nullptr;
+        for (uint32_t g = 0; g < V.gridCount; ++g)
+            if (V.grids[g].brickSizeAndVolume.w == vi) {
+                grid = &V.grids[g];
+                break;
+            }
+        if (!grid) continue;
+        uint32_t gd[3] = {grid->dimensionsAndIndexOffset.x, grid->dimensionsAndIndexOffset.y,
+                          grid->dimensionsAndIndexOffset.z};
+        uint32_t gio = grid->dimensionsAndIndexOffset.w;
+        uint32_t bs[3] = {max(grid->brickSizeAndVolume.x, 1u), max(grid->brickSizeAndVolume.y, 1u),
+                          max(grid->brickSizeAndVolume.z, 1u)};
+        if (gd[0] == 0u || gd[1] == 0u || gd[2] == 0u) continue;
+        float tNear, tFar;
+        if (!sdfAABB(ro, rd, xyzf(vol.worldBoundsMin), xyzf(vol.worldBoundsMax), best, tNear, tFar)) continue;
+        vec3 lo = mulCols(vol.w2l, ro, 1.0f);
+        vec3 ld = mulCols(vol.w2l, rd, 0.0f);
+        vec3 lbmin = xyzf(vol.localBoundsMin);
+        vec3 extent = vmax(xyzf(vol.localBoundsMax) - lbmin, v3s(1e-6f));
+        vec3 den = vmax(v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z) - v3s(1.0f),
+                        v3s(1.0f));
+        vec3 bsf = vmax(v3((float)bs[0], (float)bs[1], (float)bs[2]), v3s(1.0f));
+        vec3 spacing = extent / den;
+        float cellSize = fminf(spacing.x, fminf(spacing.y, spacing.z));
+        float lupw = fmaxf(length(ld), 1e-6f);
+        float t = tNear;
+        vec3 lp = lo + ld * t;
+        vec3 samplePos = ((lp - lbmin) / extent) * den;
+        vec3 brickPos = (samplePos + v3s(0.5f)) / bsf;
+        vec3 gdm1 = v3((float)gd[0] - 1.0f, (float)gd[1] - 1.0f, (float)gd[2] - 1.0f);
+        vec3 fc = vclamp(v3(floorf(brickPos.x), floorf(brickPos.y), floorf(brickPos.z)), v3s(0.0f), gdm1);
+        int32_t cell[3] = {(int32_t)fc.x, (int32_t)fc.y, (int32_t)fc.z};
+        vec3 vel = (ld / extent) * den / bsf;
+        float velv[3] = {vel.x, vel.y, vel.z};
+        float bpv[3] = {brickPos.x, brickPos.y, brickPos.z};
+        int32_t stepDir[3];
+        float tMax[3], tDelta[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            stepDir[a] = velv[a] > 0.0f ? 1 : (velv[a] < 0.0f ? -1 : 0);
+            float nb = stepDir[a] > 0 ? (float)cell[a] + 1.0f : (float)cell[a];
+            tMax[a] = stepDir[a] == 0 ? __builtin_inff() : t + (nb - bpv[a]) / velv[a];
+            tDelta[a] = stepDir[a] == 0 ? __builtin_inff() : fabsf(1.0f / velv[a]);
+        }
+        uint32_t md[3] = {grid->macroDimensionsAndIndexOffset.x, grid->macroDimensionsAndIndexOffset.y,
+                          grid->macroDimensionsAndIndexOffset.z};
+        uint32_t mo = grid->macroDimensionsAndIndexOffset.w;
+        uint32_t ms[3] = {max(grid->macroSizeAndReserved.x, 1u), max(grid->macroSizeAndReserved.y, 1u),
+                          max(grid->macroSizeAndReserved.z, 1u)};
+        bool hasMacro = md[0] > 0u && md[1] > 0u && md[2] > 0u && mo < V.gridIndexCount;
+        uint32_t maxSteps = min(gd[0] + gd[1] + gd[2] + 8u, 1024u);
+        bool volHit = false;
+        DrVolHit closest;
+        closest.t = best;
+        for (uint32_t step = 0; step < maxSteps && t <= tFar && t < closest.t; ++step) {
+            sc.gridCells++;
+            if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] >= (int32_t)gd[0] || cell[1] >= (int32_t)gd[1] ||
+                cell[2] >= (int32_t)gd[2])
+                break;
+            if (hasMacro) {
+                uint32_t mc[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] / ms[a], md[a] - 1u);
+                uint32_t slot = mo + mc[0] + mc[1] * md[0] + mc[2] * md[0] * md[1];
+                if (slot < V.gridIndexCount && V.gridIndices[slot] == 0u) {
+                    vec3 cur = brickPos + vel * (t - tNear);
+                    float cv[3] = {cur.x, cur.y, cur.z};
+                    float mt[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        float boundary = stepDir[a] > 0 ? (float)min((mc[a] + 1u) * ms[a], gd[a]) : (float)(mc[a] * ms[a]);
+                        mt[a] = stepDir[a] == 0 ? __builtin_inff() : t + (boundary - cv[a]) / velv[a];
+                    }
+                    float macroNextT = fminf(mt[0], fminf(mt[1], mt[2]));
+                    if (macroNextT > t + 1e-6f) {
+                        sc.macroSkips++;
+                        if (macroNextT > tFar || macroNextT >= closest.t) break;
+                        t = macroNextT + 1e-5f;
+                        cur = brickPos + vel * (t - tNear);
+                        vec3 fc2 = vclamp(v3(floorf(cur.x), floorf(cur.y), floorf(cur.z)), v3s(0.0f), gdm1);
+                        cell[0] = (int32_t)fc2.x;
+                        cell[1] = (int32_t)fc2.y;
+                        cell[2] = (int32_t)fc2.z;
+                        float cv2[3] = {cur.x, cur.y, cur.z};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            float nb = stepDir[a] > 0 ? (float)cell[a] + 1.0f : (float)cell[a];
+                            tMax[a] = stepDir[a] == 0 ? __builtin_inff() : t + (nb - cv2[a]) / velv[a];
+                        }
+                        continue;
+                    }
+                }
+            }
+            uint32_t slot = gio + (uint32_t)cell[0] + (uint32_t)cell[1] * gd[0] + (uint32_t)cell[2] * gd[0] * gd[1];
+            if (slot < V.gridIndexCount) {
+                uint32_t bi = V.gridIndices[slot];
+                if (bi != 0xffffffffu && bi < V.brickCount) {
+                    sc.brickTests++;
+                    DrVolHit cand;
+                    if (intersectBrick(V, ro, rd, vol, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cand) &&
+                        cand.t < closest.t) {
+                        closest = cand;
+                        closest.volume = vi;
+                        volHit = true;
+                    }
+                }
+            }
+            float nextT = fminf(tMax[0], fminf(tMax[1], tMax[2]));
+            if (volHit && closest.t <= nextT + 0.0005f) break;
+            if (nextT > tFar || nextT >= closest.t) break;
+            if (tMax[0] <= tMax[1] && tMax[0] <= tMax[2]) {
+                cell[0] += stepDir[0];
+                t = tMax[0];
+                tMax[0] += tDelta[0];
+            } else if (tMax[1] <= tMax[2]) {
+                cell[1] += stepDir[1];
+                t = tMax[1];
+                tMax[1] += tDelta[1];
+            } else {
+                cell[2] += stepDir[2];
+                t = tMax[2];
+                tMax[2] += tDelta[2];
+            }
+        }
+        if (volHit && closest.t < best) {
+            out = closest;
+            best = closest.t;
+            any = true;
+        }
+    }
+    return any;
+}

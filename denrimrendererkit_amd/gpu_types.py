@@ -59,3 +59,26 @@ assert LIGHT.itemsize == 32
 assert ENV_SAMPLE.itemsize == 8
 assert TEXTURE_DESC.itemsize == 16
 assert NODE.itemsize == 48
+
+# Sparse distance volumes (PathTrace.metal:74-148).
+VOLUME = np.dtype([
+    ("worldBoundsMin", *F4), ("worldBoundsMax", *F4), ("localBoundsMin", *F4), ("localBoundsMax", *F4),
+    ("dimensions", *U4), ("metadata", *U4), ("materialProgram", *U4),
+    ("worldToLocal", "<f4", (4, 4)),      # 4 columns
+    ("normalTransform", "<f4", (4, 4)),   # 4 columns
+])
+BRICK = np.dtype([
+    ("worldBoundsMin", *F4), ("worldBoundsMax", *F4), ("localBoundsMin", *F4), ("localBoundsMax", *F4),
+    ("sampleBoundsMin", *F4), ("sampleBoundsMax", *F4),
+    ("gridOriginAndVolume", *U4), ("dimensionsAndSampleOffset", *U4),
+])
+BRICK_SAMPLE = np.dtype([("distance", "<f4"), ("materialA", "<u4"), ("materialB", "<u4"), ("materialBlend", "<f4")])
+BRICK_GRID = np.dtype([
+    ("dimensionsAndIndexOffset", *U4), ("brickSizeAndVolume", *U4),
+    ("macroDimensionsAndIndexOffset", *U4), ("macroSizeAndReserved", *U4),
+])
+
+assert VOLUME.itemsize == 240
+assert BRICK.itemsize == 128
+assert BRICK_SAMPLE.itemsize == 16
+assert BRICK_GRID.itemsize == 64

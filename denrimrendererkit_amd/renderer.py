@@ -94,11 +94,14 @@ class GpuScene:
         self.build = build
         keep = [np.ascontiguousarray(x) for x in (build.triangles, build.materials, build.lights,
                                                    build.environment_samples, build.texture_descriptors,
-                                                   build.texture_pixels)]
-        t, m, li, e, td, tp = keep
+                                                   build.texture_pixels, build.volumes, build.volume_bricks,
+                                                   build.volume_brick_samples, build.volume_brick_grids,
+                                                   build.volume_brick_grid_indices)]
+        t, m, li, e, td, tp, v, vb, vs, vg, vi = keep
         d = _lib.SceneDesc(_ptr(t), len(t), _ptr(m), len(m), _ptr(li), len(li), _ptr(e), len(e), _ptr(td), len(td),
                            _ptr(tp), len(tp), build.environment_texture_index_plus_one, build.environment_intensity,
-                           build.environment_rotation_y, build.environment_max_radiance, max_leaf)
+                           build.environment_rotation_y, build.environment_max_radiance, max_leaf,
+                           _ptr(v), len(v), _ptr(vb), len(vb), _ptr(vs), len(vs), _ptr(vg), len(vg), _ptr(vi), len(vi))
         h = C.c_void_p()
         _lib.check(_lib.load().dr_hip_scene_upload(renderer._h, C.byref(d), C.byref(h)), "scene upload")
         self._h = h

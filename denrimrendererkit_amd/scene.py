@@ -456,6 +456,7 @@ class RenderScene:
         self.environment = environment or Environment()
         self.materials: List[Material] = []
         self.mesh_instances: List[MeshInstance] = []
+        self.sparse_volume_instances: list = []
 
     def add_material(self, material: Material) -> int:
         self.materials.append(material)
@@ -463,6 +464,11 @@ class RenderScene:
 
     def add(self, mesh: Mesh, material: int, transform: Optional[Transform] = None) -> None:
         self.mesh_instances.append(MeshInstance(mesh, material, transform or Transform.identity()))
+
+    def add_sparse_volume(self, volume, material: int, transform: Optional[Transform] = None) -> None:
+        """add(sparseVolume:material:transform:) (RenderScene.swift:154-166)."""
+        from .sdf import SparseVolumeInstance
+        self.sparse_volume_instances.append(SparseVolumeInstance(volume, material, transform or Transform.identity()))
 
     def add_quad_light(self, light: QuadLight) -> int:
         mid = self.add_material(light.material)
@@ -526,6 +532,20 @@ class AccelerationBuild:
     environment_intensity: float = 1.0
     environment_rotation_y: float = 0.0
     environment_max_radiance: float = 16.0
+    volumes: Optional[np.ndarray] = None
+    volume_bricks: Optional[np.ndarray] = None
+    volume_brick_samples: Optional[np.ndarray] = None
+    volume_brick_grids: Optional[np.ndarray] = None
+    volume_brick_grid_indices: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        from .gpu_types import BRICK, BRICK_GRID, BRICK_SAMPLE, VOLUME
+        if self.volumes is None:
+            self.volumes = np.zeros(0, VOLUME)
+            self.volume_bricks = np.zeros(0, BRICK)
+            self.volume_brick_samples = np.zeros(0, BRICK_SAMPLE)
+            self.volume_brick_grids = np.zeros(0, BRICK_GRID)
+            self.volume_brick_grid_indices = np.zeros(0, np.uint32)
 
 
 def _materials_and_textures(scene: RenderScene):
@@ -663,11 +683,16 @@ def environment_samples(env: Environment) -> np.ndarray:
 
 def compile_scene(scene: RenderScene) -> AccelerationBuild:
     """LinearTriangleAccelerationBackend.build (AccelerationBackend.swift:7-67), triangle path."""
-    if not scene.materials:
-        raise DenrimRendererError.invalid_scene("Scene must contain at least one material.")
+    if not scene.materials:  # RenderSession.swift:325-327
+        raise DenrimRendererError.invalid_scene("Scene contains no materials.")
     tris = materialize_triangles(scene)
-    if len(tris) == 0:
-        raise DenrimRendererError.invalid_scene("Scene must contain geometry.")
+    vols = (None,) * 5
+    if scene.sparse_volume_instances:
+        from .sdf import encode_sparse_volumes
+        vols = encode_sparse_volumes(scene.sparse_volume_instances, 0, len(scene.mesh_instances), 0,
+                                     len(scene.materials))
+    if len(tris) == 0 and vols[0] is None:  # RenderSession.swift:322-324
+        raise DenrimRendererError.invalid_scene("Scene contains no renderable geometry.")
     mats, descs, pixels, env_index = _materials_and_textures(scene)
     lights = light_records(tris, mats)
     env = environment_samples(scene.environment)
@@ -677,4 +702,6 @@ def compile_scene(scene: RenderScene) -> AccelerationBuild:
         environment_intensity=float(scene.environment.intensity),
         environment_rotation_y=float(scene.environment.rotation_y),
         environment_max_radiance=float(scene.environment.max_radiance),
+        volumes=vols[0], volume_bricks=vols[1], volume_brick_samples=vols[2], volume_brick_grids=vols[3],
+        volume_brick_grid_indices=vols[4],
     )

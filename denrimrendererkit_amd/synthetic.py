@@ -15,6 +15,11 @@ textures. Render randomness is the reference's per-pixel stream
   clearcoat, thin-film, sheen, random-walk subsurface}, 64 emissive
   triangles and a 2048x1024 log-normal HDRI with a 1e4 Gaussian sun;
   1920x1080, 64 spp, depth 8. ``scale`` < 1 shrinks every count for tests.
+* ``sdf_bricks_scene``      -- sparse SDF volume: virtual 768^3 grid of 8^3
+  bricks with a 1-sample apron, narrow band 2 voxels, FP16-exact distances;
+  the field is the smooth union (k = 0.05) of 200 seeded spheres and tori
+  (radii U[0.02, 0.15]) in the unit cube, each a metal or Lambertian
+  material, on a 2-triangle ground under the 1024x512 HDRI; 1024x1024, 16 spp.
 * ``material_preview_scene`` -- a 20,000-tri sphere on a 2-tri plane under a
   1024x512 synthetic HDRI, with seeded layered-material parameters.
 """
@@ -278,6 +283,42 @@ def layered_terrain_scene(scale: float = 1.0, hdri: Tuple[int, int] = (2048, 102
         p = [(cx + r * math.cos(a + k * 2.0944), cy, cz + r * math.sin(a + k * 2.0944)) for k in range(3)]
         tris.extend(_facing(p, (cx, cy - 1.0, cz)))  # emits downwards
     s.add(Mesh(np.array(tris, f32), np.arange(len(tris), dtype=np.uint32)), lights)
+    return s
+
+
+def sdf_primitives(rng: np.random.Generator, count: int = 200, metal: int = 1, matte: int = 2):
+    from .sdf import SDFPrimitive
+    prims = []
+    for _ in range(count):
+        r = float(rng.uniform(0.02, 0.15))
+        c = tuple(float(x) for x in rng.uniform(0.15, 0.85, 3))
+        mat = metal if rng.uniform() < 0.5 else matte
+        if rng.uniform() < 0.5:
+            prims.append(SDFPrimitive("sphere", mat, c, r))
+        else:
+            axis = rng.standard_normal(3)
+            axis /= np.linalg.norm(axis)
+            prims.append(SDFPrimitive("torus", mat, c, r, tuple(float(x) for x in axis),
+                                      float(r * rng.uniform(0.2, 0.45))))
+    return prims
+
+
+def sdf_bricks_scene(resolution: int = 768, primitives: int = 200, hdri: Tuple[int, int] = (1024, 512)) -> RenderScene:
+    """The sparse-SDF workload (see module docstring); `resolution` shrinks
+    the virtual grid for tests (the narrow band stays 2 voxels)."""
+    from .sdf import build_sparse_volume
+    rng = content_rng(3)
+    env = Environment(texture=lognormal_hdri(content_rng(2), *hdri), max_radiance=0.0)
+    cam = Camera(origin=(0.5, 0.95, 2.2), target=(0.5, 0.45, 0.5), vertical_field_of_view_degrees=45)
+    s = RenderScene(camera=cam, environment=env)
+    ground = s.add_material(Material(base_color=(0.5, 0.5, 0.5), roughness=0.8))
+    metal = s.add_material(Material(base_color=(0.9, 0.75, 0.55), metallic=1.0, roughness=0.3))
+    matte = s.add_material(Material(base_color=(0.7, 0.7, 0.7), specular=0.0, roughness=0.5))
+    s.add(Mesh.quad(*_facing(((-3, 0, -3), (3, 0, -3), (3, 0, 3), (-3, 0, 3)), (0, 1, 0))), ground)
+    prims = sdf_primitives(rng, primitives, metal, matte)
+    vol = build_sparse_volume(prims, dims=(resolution,) * 3, brick_size=8, apron=1,
+                              narrow_band=2.0 / (resolution - 1), smooth_radius=0.05)
+    s.add_sparse_volume(vol, metal)
     return s
 
 

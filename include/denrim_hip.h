@@ -86,6 +86,19 @@ typedef struct {
     float environment_rotation_y;
     float environment_max_radiance;
     uint32_t max_leaf_triangles;      /* 0 = default (4) */
+    /* Sparse signed-distance volumes: the volume and brick arrays of
+     * AccelerationBuild (AccelerationBuild.swift:5-20) in the reference
+     * layouts (Metal/GPUTypes.swift: GPUVolumeDescriptor 240 B,
+     * GPUVolumeBrickDescriptor 128 B, GPUVolumeBrickSample 16 B,
+     * GPUVolumeBrickGrid 64 B, grid + macro-grid indices u32). Distances are
+     * re-encoded as FP16 on upload and must be FP16-exact; material ids
+     * < 256; blends multiples of 1/65535 (dr_sdf_build_sparse produces such
+     * data). Dense volumes (metadata.w == 0) are not supported. */
+    const void *volumes;              uint32_t volume_count;
+    const void *volume_bricks;        uint32_t volume_brick_count;
+    const void *volume_brick_samples; uint32_t volume_brick_sample_count;
+    const void *volume_brick_grids;   uint32_t volume_brick_grid_count;
+    const uint32_t *volume_brick_grid_indices; uint32_t volume_brick_grid_index_count;
 } dr_scene_desc;
 
 /* RenderSettings (Renderer/RenderSettings.swift:29-101) as the session
@@ -118,6 +131,30 @@ typedef enum {
     DR_STAT_SUBSURFACE_QUERIES = 7,   /* random-walk queries, also in BOUNCE_QUERIES */
     DR_STAT_COUNT = 8
 } dr_stat;
+
+/* Sparse-field primitive for dr_sdf_build_sparse (SDFPrimitive,
+ * Scene/SDFModel.swift:28-70, in world space; the torus is an addition). */
+enum { DR_SDF_SPHERE = 0, DR_SDF_TORUS = 1 };
+typedef struct {
+    uint32_t kind;          /* DR_SDF_SPHERE or DR_SDF_TORUS */
+    uint32_t material;
+    float center[3];
+    float radius;           /* sphere radius / torus major radius */
+    float axis[3];          /* torus axis */
+    float minor_radius;     /* torus tube radius */
+} dr_sdf_primitive;
+
+/* SparseDistanceVolumeBuildSettings (Scene/SDFModel.swift:256-275). */
+typedef struct {
+    uint32_t dims[3];       /* virtual sample grid */
+    uint32_t brick_size;    /* core samples per brick side */
+    uint32_t apron;         /* stored samples beyond the core per side */
+    float narrow_band;      /* keep bricks whose samples reach |d| <= band */
+    float smooth_radius;    /* smooth-union radius of every primitive */
+    float bounds_min[3], bounds_max[3];
+} dr_sdf_build_settings;
+
+typedef struct dr_sdf_volume dr_sdf_volume;
 
 /* Thread-local message of the last failing call ("" if none). */
 const char *dr_last_error(void);
@@ -184,6 +221,20 @@ dr_status dr_hip_session_reset_stats(dr_session *session);
 dr_status dr_hip_session_set_stage_timing(dr_session *session, int enabled);
 dr_status dr_hip_session_timing(dr_session *session, double *total_ms, double *stage_ms, uint64_t *stage_launches,
                                 size_t stages);
+
+/* DistanceVolumeBuilder.buildSparse (Scene/SDFModel.swift:256-355) on the
+ * host: samples the smooth union of `prims` on the virtual grid and keeps the
+ * narrow-band bricks (distances rounded to FP16). */
+dr_status dr_sdf_build_sparse(const dr_sdf_primitive *prims, uint32_t count, const dr_sdf_build_settings *settings,
+                              dr_sdf_volume **out);
+dr_status dr_sdf_volume_info(const dr_sdf_volume *volume, uint32_t *bricks, uint64_t *samples, uint64_t *candidates);
+/* Per brick (z, y, x order): stored origin / dims and core origin / dims
+ * (3 x int32 each); per stored sample (x fastest): distance, material A / B,
+ * blend. Any pointer may be NULL. */
+dr_status dr_sdf_volume_export(const dr_sdf_volume *volume, int32_t *stored_origin, int32_t *stored_dims,
+                               int32_t *core_origin, int32_t *core_dims, float *distance, uint32_t *material_a,
+                               uint32_t *material_b, float *blend);
+dr_status dr_sdf_volume_free(dr_sdf_volume *volume);
 
 #ifdef __cplusplus
 }

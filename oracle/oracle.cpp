@@ -141,13 +141,15 @@ template <class R> Mat<R> zeroMat() { Mat<R> m; std::memset(&m, 0, sizeof(m)); r
 
 // ----------------------------------------------------------- ray and hit --
 template <class R> struct Ray { V3<R> origin, direction; };
-template <class R> struct Hit {  // PathTrace.metal:229-253 (triangle fields)
+template <class R> struct Hit {  // PathTrace.metal:229-253 (triangle and sparse-volume fields)
     bool hit;
     R t;
     V3<R> position, normal;
     V2<R> uv;
     V3<R> tangent, bitangent;
-    uint32_t materialID, objectID, primitiveID;
+    uint32_t materialID, materialID2;
+    R materialBlend;
+    uint32_t objectID, primitiveID;
     bool frontFacing;
 };
 template <class R> Hit<R> emptyHit() {  // :1883-1909
@@ -160,6 +162,8 @@ template <class R> Hit<R> emptyHit() {  // :1883-1909
     h.tangent = V3<R>(1, 0, 0);
     h.bitangent = V3<R>(0, 0, 1);
     h.materialID = 0;
+    h.materialID2 = 0;
+    h.materialBlend = R(0);
     h.objectID = 0;
     h.primitiveID = 0;
     h.frontFacing = true;
@@ -638,6 +642,8 @@ template <class R> void fillTriangleHit(const Ray<R> &ray, const or_triangle &tr
     h.tangent = normalize(v3<R>(tri.tangent));
     h.bitangent = normalize(v3<R>(tri.bitangent));
     h.materialID = tri.materialID;
+    h.materialID2 = tri.materialID;
+    h.materialBlend = R(0);
     h.objectID = tri.objectID;
     h.primitiveID = index;
 }
@@ -661,6 +667,8 @@ template <class R> bool intersectBounds(const Ray<R> &ray, V3<R> inv, const Node
     return tNear <= tFar;
 }
 
+#include "oracle_sdf.h"
+
 } // namespace
 
 // ------------------------------------------------------------------ scene --
@@ -673,6 +681,11 @@ struct or_scene {
     std::vector<or_float4> texels;
     std::vector<Node> nodes;
     std::vector<uint32_t> primIndices;
+    std::vector<or_volume> volumes;
+    std::vector<or_brick> bricks;
+    std::vector<or_brick_sample> brickSamples;
+    std::vector<or_brick_grid> brickGrids;
+    std::vector<uint32_t> brickGridIndices;
 };
 
 namespace {
@@ -746,7 +759,23 @@ template <class R> struct Ctx {
     const or_camera *prevCam;
     int traversal;
     uint64_t cnt[OR_CNT_COUNT];
+    SdfScene sdf;
 };
+
+template <class R> void initCtx(Ctx<R> &c) {
+    const or_scene &s = *c.s;
+    c.sdf.volumes = s.volumes.data();
+    c.sdf.volumeCount = uint32_t(s.volumes.size());
+    c.sdf.bricks = s.bricks.data();
+    c.sdf.brickCount = uint32_t(s.bricks.size());
+    c.sdf.samples = s.brickSamples.data();
+    c.sdf.sampleCount = uint32_t(s.brickSamples.size());
+    c.sdf.grids = s.brickGrids.data();
+    c.sdf.gridCount = uint32_t(s.brickGrids.size());
+    c.sdf.gridIndices = s.brickGridIndices.empty() ? nullptr : s.brickGridIndices.data();
+    c.sdf.gridIndexCount = uint32_t(s.brickGridIndices.size());
+    c.sdf.renderQuality = 2u;  // RenderQuality.final
+}
 
 // intersectScene (PathTrace.metal:3294-3421, triangle part) and
 // intersectSceneLinear (:3265-3292), with the lowest-index tie-break.
@@ -790,13 +819,46 @@ template <class R> Hit<R> intersectScene(Ctx<R> &c, const Ray<R> &ray) {
     }
     Hit<R> h = emptyHit<R>();
     if (best != 0xffffffffu) fillTriangleHit(ray, s.triangles[best], best, bestT, bestU, bestV, h);
+    if (!s.volumes.empty()) {
+        // closestVolumeHit after the triangles, limited by their closest t (:3397-3418).
+        VolHit<R> vh = closestVolumeHit(c.sdf, ray, h.t, c.cnt + OR_CNT_SDF_BASE);
+        if (vh.hit) {
+            h.hit = true;
+            h.t = vh.t;
+            h.position = vh.position;
+            h.normal = vh.normal;
+            h.uv = V2<R>{R(0), R(0)};
+            V3<R> helper = std::fabs(vh.normal.y) < R(0.999) ? V3<R>(0, 1, 0) : V3<R>(1, 0, 0);
+            h.tangent = normalize(cross(helper, vh.normal));
+            h.bitangent = normalize(cross(vh.normal, h.tangent));
+            h.materialID = vh.materialA;
+            h.materialID2 = vh.materialB;
+            h.materialBlend = vh.blend;
+            h.objectID = vh.objectID;
+            h.primitiveID = vh.primitiveID;
+            h.frontFacing = vh.frontFacing;
+        }
+    }
     return h;
 }
 
-template <class R> Mat<R> materialForHit(const Ctx<R> &c, const Hit<R> &h) {  // :1096-1137 (triangle path)
+template <class R> Mat<R> blendMaterials(Mat<R> a, const Mat<R> &b, R blend) {  // :549-569
+    R t = clampv(blend, R(0), R(1));
+    V4<R> *pa = &a.baseColor;
+    const V4<R> *pb = &b.baseColor;
+    for (int k = 0; k < 17; ++k) pa[k] = mix4(pa[k], pb[k], t);
+    return a;
+}
+
+template <class R> Mat<R> materialForHit(const Ctx<R> &c, const Hit<R> &h) {  // :1096-1137
     uint32_t count = uint32_t(c.s->materials.size());
     uint32_t a = std::min<uint32_t>(h.materialID, count - 1u);
-    return toMat<R>(c.s->materials[a]);
+    Mat<R> m = toMat<R>(c.s->materials[a]);
+    if (h.materialBlend > R(0)) {
+        uint32_t b = std::min<uint32_t>(h.materialID2, count - 1u);
+        m = blendMaterials(m, toMat<R>(c.s->materials[b]), h.materialBlend);
+    }
+    return m;  // no material-field flags or programs on this path
 }
 
 template <class R> V4<R> texel(const or_scene &s, uint32_t i) { return v4<R>(s.texels[i]); }
@@ -1428,7 +1490,8 @@ int renderT(const or_scene *s, const or_settings *st, const or_camera *cam, cons
     const int64_t rows = th;
 #pragma omp parallel
     {
-        Ctx<R> c{s, st, cam, prev, traversal, {0}};
+        Ctx<R> c{s, st, cam, prev, traversal, {0}, {}};
+        initCtx(c);
 #pragma omp for schedule(dynamic, 1)
         for (int64_t r = 0; r < rows; ++r) {
             uint32_t py = ty0 + uint32_t(r);
@@ -1463,6 +1526,12 @@ int or_scene_create(const or_scene_desc *d, uint32_t maxLeaf, or_scene **out) {
     if (d->envSampleCount) s->env.assign(d->envSamples, d->envSamples + d->envSampleCount);
     if (d->textureCount) s->textures.assign(d->textureDescs, d->textureDescs + d->textureCount);
     if (d->texturePixelCount) s->texels.assign(d->texturePixels, d->texturePixels + d->texturePixelCount);
+    if (d->volumeCount) s->volumes.assign(d->volumes, d->volumes + d->volumeCount);
+    if (d->brickCount) s->bricks.assign(d->bricks, d->bricks + d->brickCount);
+    if (d->brickSampleCount) s->brickSamples.assign(d->brickSamples, d->brickSamples + d->brickSampleCount);
+    if (d->brickGridCount) s->brickGrids.assign(d->brickGrids, d->brickGrids + d->brickGridCount);
+    if (d->brickGridIndexCount)
+        s->brickGridIndices.assign(d->brickGridIndices, d->brickGridIndices + d->brickGridIndexCount);
     if (s->materials.empty()) return 2;
     const uint32_t n = d->triangleCount;
     if (n > 0) {
@@ -1514,7 +1583,8 @@ int or_intersect(const or_scene *s, const float *rays, uint32_t count, int trave
     or_camera cam{};
 #pragma omp parallel
     {
-        Ctx<float> c{s, &dummy, &cam, &cam, traversal, {0}};
+        Ctx<float> c{s, &dummy, &cam, &cam, traversal, {0}, {}};
+        initCtx(c);
 #pragma omp for schedule(dynamic, 256)
         for (int64_t i = 0; i < int64_t(count); ++i) {
             const float *r = rays + 8 * i;

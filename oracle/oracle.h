@@ -50,6 +50,28 @@ typedef struct {
 } or_light;                                   /* 32 B */
 
 typedef struct { float cdf, pdf; } or_env_sample;   /* 8 B */
+
+/* GPUVolumeDescriptor (PathTrace.metal:74-90), 240 B. */
+typedef struct {
+    or_float4 worldBoundsMin, worldBoundsMax, localBoundsMin, localBoundsMax;
+    or_uint4 dimensions, metadata, materialProgram;
+    or_float4 worldToLocal0, worldToLocal1, worldToLocal2, worldToLocal3;
+    or_float4 normalTransform0, normalTransform1, normalTransform2, normalTransform3;
+} or_volume;
+
+/* GPUVolumeBrickDescriptor (PathTrace.metal:132-141), 128 B. */
+typedef struct {
+    or_float4 worldBoundsMin, worldBoundsMax, localBoundsMin, localBoundsMax, sampleBoundsMin, sampleBoundsMax;
+    or_uint4 gridOriginAndVolume, dimensionsAndSampleOffset;
+} or_brick;
+
+/* GPUVolumeBrickSample (PathTrace.metal:103-108), 16 B. */
+typedef struct { float distance; uint32_t materialA, materialB; float materialBlend; } or_brick_sample;
+
+/* GPUVolumeBrickGrid (PathTrace.metal:143-148), 64 B. */
+typedef struct {
+    or_uint4 dimensionsAndIndexOffset, brickSizeAndVolume, macroDimensionsAndIndexOffset, macroSizeAndReserved;
+} or_brick_grid;
 typedef struct { uint32_t offset, width, height, mode; } or_texture_desc; /* 16 B */
 
 /* The reference's per-dispatch constants that the triangle path reads
@@ -77,6 +99,11 @@ typedef struct {
     const or_env_sample *envSamples; uint32_t envSampleCount;
     const or_texture_desc *textureDescs; uint32_t textureCount;
     const or_float4 *texturePixels; uint64_t texturePixelCount;
+    const or_volume *volumes; uint32_t volumeCount;
+    const or_brick *bricks; uint32_t brickCount;
+    const or_brick_sample *brickSamples; uint32_t brickSampleCount;
+    const or_brick_grid *brickGrids; uint32_t brickGridCount;
+    const uint32_t *brickGridIndices; uint32_t brickGridIndexCount;
 } or_scene_desc;
 
 typedef struct or_scene or_scene;
@@ -88,7 +115,9 @@ enum {
     OR_CNT_SHADOW_QUERIES = 2,
     OR_CNT_NODE_VISITS = 3,
     OR_CNT_TRIANGLE_TESTS = 4,
-    OR_CNT_COUNT = 8
+    /* sparse-SDF counters at 8 + the reference's slot (PathTrace.metal:191-200) */
+    OR_CNT_SDF_BASE = 8,
+    OR_CNT_COUNT = 24
 };
 
 /* Scene: copies the arrays and builds the count-median BVH. */

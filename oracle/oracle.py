@@ -17,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 CNT_PRIMARY, CNT_BOUNCE, CNT_SHADOW, CNT_NODES, CNT_TRIS = 0, 1, 2, 3, 4
+CNT_SDF = 8  # + the reference's SDF counter slot (2 grid cells ... 9 brick hits)
+N_COUNTERS = 24
 
 
 def build(force: bool = False) -> str:
@@ -33,6 +35,11 @@ class _SceneDesc(C.Structure):
         ("envSamples", C.c_void_p), ("envSampleCount", C.c_uint32),
         ("textureDescs", C.c_void_p), ("textureCount", C.c_uint32),
         ("texturePixels", C.c_void_p), ("texturePixelCount", C.c_uint64),
+        ("volumes", C.c_void_p), ("volumeCount", C.c_uint32),
+        ("bricks", C.c_void_p), ("brickCount", C.c_uint32),
+        ("brickSamples", C.c_void_p), ("brickSampleCount", C.c_uint32),
+        ("brickGrids", C.c_void_p), ("brickGridCount", C.c_uint32),
+        ("brickGridIndices", C.c_void_p), ("brickGridIndexCount", C.c_uint32),
     ]
 
 
@@ -84,10 +91,13 @@ class OracleScene:
         L = lib()
         self._keep = [np.ascontiguousarray(x) for x in (build.triangles, build.materials, build.lights,
                                                          build.environment_samples, build.texture_descriptors,
-                                                         build.texture_pixels)]
-        t, m, li, e, td, tp = self._keep
+                                                         build.texture_pixels, build.volumes, build.volume_bricks,
+                                                         build.volume_brick_samples, build.volume_brick_grids,
+                                                         build.volume_brick_grid_indices)]
+        t, m, li, e, td, tp, v, vb, vs, vg, vi = self._keep
         d = _SceneDesc(_ptr(t), len(t), _ptr(m), len(m), _ptr(li), len(li), _ptr(e), len(e),
-                       _ptr(td), len(td), _ptr(tp), len(tp))
+                       _ptr(td), len(td), _ptr(tp), len(tp), _ptr(v), len(v), _ptr(vb), len(vb), _ptr(vs), len(vs),
+                       _ptr(vg), len(vg), _ptr(vi), len(vi))
         h = C.c_void_p()
         rc = L.or_scene_create(C.byref(d), max_leaf, C.byref(h))
         if rc != 0:
@@ -154,7 +164,7 @@ def render(scene: OracleScene, settings: _Settings, camera: np.ndarray, previous
     acc = np.zeros((H, W, 4), dt) if accumulation is None else accumulation
     assert acc.dtype == dt and acc.flags.c_contiguous
     aov = np.zeros((7, H, W, 4), dt) if with_aovs else None
-    cnt = np.zeros(8, np.uint64)
+    cnt = np.zeros(N_COUNTERS, np.uint64)
     cam = np.ascontiguousarray(camera)
     prev = np.ascontiguousarray(previous_camera) if previous_camera is not None else cam
     rc = lib().or_render(scene.handle, C.byref(settings), _ptr(cam), _ptr(prev), sample_begin, sample_count,
