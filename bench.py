@@ -29,6 +29,12 @@ HBM_PEAK_GBS = 8000.0
 # triangle test reads 36 B of positions, each query reads its 32-B ray and
 # writes a 16-B hit.
 NODE_BYTES, TRI_BYTES, RAY_BYTES = 64, 36, 48
+# Sparse-volume units: a DDA cell reads its 4-B brick index and 4-B macro
+# word; a brick test reads the 64-B brick header; a march step reads the 8
+# FP16 corner samples (16 B); a hit adds 8 bisection lookups (8 x 16 B), two
+# cubic polishes and a cubic normal (3 x 64 FP16 samples) and the 8-B
+# material pair: 128 + 384 + 8 = 520 B.
+CELL_BYTES, BRICK_BYTES, STEP_BYTES, HIT_BYTES = 8, 64, 16, 520
 
 
 def log(msg):
@@ -42,6 +48,12 @@ def workload(name):
             "workload": "layered-terrain 1920x1080, 6,168,640 tris (1M fBm terrain + 256 x 20k displaced spheres), "
                         "6 layered material kinds, 64 emissive tris, 2048x1024 HDRI, max depth 8",
             "width": 1920, "height": 1080, "max_bounces": 8, "triangles": 6168640}
+    if name == "sdf_1024":
+        return synthetic.sdf_bricks_scene(resolution=768, primitives=200, hdri=(2048, 1024)), 1024, 1024, 4, {
+            "workload": "sparse SDF bricks 1024x1024: virtual 768^3 grid of 8^3 bricks (1-voxel apron, FP16 "
+                        "distances), smooth-min union (k=0.05) of 200 spheres and tori, metal/Lambertian, "
+                        "2048x1024 HDRI, ground quad, max depth 4",
+            "width": 1024, "height": 1024, "max_bounces": 4}
     if name == "room_320x180":
         return synthetic.cornell_room_scene(), 320, 180, 4, {
             "workload": "room 320x180, 8,334 tris, Lambertian + GGX metal, 1 area light, max depth 4",
@@ -96,7 +108,8 @@ def main():
     t0 = time.perf_counter()
     scene, W, H, bounces, config = workload(args.workload)
     build = compile_scene(scene)
-    log(f"scene compiled: {len(build.triangles)} tris in {time.perf_counter() - t0:.1f} s")
+    log(f"scene compiled: {len(build.triangles)} tris, {len(build.volume_bricks)} bricks in "
+        f"{time.perf_counter() - t0:.1f} s")
     renderer = DenrimRenderer(0)
     t1 = time.perf_counter()
     gscene = renderer.upload(build)
@@ -111,6 +124,7 @@ def main():
     cs.render(1)
     c = cs.stats().astype(np.float64)
     cs.close()
+    sdf_ext, sdf_sh = c[8:16], c[16:24]  # cells, macro skips, brick tests, invalid, culls, marches, steps, hits
     queries_per_step = c[0] + c[1] + c[2]
     log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
         f"shadow {c[2]:.0f}), "
@@ -137,14 +151,13 @@ def main():
     # Roofline of the dominant kernel (largest share of stage time).
     st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
     dom = max(("extend", "shadow", "shade"), key=lambda k: st_ms[k])
-    if dom == "extend":
-        nodes, tris, rays = c[3], c[4], c[0] + c[1]
-    elif dom == "shadow":
-        nodes, tris, rays = c[5], c[6], c[2]
+    if dom == "shadow":
+        nodes, tris, rays, sdf = c[5], c[6], c[2], sdf_sh
     else:
-        nodes, tris, rays = c[3], c[4], c[0] + c[1]
+        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1], sdf_ext
     launches_per_step = max(1, st_n[dom] // max(1, args.steps))
-    bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays) / launches_per_step
+    sdf_bytes = CELL_BYTES * sdf[0] + BRICK_BYTES * sdf[2] + STEP_BYTES * sdf[6] + HIT_BYTES * sdf[7]
+    bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays + sdf_bytes) / launches_per_step
     avg_ms = st_ms[dom] / max(1, st_n[dom])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     roofline = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
@@ -158,6 +171,12 @@ def main():
 
     config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"],
                    "ray_queries_per_step": int(queries_per_step), "rays_per_path": round(queries_per_step / (W * H), 3)})
+    if len(build.volume_bricks):
+        tot = sdf_ext + sdf_sh
+        config.update({"bricks": int(len(build.volume_bricks)), "brick_samples": int(len(build.volume_brick_samples)),
+                       "sdf_per_step": {"grid_cells": int(tot[0]), "macro_skips": int(tot[1]),
+                                        "brick_tests": int(tot[2]), "marches": int(tot[5]),
+                                        "march_steps": int(tot[6]), "hits": int(tot[7])}})
     out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

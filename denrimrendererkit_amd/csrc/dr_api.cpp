@@ -83,6 +83,13 @@ struct dr_scene {
     uint32_t *envGuide = nullptr;
     uint4 *texDesc = nullptr;
     float4 *texels = nullptr;
+    DrVolume *volumes = nullptr;
+    DrBrick *bricks = nullptr;
+    uint16_t *dist16 = nullptr;
+    float *dist32 = nullptr;
+    uint2 *sampleMaterial = nullptr;
+    DrBrickGrid *grids = nullptr;
+    uint32_t *gridIndices = nullptr;
     dr::BvhBuildStats stats;
     int sessions = 0;
     bool released = false;
@@ -90,6 +97,7 @@ struct dr_scene {
 
 struct dr_session {
     dr_scene *scene = nullptr;
+    DrSceneDev sceneDev{};  // the scene's device view with this session's quality level
     dr_settings settings{};
     DrCamera camera{}, previousCamera{};
     uint32_t sampleCount = 0;
@@ -148,15 +156,57 @@ static void freeScene(dr_scene *s) {
     devFree(s->envGuide);
     devFree(s->texDesc);
     devFree(s->texels);
+    devFree(s->volumes);
+    devFree(s->bricks);
+    devFree(s->dist16);
+    devFree(s->dist32);
+    devFree(s->sampleMaterial);
+    devFree(s->grids);
+    devFree(s->gridIndices);
 }
+
+namespace {
+
+// Reference-layout sparse-volume records (PathTrace.metal:92-148).
+struct RefBrick {
+    float worldBoundsMin[4], worldBoundsMax[4], localBoundsMin[4], localBoundsMax[4];
+    float sampleBoundsMin[4], sampleBoundsMax[4];
+    uint32_t gridOriginAndVolume[4], dimensionsAndSampleOffset[4];
+};
+static_assert(sizeof(RefBrick) == 128, "GPUVolumeBrick layout");
+struct RefBrickSample {
+    float distance;
+    uint32_t materialA, materialB;
+    float materialBlend;
+};
+static_assert(sizeof(RefBrickSample) == 16, "GPUVolumeBrickSample layout");
+
+bool halfExact(float v, uint16_t &bits) {
+    _Float16 h = (_Float16)v;
+    if ((float)h != v && !(v != v)) return false;
+    if (v != v) return false;  // NaN payloads are not carried
+    bits = __builtin_bit_cast(uint16_t, h);
+    return true;
+}
+
+}  // namespace
 
 dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scene **out) {
     if (!device || !d || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     // RenderSession.init checks (Renderer/RenderSession.swift:322-327).
-    if (d->material_count == 0 || !d->materials)
-        return fail(DR_ERR_INVALID_SCENE, "Scene must contain at least one material.");
-    if (d->triangle_count == 0 || !d->triangles) return fail(DR_ERR_INVALID_SCENE, "Scene must contain geometry.");
+    if ((d->triangle_count == 0 || !d->triangles) && d->volume_count == 0)
+        return fail(DR_ERR_INVALID_SCENE, "Scene contains no renderable geometry.");
+    if (d->material_count == 0 || !d->materials) return fail(DR_ERR_INVALID_SCENE, "Scene contains no materials.");
+    if (d->volume_count &&
+        (!d->volumes || (d->volume_brick_count && !d->volume_bricks) ||
+         (d->volume_brick_sample_count && !d->volume_brick_samples) ||
+         (d->volume_brick_grid_count && !d->volume_brick_grids) ||
+         (d->volume_brick_grid_index_count && !d->volume_brick_grid_indices)))
+        return fail(DR_ERR_INVALID_ARGUMENT, "volume arrays are NULL");
+    if (d->volume_count && d->material_count > 65536u)
+        return fail(DR_ERR_INVALID_SCENE, "scenes with sparse volumes are limited to 65536 materials");
+    if (d->volume_count >= (1u << 20)) return fail(DR_ERR_INVALID_SCENE, "too many volumes");
     if (d->light_count && !d->lights) return fail(DR_ERR_INVALID_ARGUMENT, "lights is NULL");
     if (d->environment_sample_count && !d->environment_samples)
         return fail(DR_ERR_INVALID_ARGUMENT, "environment_samples is NULL");
@@ -178,7 +228,55 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     s->device = device;
     std::vector<DrNode> nodes;
     std::vector<float4> triPos;
-    dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos, &s->stats);
+    if (d->triangle_count)
+        dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos,
+                     &s->stats);
+
+    // Sparse volumes: compact brick headers, FP16 distances when every
+    // distance is FP16-exact (else FP32), per-sample material pairs with the
+    // material indices clamped as materialForHit does (:1096-1137).
+    std::vector<DrBrick> bricks(d->volume_brick_count);
+    std::vector<uint16_t> dist16;
+    std::vector<float> dist32;
+    std::vector<uint2> sampleMat;
+    bool distHalf = true;
+    if (d->volume_count) {
+        const auto *rb = static_cast<const RefBrick *>(d->volume_bricks);
+        for (uint32_t i = 0; i < d->volume_brick_count; ++i) {
+            const RefBrick &b = rb[i];
+            DrBrick &o = bricks[i];
+            o.smin = make_float4(b.sampleBoundsMin[0], b.sampleBoundsMin[1], b.sampleBoundsMin[2], b.sampleBoundsMin[3]);
+            o.smax = make_float4(b.sampleBoundsMax[0], b.sampleBoundsMax[1], b.sampleBoundsMax[2], b.sampleBoundsMax[3]);
+            uint32_t dx = b.dimensionsAndSampleOffset[0], dy = b.dimensionsAndSampleOffset[1],
+                     dz = b.dimensionsAndSampleOffset[2];
+            // Dimensions above 255 cannot be packed; they also never pass the
+            // reference's range check against a 32-bit sample count at
+            // useful sizes, so they are stored as invalid (0).
+            uint32_t packed = (dx < 256u && dy < 256u && dz < 256u) ? (dx | dy << 8 | dz << 16) : 0u;
+            float pf, of;
+            std::memcpy(&pf, &packed, 4);
+            std::memcpy(&of, &b.dimensionsAndSampleOffset[3], 4);
+            o.cmin = make_float4(b.localBoundsMin[0], b.localBoundsMin[1], b.localBoundsMin[2], pf);
+            o.cmax = make_float4(b.localBoundsMax[0], b.localBoundsMax[1], b.localBoundsMax[2], of);
+        }
+        const auto *rs = static_cast<const RefBrickSample *>(d->volume_brick_samples);
+        const uint32_t n = d->volume_brick_sample_count;
+        dist16.resize(n);
+        for (uint32_t i = 0; i < n && distHalf; ++i) distHalf = halfExact(rs[i].distance, dist16[i]);
+        if (!distHalf) {
+            dist16.clear();
+            dist32.resize(n);
+            for (uint32_t i = 0; i < n; ++i) dist32[i] = rs[i].distance;
+        }
+        sampleMat.resize(n);
+        const uint32_t last = d->material_count - 1u;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t a = std::min(rs[i].materialA, last), b = std::min(rs[i].materialB, last);
+            uint32_t blend;
+            std::memcpy(&blend, &rs[i].materialBlend, 4);
+            sampleMat[i] = make_uint2(a | b << 16, blend);
+        }
+    }
 
     bool opaqueOnly = true;
     const auto *mats = static_cast<const DrMaterial *>(d->materials);
@@ -222,6 +320,13 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     chain(upload(&s->envGuide, guide.data(), guide.size()));
     chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count));
     chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count));
+    chain(upload(&s->volumes, d->volumes, d->volume_count));
+    chain(upload(&s->bricks, bricks.data(), bricks.size()));
+    chain(upload(&s->dist16, dist16.data(), dist16.size()));
+    chain(upload(&s->dist32, dist32.data(), dist32.size()));
+    chain(upload(&s->sampleMaterial, sampleMat.data(), sampleMat.size()));
+    chain(upload(&s->grids, d->volume_brick_grids, d->volume_brick_grid_count));
+    chain(upload(&s->gridIndices, d->volume_brick_grid_indices, d->volume_brick_grid_index_count));
     if (e != hipSuccess) {
         freeScene(s);
         delete s;
@@ -249,6 +354,20 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     g.envRotationY = d->environment_rotation_y;
     g.envMaxRadiance = d->environment_max_radiance;
     g.opaqueOnly = opaqueOnly ? 1u : 0u;
+    DrSdfDev &v = g.sdf;
+    v.volumes = s->volumes;
+    v.bricks = s->bricks;
+    v.dist16 = s->dist16;
+    v.dist32 = s->dist32;
+    v.material = s->sampleMaterial;
+    v.grids = s->grids;
+    v.gridIndices = s->gridIndices;
+    v.volumeCount = d->volume_count;
+    v.brickCount = d->volume_brick_count;
+    v.sampleCount = d->volume_brick_sample_count;
+    v.gridCount = d->volume_brick_grid_count;
+    v.gridIndexCount = d->volume_brick_grid_index_count;
+    v.distHalf = distHalf ? 1u : 0u;
     *out = s;
     return DR_OK;
 }
@@ -288,14 +407,15 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     if (!scene || !st || !camera || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (st->width == 0 || st->height == 0)  // RenderSession.swift:304-306
-        return fail(DR_ERR_INVALID_SCENE, "Render size must be greater than zero.");
-    if (st->quality != 2)
-        return fail(DR_ERR_MISSING_SHADER_FUNCTION, "the hip backend serves the final (path-traced) quality only");
+        return fail(DR_ERR_INVALID_SCENE, "Render dimensions must be positive.");
+    if (st->quality > 2) return fail(DR_ERR_INVALID_ARGUMENT, "quality must be 0 (preview), 1 (interactive) or 2 (final)");
     if (st->max_bounces > DR_MAX_BOUNCES) return fail(DR_ERR_INVALID_ARGUMENT, "max_bounces exceeds 64");
     DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
     auto *s = new dr_session;
     s->scene = scene;
     s->settings = *st;
+    s->sceneDev = scene->dev;
+    s->sceneDev.sdf.renderQuality = st->quality;  // constants.renderQuality (RenderSettings.swift:106-115)
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : s->camera;
     const uint64_t pixels = uint64_t(st->width) * st->height;
@@ -315,7 +435,7 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
     if (e == hipSuccess) e = hipEventCreate(&s->evEnd);
-    // Path state: 7 float4 arrays of N, 4 float4 arrays of 3N, 3 u32 queues
+    // Path state: 8 float4 arrays of N, 4 float4 arrays of 3N, 3 u32 queues
     // (2 x N, 3N), counters, stats, stack overflow.
     size_t bytes = 0;
     auto take = [&](size_t n) {
@@ -324,7 +444,7 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
         return off;
     };
     size_t oRayO = take(N * 16), oRayD = take(N * 16), oThr = take(N * 16), oRad = take(N * 16), oMeta = take(N * 16),
-           oHit = take(N * 16), oNeeT = take(N * 16);
+           oHit = take(N * 16), oHitN = take(N * 16), oNeeT = take(N * 16);
     size_t oShO = take(3 * N * 16), oShD = take(3 * N * 16), oShC = take(3 * N * 16), oShR = take(3 * N * 16);
     size_t oQ0 = take(N * 4), oQ1 = take(N * 4), oSQ = take(3 * N * 4);
     size_t oCtr = take(DR_CTR_WORDS * 4), oStats = take(DR_CNT_SLOTS * 8);
@@ -348,6 +468,7 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     P.rad = reinterpret_cast<float4 *>(b + oRad);
     P.meta = reinterpret_cast<uint4 *>(b + oMeta);
     P.hit = reinterpret_cast<float4 *>(b + oHit);
+    P.hitN = reinterpret_cast<float4 *>(b + oHitN);
     P.neeT = reinterpret_cast<float4 *>(b + oNeeT);
     P.shO = reinterpret_cast<float4 *>(b + oShO);
     P.shD = reinterpret_cast<float4 *>(b + oShD);
@@ -424,7 +545,7 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
             s->stageEvents.push_back(z);
             (void)hipEventRecord(a, s->stream);
         }
-        hipError_t e = dr::launchStage(stage, b, s->scene->dev, L, s->camera, s->previousCamera, s->paths, s->out,
+        hipError_t e = dr::launchStage(stage, b, s->sceneDev, L, s->camera, s->previousCamera, s->paths, s->out,
                                        nSlots, s->traceGrid, s->stream);
         if (s->stageTiming) (void)hipEventRecord(s->stageEvents[ev + 1], s->stream);
         s->stageLaunches[stage]++;

@@ -29,7 +29,8 @@ struct DrPaths {
     float4 *thr;    // throughput.xyz, outputAlpha
     float4 *rad;    // radiance.xyz, -
     uint4 *meta;    // pixel, sampleIndex, rng state, flags
-    float4 *hit;    // t, u, v, bits(triangle id)
+    float4 *hit;    // t, u, v, bits(triangle id); volume hits: see DR_VOL_HIT
+    float4 *hitN;   // normal of a sparse-volume hit
     float4 *neeT;   // throughput at the NEE vertex
     float4 *shO;    // 3 per slot: shadow origin.xyz, max distance
     float4 *shD;    // 3 per slot: direction.xyz, pdf denominator

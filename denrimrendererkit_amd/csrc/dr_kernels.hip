@@ -25,6 +25,7 @@
 #include "dr_kernels.h"
 #include "dr_shade.h"
 #include "dr_traverse.h"
+#include "dr_sdf.h"
 
 #define DR_FLAG_PRIMARY 1u
 #define DR_FLAG_NEE 2u
@@ -47,6 +48,50 @@ __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t *counter) {
 __device__ __forceinline__ void waveAddStat(unsigned long long *stats, int slot, uint32_t v) {
     // Counting runs only (statsEnabled); one relaxed 64-bit atomic per lane.
     if (v) atomicAdd(&stats[slot], (unsigned long long)v);
+}
+__device__ __forceinline__ void addSdfStats(unsigned long long *stats, const DrSdfCounters &c,
+                                            int base = DR_CNT_SDF_GRID_CELLS) {
+    waveAddStat(stats, base + 0, c.gridCells);
+    waveAddStat(stats, base + 1, c.macroSkips);
+    waveAddStat(stats, base + 2, c.brickTests);
+    waveAddStat(stats, base + 3, c.invalid);
+    waveAddStat(stats, base + 4, c.rangeCulls);
+    waveAddStat(stats, base + 5, c.marches);
+    waveAddStat(stats, base + 6, c.marchSteps);
+    waveAddStat(stats, base + 7, c.hits);
+}
+
+// A closest hit as passed between stages. Triangles: (t, u, v, bits(id)).
+// Sparse-volume samples: (t, bits(blend), bits(A | B << 16),
+// bits(DR_VOL_HIT | frontFacing << 30 | volume)) plus the normal in hitN.
+// Miss: id 0xffffffff.
+#define DR_VOL_HIT 0x80000000u
+#define DR_VOL_FRONT 0x40000000u
+#define DR_MISS 0xffffffffu
+
+// intersectScene (PathTrace.metal:3294-3421): closest triangle (lowest index
+// on equal t), then the closest sparse-volume hit in front of it (:3397-3418).
+// VOL = false compiles the triangle-only form (scenes without volumes).
+template <bool VOL>
+__device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, uint32_t &nv,
+                                             uint32_t &nt, DrSdfCounters &sc, float4 &h, float4 &hn) {
+    DrHitRec hr{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+    bool hit = false;
+    if (S.triangleCount) {
+        DrRayPre r = dr_prep(o, d);
+        hit = dr_traverse<false>(S, r, hr, stk, nv, nt);
+    }
+    h = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
+    if constexpr (VOL) {
+        DrVolHit vh;
+        if (dr_trace_volumes(S.sdf, o, d, hr.t, sc, vh)) {
+            h = make_float4(vh.t, __uint_as_float(vh.material.y), __uint_as_float(vh.material.x),
+                            __uint_as_float(DR_VOL_HIT | (vh.frontFacing ? DR_VOL_FRONT : 0u) | vh.volume));
+            hn = make_float4(vh.normal.x, vh.normal.y, vh.normal.z, 0.0f);
+            hit = true;
+        }
+    }
+    return hit;
 }
 
 // ---------------------------------------------------------------- raygen --
@@ -79,6 +124,7 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
 }
 
 // ---------------------------------------------------------------- extend --
+template <bool VOL>
 __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t *queue = P.queue[bounce & 1u];
@@ -87,6 +133,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0;
+    DrSdfCounters sc{};
     while (true) {
         uint32_t base = 0;
         if (laneId() == 0) base = atomicAdd(steal, 64u);
@@ -96,15 +143,23 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths
         if (qi < count) {
             uint32_t slot = queue[qi];
             float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
-            DrRayPre r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
-            DrHitRec h{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
-            dr_traverse<false>(S, r, h, stk, nv, nt);
-            P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+            if constexpr (!VOL) {
+                DrRayPre r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                DrHitRec h{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+                dr_traverse<false>(S, r, h, stk, nv, nt);
+                P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+            } else {
+                float4 h, hn;
+                sceneClosest<true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, nv, nt, sc, h, hn);
+                P.hit[slot] = h;
+                if (__float_as_uint(h.w) != DR_MISS && (__float_as_uint(h.w) & DR_VOL_HIT)) P.hitN[slot] = hn;
+            }
         }
     }
     if (statsOn) {
         waveAddStat(P.stats, DR_CNT_NODES, nv);
         waveAddStat(P.stats, DR_CNT_TRIS, nt);
+        if constexpr (VOL) addSdfStats(P.stats, sc);
     }
 }
 
@@ -115,11 +170,39 @@ struct Surf {
     vec2 uv;
     float t;
     uint32_t materialID, objectID, primitiveID;
+    uint32_t materialID2;
+    float materialBlend;
     bool frontFacing;
 };
-__device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, float4 h) {
+// Sparse-volume hit record (closestVolumeHit's Hit fields, :3397-3418): the
+// tangent frame is built from the normal with the helper axis rule.
+__device__ __forceinline__ Surf volumeSurfaceAt(const DrSceneDev &S, vec3 o, vec3 d, float4 h, float4 hn) {
     Surf s;
     uint32_t id = __float_as_uint(h.w);
+    uint32_t ab = __float_as_uint(h.z);
+    const DrVolume &vol = S.sdf.volumes[id & 0x3fffffffu];
+    s.t = h.x;
+    s.position = o + d * h.x;
+    s.normal = v3(hn.x, hn.y, hn.z);
+    s.uv = vec2{0.0f, 0.0f};
+    vec3 helper = fabsf(s.normal.y) < 0.999f ? v3(0, 1, 0) : v3(1, 0, 0);
+    s.tangent = normalize(cross(helper, s.normal));
+    s.bitangent = normalize(cross(s.normal, s.tangent));
+    s.materialID = ab & 0xffffu;
+    s.materialID2 = ab >> 16;
+    s.materialBlend = clampf(__uint_as_float(__float_as_uint(h.y)), 0.0f, 1.0f);
+    s.objectID = vol.metadata.y;
+    s.primitiveID = vol.metadata.z;
+    s.frontFacing = (id & DR_VOL_FRONT) != 0u;
+    return s;
+}
+template <bool VOL>
+__device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, float4 h, float4 hn) {
+    Surf s;
+    uint32_t id = __float_as_uint(h.w);
+    if constexpr (VOL) {
+        if (id & DR_VOL_HIT) return volumeSurfaceAt(S, o, d, h, hn);
+    }
     const DrTriangle &tri = S.triangles[id];
     float t = h.x, u = h.y, v = h.z;
     float w = 1.0f - u - v;
@@ -133,14 +216,22 @@ __device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, f
     s.tangent = normalize(xyz(tri.tangent));
     s.bitangent = normalize(xyz(tri.bitangent));
     s.materialID = tri.materialID;
+    s.materialID2 = tri.materialID;
+    s.materialBlend = 0.0f;
     s.objectID = tri.objectID;
     s.primitiveID = id;
     return s;
 }
 
-__device__ __forceinline__ Mat materialFor(const DrSceneDev &S, uint32_t materialID) {  // :1096-1137 (triangles)
-    const DrMaterial *p = &S.materials[min(materialID, S.materialCount - 1u)];
-    return Mat{p, p->s[M_BASE]};
+__device__ __forceinline__ Mat materialFor(const DrSceneDev &S, const Surf &h) {  // materialForHit :1096-1137
+    const DrMaterial *p = &S.materials[min(h.materialID, S.materialCount - 1u)];
+    Mat m{p, p, 0.0f, p->s[M_BASE]};
+    if (h.materialBlend > 0.0f) {
+        m.q = &S.materials[min(h.materialID2, S.materialCount - 1u)];
+        m.t = h.materialBlend;
+        m.base = mix4(m.base, m.q->s[M_BASE], m.t);
+    }
+    return m;
 }
 
 __device__ __forceinline__ void applyNormalMap(const DrSceneDev &S, Surf &h, const Mat &m, vec3 rayDir) {  // :4021-4054
@@ -196,18 +287,13 @@ __device__ __forceinline__ float boundaryPDF(vec3 sigmaT, float d) {
     return fmaxf((tr.x + tr.y + tr.z) * 0.33333333333f, 1e-6f);
 }
 
-// Closest hit used by the nested queries of the subsurface walk.
-__device__ __forceinline__ bool traceClosest(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, DrHitRec &h,
-                                             uint32_t &nv, uint32_t &nt) {
-    DrRayPre r = dr_prep(o, d);
-    h = DrHitRec{__builtin_inff(), 0.0f, 0.0f, 0xffffffffu};
-    return dr_traverse<false>(S, r, h, stk, nv, nt);
-}
 
+template <bool VOL>
 __device__ __forceinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, const Surf &entry, const Mat &m, vec3 &ro,
                                                           vec3 &rd, vec3 &T, float &prevPdf, uint32_t &state,
                                                           DrStack &stk, uint32_t &cBounce, uint32_t &cNodes,
-                                                          uint32_t &cTris, uint32_t &cSss) {  // :3525-3621
+                                                          uint32_t &cTris, uint32_t &cSss,
+                                                          DrSdfCounters &sc) {  // :3525-3621
     float subsurface = materialSubsurface(m);
     if (subsurface <= 0.0f || !entry.frontFacing) return false;
     float e0 = randomFloat(state);
@@ -221,9 +307,9 @@ __device__ __forceinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S
     for (uint32_t step = 0u; step < 8u; ++step) {
         cBounce++;
         cSss++;
-        DrHitRec hr;
-        if (!traceClosest(S, mo, md, stk, hr, cNodes, cTris)) return false;
-        Surf b = surfaceAt(S, mo, md, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
+        float4 hr, hn;
+        if (!sceneClosest<VOL>(S, mo, md, stk, cNodes, cTris, sc, hr, hn)) return false;
+        Surf b = surfaceAt<VOL>(S, mo, md, hr, hn);
         float f0 = randomFloat(state);
         float f1 = randomFloat(state);
         float ff = sampleFreeFlight(sigmaT, f0, f1);
@@ -342,6 +428,7 @@ __device__ __forceinline__ bool continueTransmissiveSurface(const Surf &h, const
 __device__ __forceinline__ void writeAOV(float4 *plane, uint32_t pix, float4 v) { plane[pix] = v; }
 
 // ----------------------------------------------------------------- shade --
+template <bool VOL>
 __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
                                                          DrPaths P, DrOutputs O, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
@@ -353,6 +440,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     uint32_t gl = qi;
     DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t cPrimary = 0u, cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
+    DrSdfCounters sc{};
 
     const uint32_t slot = queue[qi];
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
@@ -391,8 +479,10 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
             radiance += clampSampleContribution(c, clampValue);
         }
     } else {
-        Surf h = surfaceAt(S, ro, rd, h4);
-        Mat m = materialFor(S, h.materialID);
+        float4 hn4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+        if (VOL && (id & DR_VOL_HIT)) hn4 = P.hitN[slot];
+        Surf h = surfaceAt<VOL>(S, ro, rd, h4, hn4);
+        Mat m = materialFor(S, h);
         applyNormalMap(S, h, m, rd);
         applyBaseColorTexture(S, m, h);
         if (m.base.w <= 0.001f) {
@@ -432,7 +522,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                 if (subsurface > 0.0f) {
                     float gate = randomFloat(state);
                     if (gate < subsurface &&
-                        continueSubsurfaceRandomWalk(S, h, m, ro, rd, T, prevPdf, state, stk, cBounce, cNodes, cTris, cSss)) {
+                        continueSubsurfaceRandomWalk<VOL>(S, h, m, ro, rd, T, prevPdf, state, stk, cBounce, cNodes, cTris, cSss, sc)) {
                         handled = true;
                         alive = !terminatePathWithRussianRoulette(bounce, T, state);
                     }
@@ -596,12 +686,14 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
         waveAddStat(P.stats, DR_CNT_NODES, cNodes);
         waveAddStat(P.stats, DR_CNT_TRIS, cTris);
         waveAddStat(P.stats, DR_CNT_SSS, cSss);
+        if constexpr (VOL) addSdfStats(P.stats, sc);
     }
 }
 
 // ---------------------------------------------------------------- shadow --
 // shadowOccluded (PathTrace.metal:4196-4274) for one NEE ray; writes the
 // transmittance (0 if occluded) into shC[k].w's companion record shR[k].
+template <bool VOL>
 __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce];
@@ -609,6 +701,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
+    DrSdfCounters sc{};
     while (true) {
         uint32_t base = 0;
         if (laneId() == 0) base = atomicAdd(steal, 64u);
@@ -622,7 +715,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         float maxDistance = o4.w;
         vec3 tr = v3s(1.0f);
         bool occluded;
-        if (S.opaqueOnly) {
+        if (!VOL && S.opaqueOnly) {
             ++nq;
             DrRayPre r = dr_prep(o, d);
             DrHitRec h{maxDistance, 0.0f, 0.0f, 0u};  // id 0: the limit is exclusive
@@ -632,20 +725,30 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
             float traveled = 0.0f;
             for (uint32_t step = 0; step < 8u; ++step) {
                 ++nq;
-                DrRayPre r = dr_prep(o, d);
-                // Hits at or beyond maxDistance - traveled end the chain, so
-                // the search is limited there: exactly (exclusive limit) on
-                // the first step, with a relative margin afterwards where the
-                // reference test traveled + t >= maxDistance decides.
-                float limit = step == 0u ? maxDistance : (maxDistance - traveled) * 1.000001f + 1e-30f;
-                DrHitRec h{limit, 0.0f, 0.0f, step == 0u ? 0u : 0xffffffffu};
-                bool hit = dr_traverse<false>(S, r, h, stk, nv, nt);
-                if (!hit || traveled + h.t >= maxDistance) {
+                float4 h4, hn4;
+                bool hit;
+                if constexpr (!VOL) {
+                    DrRayPre r = dr_prep(o, d);
+                    // Hits at or beyond maxDistance - traveled end the chain,
+                    // so the search is limited there: exactly (exclusive
+                    // limit) on the first step, with a relative margin
+                    // afterwards where the reference test
+                    // traveled + t >= maxDistance decides.
+                    float limit = step == 0u ? maxDistance : (maxDistance - traveled) * 1.000001f + 1e-30f;
+                    DrHitRec h{limit, 0.0f, 0.0f, step == 0u ? 0u : DR_MISS};
+                    hit = dr_traverse<false>(S, r, h, stk, nv, nt);
+                    h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+                } else {
+                    // The volume search is bounded by the closest triangle,
+                    // wherever it lies, so the triangle query is unlimited.
+                    hit = sceneClosest<true>(S, o, d, stk, nv, nt, sc, h4, hn4);
+                }
+                if (!hit || traveled + h4.x >= maxDistance) {
                     occluded = false;
                     break;
                 }
-                Surf sh = surfaceAt(S, o, d, make_float4(h.t, h.u, h.v, __uint_as_float(h.id)));
-                Mat m = materialFor(S, sh.materialID);
+                Surf sh = surfaceAt<VOL>(S, o, d, h4, hn4);
+                Mat m = materialFor(S, sh);
                 if (m.base.w <= 0.001f) {
                     traveled += sh.t;
                     o = sh.position + d * 0.001f;
@@ -668,6 +771,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         waveAddStat(P.stats, DR_CNT_SHADOW, nq);
         waveAddStat(P.stats, DR_CNT_SHADOW_NODES, nv);
         waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, nt);
+        if constexpr (VOL) addSdfStats(P.stats, sc, DR_CNT_SDF_SHADOW_BASE);
     }
 }
 
@@ -701,13 +805,15 @@ __global__ void k_resolve(DrLaunch L, DrPaths P, uint32_t bounce) {
 // ---------------------------------------------------------------- finish --
 // Primary AOV ray used when a denoiser is enabled (tracePrimaryAOV,
 // PathTrace.metal:4093-4160).
-__device__ bool tracePrimaryAOV(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, Surf &out, Mat &outM) {
+template <bool VOL>
+__device__ __forceinline__ bool tracePrimaryAOV(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, Surf &out, Mat &outM,
+                                DrSdfCounters &sc) {
     for (uint32_t pass = 0; pass < 32u; ++pass) {
-        DrHitRec hr;
+        float4 hr, hn;
         uint32_t nv = 0u, nt = 0u;
-        if (!traceClosest(S, o, d, stk, hr, nv, nt)) return false;
-        Surf h = surfaceAt(S, o, d, make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id)));
-        Mat m = materialFor(S, h.materialID);
+        if (!sceneClosest<VOL>(S, o, d, stk, nv, nt, sc, hr, hn)) return false;
+        Surf h = surfaceAt<VOL>(S, o, d, hr, hn);
+        Mat m = materialFor(S, h);
         applyNormalMap(S, h, m, d);
         applyBaseColorTexture(S, m, h);
         if (m.base.w <= 0.001f) {
@@ -721,6 +827,7 @@ __device__ bool tracePrimaryAOV(const DrSceneDev &S, vec3 o, vec3 d, DrStack &st
     return false;
 }
 
+template <bool VOL>
 __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
                                                             DrPaths P, DrOutputs O) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
@@ -751,7 +858,9 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLau
         makeCameraRay(cam, L.width, L.height, px, py, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
         Surf h;
         Mat m;
-        has = tracePrimaryAOV(S, o, d, stk, h, m);
+        DrSdfCounters sc{};
+        has = tracePrimaryAOV<VOL>(S, o, d, stk, h, m, sc);
+        if (VOL && L.statsEnabled) addSdfStats(P.stats, sc);
         if (has) {
             vec3 en = h.normal * 0.5f + v3s(0.5f);
             float mid = (float)(h.materialID + 1u), oid = (float)(h.objectID + 1u);
@@ -788,27 +897,42 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
                        uint32_t traceGrid, hipStream_t st) {
     const uint32_t bThreads = 256;
+    const bool vol = S.sdf.volumeCount != 0u;
     switch (stage) {
     case DR_STAGE_RAYGEN:
         hipLaunchKernelGGL(k_raygen, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, cam, P, nSlots);
         break;
     case DR_STAGE_EXTEND:
-        hipLaunchKernelGGL(k_extend, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        if (vol)
+            hipLaunchKernelGGL(k_extend<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        else
+            hipLaunchKernelGGL(k_extend<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         break;
     case DR_STAGE_SHADE:
-        hipLaunchKernelGGL(k_shade, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK), 0, st,
-                           S, L, cam, prevCam, P, O, bounce);
+        if (vol)
+            hipLaunchKernelGGL(k_shade<true>, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK), 0,
+                               st, S, L, cam, prevCam, P, O, bounce);
+        else
+            hipLaunchKernelGGL(k_shade<false>, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK),
+                               0, st, S, L, cam, prevCam, P, O, bounce);
         break;
     case DR_STAGE_SHADOW:
-        hipLaunchKernelGGL(k_shadow, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        if (vol)
+            hipLaunchKernelGGL(k_shadow<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        else
+            hipLaunchKernelGGL(k_shadow<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         break;
     case DR_STAGE_RESOLVE:
         hipLaunchKernelGGL(k_resolve, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, P, bounce);
         break;
     case DR_STAGE_FINALIZE: {
         uint32_t tilePixels = L.tileWidth * L.tileHeight;
-        hipLaunchKernelGGL(k_finalize, dim3((tilePixels + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK),
-                           0, st, S, L, cam, prevCam, P, O);
+        if (vol)
+            hipLaunchKernelGGL(k_finalize<true>, dim3((tilePixels + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK),
+                               dim3(DR_SHADE_BLOCK), 0, st, S, L, cam, prevCam, P, O);
+        else
+            hipLaunchKernelGGL(k_finalize<false>, dim3((tilePixels + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK),
+                               dim3(DR_SHADE_BLOCK), 0, st, S, L, cam, prevCam, P, O);
         break;
     }
     default:

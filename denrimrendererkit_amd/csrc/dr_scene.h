@@ -52,6 +52,42 @@ struct DrNode {
 };
 static_assert(sizeof(DrNode) == 64, "node layout");
 
+// Compact brick header (64 B).
+struct DrBrick {
+    float4 smin;  // sampleBoundsMin.xyz, min distance
+    float4 smax;  // sampleBoundsMax.xyz, max distance
+    float4 cmin;  // core localBoundsMin.xyz, bits(dims.x | dims.y << 8 | dims.z << 16)
+    float4 cmax;  // core localBoundsMax.xyz, bits(sample offset)
+};
+static_assert(sizeof(DrBrick) == 64, "brick header");
+
+// GPUVolumeDescriptor (PathTrace.metal:74-90).
+struct DrVolume {
+    float4 worldBoundsMin, worldBoundsMax, localBoundsMin, localBoundsMax;
+    uint4 dimensions, metadata, materialProgram;
+    float4 w2l[4];
+    float4 nrm[4];
+};
+static_assert(sizeof(DrVolume) == 240, "GPUVolumeDescriptor layout");
+
+// GPUVolumeBrickGrid (PathTrace.metal:143-148).
+struct DrBrickGrid {
+    uint4 dimensionsAndIndexOffset, brickSizeAndVolume, macroDimensionsAndIndexOffset, macroSizeAndReserved;
+};
+
+struct DrSdfDev {
+    const DrVolume *volumes;
+    const DrBrick *bricks;
+    const uint16_t *dist16;     // FP16 bits (distHalf != 0)
+    const float *dist32;        // FP32 (distHalf == 0)
+    const uint2 *material;      // x: A | B << 16, y: bits(blend)
+    const DrBrickGrid *grids;
+    const uint32_t *gridIndices;
+    uint32_t volumeCount, brickCount, sampleCount, gridCount, gridIndexCount;
+    uint32_t renderQuality;     // 2 = final
+    uint32_t distHalf;
+};
+
 struct DrSceneDev {
     const DrNode *nodes;
     const float4 *triPos;        // 3 float4 per triangle in leaf order
@@ -66,6 +102,7 @@ struct DrSceneDev {
     uint32_t envTexIndexPlusOne;
     float envIntensity, envRotationY, envMaxRadiance;
     uint32_t opaqueOnly;         // 1: no material can let a shadow ray through
+    DrSdfDev sdf;                // sparse distance volumes (volumeCount 0: none)
 };
 
 // Per-launch settings: the fields of GPURenderConstants
@@ -100,5 +137,17 @@ enum {
     DR_CNT_SHADOW_NODES = 5,
     DR_CNT_SHADOW_TRIS = 6,
     DR_CNT_SSS = 7,  // subsurface-walk queries (also counted in BOUNCE)
-    DR_CNT_SLOTS = 8
+    // Sparse-volume counters, the reference's slots 2-9 (PathTrace.metal:193-200).
+    DR_CNT_SDF_GRID_CELLS = 8,
+    DR_CNT_SDF_MACRO_SKIPS = 9,
+    DR_CNT_SDF_BRICK_TESTS = 10,
+    DR_CNT_SDF_BRICK_INVALID = 11,
+    DR_CNT_SDF_RANGE_CULLS = 12,
+    DR_CNT_SDF_MARCHES = 13,
+    DR_CNT_SDF_MARCH_STEPS = 14,
+    DR_CNT_SDF_BRICK_HITS = 15,
+    // The same eight counters for NEE shadow queries (k_shadow) only; the
+    // reference's totals are the sums of both groups.
+    DR_CNT_SDF_SHADOW_BASE = 16,
+    DR_CNT_SLOTS = 24
 };

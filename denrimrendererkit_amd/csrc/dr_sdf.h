@@ -6,57 +6,19 @@
 //   intersectVolumeBrickPrepared    :2608-2807 (back-track, sphere trace <= 192
 //                                   steps, 8 bisections, cubic polish)
 //   trilinear / Catmull-Rom gradients and volumeBrickNormal :2096-2469
-// Storage is this backend's: distances as FP16 (widened to FP32 on load, so
-// tracing is FP32), one u32 material word per sample (A | B << 8 |
-// round(blend * 65535) << 16), and a 64-B brick header. With FP16-exact
-// input the values seen here equal the reference-layout FP32 values, and the
-// arithmetic (only + - * / floor round sqrt) is IEEE, so hits are bit-exact
-// with an FP32 evaluation of the reference over the same field.
+// Storage is this backend's: distances as FP16 when every input distance is
+// exactly representable in FP16 (widened to FP32 on load, so tracing is FP32),
+// else as FP32; materials as one uint2 per sample (A | B << 16, bits of the
+// blend weight), read once per hit; a 64-B brick header. The values seen here
+// equal the reference-layout FP32 values, and the arithmetic (only + - * /
+// floor round sqrt) is IEEE, so hits are bit-exact with an FP32 evaluation of
+// the reference over the same field.
 #pragma once
 
 #include "dr_math.h"
 #include "dr_scene.h"
 
-// Compact brick header (64 B).
-struct DrBrick {
-    float4 smin;  // sampleBoundsMin.xyz, min distance
-    float4 smax;  // sampleBoundsMax.xyz, max distance
-    float4 cmin;  // core localBoundsMin.xyz, bits(dims.x | dims.y << 8 | dims.z << 16)
-    float4 cmax;  // core localBoundsMax.xyz, bits(sample offset)
-};
-static_assert(sizeof(DrBrick) == 64, "brick header");
-
-// GPUVolumeDescriptor (PathTrace.metal:74-90).
-struct DrVolume {
-    float4 worldBoundsMin, worldBoundsMax, localBoundsMin, localBoundsMax;
-    uint4 dimensions, metadata, materialProgram;
-    float4 w2l[4];
-    float4 nrm[4];
-};
-static_assert(sizeof(DrVolume) == 240, "GPUVolumeDescriptor layout");
-
-// GPUVolumeBrickGrid (PathTrace.metal:143-148).
-struct DrBrickGrid {
-    uint4 dimensionsAndIndexOffset, brickSizeAndVolume, macroDimensionsAndIndexOffset, macroSizeAndReserved;
-};
-
-struct DrSdfDev {
-    const DrVolume *volumes;
-    const DrBrick *bricks;
-    const uint16_t *dist;       // FP16 bits
-    const uint32_t *material;   // A | B << 8 | q16 << 16
-    const DrBrickGrid *grids;
-    const uint32_t *gridIndices;
-    uint32_t volumeCount, brickCount, sampleCount, gridCount, gridIndexCount;
-    uint32_t renderQuality;     // 2 = final
-};
-
-// SDF counter slots in the session stats (reference slots 2..9 at 8..15).
-enum {
-    DR_SC_GRID_CELLS = 8, DR_SC_MACRO_SKIPS, DR_SC_BRICK_TESTS, DR_SC_BRICK_INVALID, DR_SC_RANGE_CULLS,
-    DR_SC_MARCHES, DR_SC_MARCH_STEPS, DR_SC_BRICK_HITS
-};
-
+// Per-lane sparse-volume counters, added to DR_CNT_SDF_* at the end of a kernel.
 struct DrSdfCounters {
     uint32_t gridCells, macroSkips, brickTests, invalid, rangeCulls, marches, marchSteps, hits;
 };
@@ -64,12 +26,12 @@ struct DrSdfCounters {
 struct DrVolHit {
     float t;
     vec3 normal;
-    uint32_t material;  // packed material word
+    uint2 material;  // A | B << 16, bits(blend)
     uint32_t volume;
     bool frontFacing;
 };
 
-__device__ __forceinline__ float dr_half(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ float dr_half(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
 __device__ __forceinline__ vec3 mulCols(const float4 *c, vec3 v, float w) {
     return vec3{c[0].x * v.x + c[1].x * v.y + c[2].x * v.z + c[3].x * w,
@@ -114,7 +76,8 @@ __device__ __forceinline__ DrBrickCtx brickCtx(const DrSdfDev &V, const DrBrick 
 }
 
 __device__ __forceinline__ float sdAt(const DrSdfDev &V, const DrBrickCtx &c, uint32_t x, uint32_t y, uint32_t z) {
-    return dr_half(V.dist[c.offset + x + y * c.strideY + z * c.strideZ]);
+    uint32_t i = c.offset + x + y * c.strideY + z * c.strideZ;
+    return V.distHalf ? dr_half(V.dist16[i]) : V.dist32[i];
 }
 
 __device__ __forceinline__ void brickGrid(const DrBrickCtx &c, vec3 p, uint32_t b[3], uint32_t n[3], vec3 &f) {
@@ -183,7 +146,7 @@ __device__ __forceinline__ float catmullRomD(float p0, float p1, float p2, float
                    3.0f * (-p0 + 3.0f * p1 - 3.0f * p2 + p3) * t2);
 }
 
-__device__ __noinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2257-2314
+__device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2257-2314
     DrDG inv{__builtin_inff(), v3s(0.0f)};
     if (!c.valid) return inv;
     vec3 uvw = vclamp((p - c.smin) / c.extent, v3s(0.0f), v3s(1.0f));
@@ -218,16 +181,12 @@ __device__ __noinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, const
     return r;
 }
 
-__device__ __forceinline__ float brickDistanceAt(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {
-    return brickDistance(V, c, p);
-}
-
-__device__ __noinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &vol, const DrBrickCtx &c, vec3 p,
+__device__ __forceinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &vol, const DrBrickCtx &c, vec3 p,
                                          bool cubic, bool coarse) {  // :2420-2469
     if (!c.valid) return v3(0, 1, 0);
     vec3 ln = cubic ? brickDistanceGradientCubic(V, c, p).gradient : brickDistanceGradient(V, c, p).gradient;
     if (coarse) {
-        vec3 ext = vmax(xyzf(vol.localBoundsMax) - xyzf(vol.localBoundsMin), v3s(1e-6f));
+        vec3 ext = vmax(v3f(vol.localBoundsMax) - v3f(vol.localBoundsMin), v3s(1e-6f));
         vec3 dims = v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z);
         vec3 sp = (ext / vmax(dims - v3s(1.0f), v3s(1.0f))) * 1.5f;
         vec3 mn = c.smin, mx = c.smax;
@@ -250,7 +209,7 @@ __device__ __noinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &vol,
     return normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
 }
 
-__device__ __noinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd, const DrVolume &vol,
+__device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd, const DrVolume &vol,
                                             const DrBrick &brick, float closestT, vec3 lo, vec3 ld, float lupw,
                                             float cellSize, DrSdfCounters &sc, DrVolHit &hit) {  // :2608-2807
     float tNear, tFar;
@@ -327,9 +286,9 @@ __device__ __noinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd,
             if (!ff) n = -n;
             // Nearest sample's material (sampleVolumeBrickMaterial, :2350-2356).
             vec3 uvw = vclamp((rp - ctx.smin) / ctx.extent, v3s(0.0f), v3s(1.0f));
-            uint32_t nx = min((uint32_t)rintf(uvw.x * (float)(ctx.dim[0] - 1u)), ctx.dim[0] - 1u);
-            uint32_t ny = min((uint32_t)rintf(uvw.y * (float)(ctx.dim[1] - 1u)), ctx.dim[1] - 1u);
-            uint32_t nz = min((uint32_t)rintf(uvw.z * (float)(ctx.dim[2] - 1u)), ctx.dim[2] - 1u);
+            uint32_t nx = min((uint32_t)roundf(uvw.x * (float)(ctx.dim[0] - 1u)), ctx.dim[0] - 1u);
+            uint32_t ny = min((uint32_t)roundf(uvw.y * (float)(ctx.dim[1] - 1u)), ctx.dim[1] - 1u);
+            uint32_t nz = min((uint32_t)roundf(uvw.z * (float)(ctx.dim[2] - 1u)), ctx.dim[2] - 1u);
             hit.t = refinedT;
             hit.normal = n;
             hit.material = V.material[ctx.offset + nx + ny * ctx.strideY + nz * ctx.strideZ];
@@ -345,7 +304,7 @@ __device__ __noinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd,
 }
 
 // Closest sparse-volume hit below closestT (closestVolumeHit, brick-grid path).
-__device__ __noinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
+__device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
                                               DrVolHit &out) {
     bool any = false;
     float best = closestT;
@@ -367,11 +326,11 @@ __device__ __noinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 r
                           max(grid->brickSizeAndVolume.z, 1u)};
         if (gd[0] == 0u || gd[1] == 0u || gd[2] == 0u) continue;
         float tNear, tFar;
-        if (!sdfAABB(ro, rd, xyzf(vol.worldBoundsMin), xyzf(vol.worldBoundsMax), best, tNear, tFar)) continue;
+        if (!sdfAABB(ro, rd, v3f(vol.worldBoundsMin), v3f(vol.worldBoundsMax), best, tNear, tFar)) continue;
         vec3 lo = mulCols(vol.w2l, ro, 1.0f);
         vec3 ld = mulCols(vol.w2l, rd, 0.0f);
-        vec3 lbmin = xyzf(vol.localBoundsMin);
-        vec3 extent = vmax(xyzf(vol.localBoundsMax) - lbmin, v3s(1e-6f));
+        vec3 lbmin = v3f(vol.localBoundsMin);
+        vec3 extent = vmax(v3f(vol.localBoundsMax) - lbmin, v3s(1e-6f));
         vec3 den = vmax(v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z) - v3s(1.0f),
                         v3s(1.0f));
         vec3 bsf = vmax(v3((float)bs[0], (float)bs[1], (float)bs[2]), v3s(1.0f));

@@ -12,13 +12,22 @@
 // A material is read in place from the scene's GPUMaterial array; only the
 // base colour can be changed per hit (base-colour texture), so it is held
 // by value.
+// A hit's material: record p, or p blended toward q by t when t > 0
+// (blendMaterials, PathTrace.metal:549-569, as materialForHit applies it for
+// sparse-volume samples). base holds the (textured) base colour.
 struct Mat {
     const DrMaterial *p;
+    const DrMaterial *q;
+    float t;
     float4 base;
 };
 template <int S> __device__ __forceinline__ float4 ms(const Mat &m) {
     if constexpr (S == M_BASE) return m.base;
-    else return m.p->s[S];
+    else {
+        float4 v = m.p->s[S];
+        if (m.t > 0.0f) v = mix4(v, m.q->s[S], m.t);
+        return v;
+    }
 }
 __device__ __forceinline__ vec3 xyz(float4 f) { return vec3{f.x, f.y, f.z}; }
 

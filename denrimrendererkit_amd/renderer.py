@@ -16,7 +16,7 @@ session.resetAccumulation()            session.reset_accumulation()
 session.updateCamera(_:previous...)    session.update_camera(camera, previous_camera)
 session.sampleCount                    session.sample_count
 session.pixels(for:)                   session.pixels(output) -> (H, W, 4) float32
-session.sdfTraversalStats()            session.stats()
+session.sdfTraversalStats()            session.sdf_traversal_stats()
 =====================================  ==========================================
 """
 from __future__ import annotations
@@ -32,6 +32,26 @@ from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, 
 from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
 
 STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize")
+N_STATS = 24
+
+
+@dataclass(frozen=True)
+class SDFTraversalStats:
+    """SDFTraversalStats (Renderer/RenderSession.swift:41-73)."""
+    dense_volume_tests: int = 0
+    dense_march_steps: int = 0
+    sparse_grid_cells_visited: int = 0
+    sparse_grid_empty_cells: int = 0
+    sparse_grid_macro_skips: int = 0
+    sparse_brick_tests: int = 0
+    sparse_brick_invalid: int = 0
+    sparse_brick_range_culls: int = 0
+    sparse_brick_marches: int = 0
+    sparse_brick_march_steps: int = 0
+    sparse_brick_hits: int = 0
+    primary_scene_queries: int = 0
+    bounce_scene_queries: int = 0
+    shadow_scene_queries: int = 0
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -74,7 +94,7 @@ class DenrimRenderer:
                                      RenderAccelerationMode.FLAT_BVH):
             raise DenrimRendererError("missingShaderFunction", f"{acceleration_mode.value} is not available")
         if settings.width <= 0 or settings.height <= 0:
-            raise DenrimRendererError.invalid_scene("Render size must be greater than zero.")
+            raise DenrimRendererError.invalid_scene("Render dimensions must be positive.")
         if isinstance(scene, RenderScene):
             gpu_scene = self.upload(compile_scene(scene))
             cam = camera or scene.camera
@@ -204,9 +224,22 @@ class RenderSession:
         return self.pixels(RenderOutput.TRIANGLE_ID)[..., 0].view(np.uint32).copy()
 
     def stats(self) -> np.ndarray:
-        out = (C.c_uint64 * 8)()
-        _lib.check(_lib.load().dr_hip_session_stats(self._h, out, 8), "stats")
+        """All counters (include/denrim_hip.h dr_stat order), uint64."""
+        out = (C.c_uint64 * N_STATS)()
+        _lib.check(_lib.load().dr_hip_session_stats(self._h, out, N_STATS), "stats")
         return np.array(list(out), dtype=np.uint64)
+
+    def sdf_traversal_stats(self) -> SDFTraversalStats:
+        """sdfTraversalStats() (RenderSession.swift:807-831). The dense-volume
+        counters are zero: dense volumes are not traced by this backend."""
+        s = [int(x) for x in self.stats()]
+        c = [s[8 + k] + s[16 + k] for k in range(8)]  # extension/subsurface + shadow queries
+        cells, tests = c[0], c[2]
+        return SDFTraversalStats(0, 0, cells, cells - min(cells, tests), c[1], tests, c[3], c[4], c[5], c[6],
+                                 c[7], s[0], s[1], s[2])
+
+    def reset_sdf_traversal_stats(self) -> None:
+        self.reset_stats()
 
     def reset_stats(self) -> None:
         _lib.check(_lib.load().dr_hip_session_reset_stats(self._h), "reset stats")

@@ -59,6 +59,34 @@ class SparseDistanceVolume:
     def grid_dimensions(self) -> np.ndarray:
         return (self.dimensions + self.brick_size - 1) // self.brick_size
 
+    @staticmethod
+    def from_bricks(dimensions, brick_size: int, bounds_min, bounds_max, bricks) -> "SparseDistanceVolume":
+        """SparseDistanceVolume(dimensions:brickSize:boundsMin:boundsMax:bricks:)
+        (Scene/DistanceVolume.swift:309-491) from explicit bricks. Each brick
+        is a dict with origin, dimensions, core_origin, core_dimensions,
+        distances (x fastest) and material_a, plus optional material_b and
+        blend (scalars or per-sample arrays)."""
+        so, sd, co, cd, dist, ma, mb, bl = [], [], [], [], [], [], [], []
+        for br in bricks:
+            dims = np.asarray(br["dimensions"], np.int32)
+            n = int(np.prod(dims))
+            d = np.asarray(br["distances"], f32).reshape(-1)
+            if d.size != n:
+                from .scene import DenrimRendererError
+                raise DenrimRendererError.invalid_scene("Sparse distance volume brick sample count does not match its dimensions.")
+            so.append(br["origin"]); sd.append(dims); co.append(br["core_origin"]); cd.append(br["core_dimensions"])
+            dist.append(d)
+            ma.append(np.broadcast_to(np.asarray(br["material_a"], np.uint32), (n,)))
+            mb.append(np.broadcast_to(np.asarray(br.get("material_b", br["material_a"]), np.uint32), (n,)))
+            bl.append(np.broadcast_to(np.asarray(br.get("blend", 0.0), f32), (n,)))
+        counts = np.array([int(np.prod(x)) for x in sd], np.int64)
+        offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64) if len(counts) else np.zeros(0, np.int64)
+        cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)  # noqa: E731
+        arr = lambda xs: np.asarray(xs, np.int32).reshape(-1, 3)  # noqa: E731
+        return SparseDistanceVolume(np.asarray(dimensions, np.int64), int(brick_size), np.asarray(bounds_min, f32),
+                                    np.asarray(bounds_max, f32), arr(so), arr(sd), arr(co), arr(cd), offsets,
+                                    cat(dist, f32), cat(ma, np.uint32), cat(mb, np.uint32), cat(bl, f32))
+
 
 def build_sparse_volume(primitives: List[SDFPrimitive], dims=(768, 768, 768), brick_size: int = 8, apron: int = 1,
                         narrow_band: float = 2.0 / 767.0, smooth_radius: float = 0.05,

@@ -106,7 +106,7 @@ typedef struct {
 typedef struct {
     uint32_t width, height;
     uint32_t max_bounces;             /* clamped to >= 1 like RenderSession.swift:916 */
-    uint32_t quality;                 /* 0 preview, 1 interactive, 2 final; only final (the path tracer) is served */
+    uint32_t quality;                 /* 0 preview, 1 interactive, 2 final: constants.renderQuality (RenderSettings.swift:106-115) */
     float sample_radiance_clamp;      /* < 0: the quality default (10 / 24 / 64) */
     uint32_t transparent_background;
     uint32_t shows_environment_background;
@@ -129,7 +129,20 @@ typedef enum {
     DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
     DR_STAT_SHADOW_TRIANGLE_TESTS = 6,
     DR_STAT_SUBSURFACE_QUERIES = 7,   /* random-walk queries, also in BOUNCE_QUERIES */
-    DR_STAT_COUNT = 8
+    /* Sparse-volume counters, sdfCounterSparse* (PathTrace.metal:193-200). */
+    DR_STAT_SPARSE_GRID_CELLS_VISITED = 8,
+    DR_STAT_SPARSE_GRID_MACRO_SKIPS = 9,
+    DR_STAT_SPARSE_BRICK_TESTS = 10,
+    DR_STAT_SPARSE_BRICK_INVALID = 11,
+    DR_STAT_SPARSE_BRICK_RANGE_CULLS = 12,
+    DR_STAT_SPARSE_BRICK_MARCHES = 13,
+    DR_STAT_SPARSE_BRICK_MARCH_STEPS = 14,
+    DR_STAT_SPARSE_BRICK_HITS = 15,
+    /* 16-23: the eight sparse-volume counters again, for NEE shadow queries
+     * only (the totals above exclude them; the reference's counters are the
+     * sums of both groups). */
+    DR_STAT_SHADOW_SPARSE_BASE = 16,
+    DR_STAT_COUNT = 24
 } dr_stat;
 
 /* Sparse-field primitive for dr_sdf_build_sparse (SDFPrimitive,

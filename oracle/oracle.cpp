@@ -774,7 +774,7 @@ template <class R> void initCtx(Ctx<R> &c) {
     c.sdf.gridCount = uint32_t(s.brickGrids.size());
     c.sdf.gridIndices = s.brickGridIndices.empty() ? nullptr : s.brickGridIndices.data();
     c.sdf.gridIndexCount = uint32_t(s.brickGridIndices.size());
-    c.sdf.renderQuality = 2u;  // RenderQuality.final
+    c.sdf.renderQuality = c.st->renderQuality;  // constants.renderQuality (RenderSettings.swift:106-115)
 }
 
 // intersectScene (PathTrace.metal:3294-3421, triangle part) and
@@ -1577,9 +1577,11 @@ int or_render(const or_scene *s, const or_settings *st, const or_camera *cam, co
                           static_cast<float *>(aovs), counters, threads);
 }
 
-int or_intersect(const or_scene *s, const float *rays, uint32_t count, int traversal, float *tOut, uint32_t *idOut) {
+int or_intersect(const or_scene *s, const float *rays, uint32_t count, int traversal, float *tOut, uint32_t *idOut,
+                 float *hitOut) {
     if (!s) return 1;
     or_settings dummy{};
+    dummy.renderQuality = 2u;
     or_camera cam{};
 #pragma omp parallel
     {
@@ -1592,6 +1594,12 @@ int or_intersect(const or_scene *s, const float *rays, uint32_t count, int trave
             Hit<float> h = intersectScene(c, ray);
             tOut[i] = h.hit ? h.t : std::numeric_limits<float>::infinity();
             idOut[i] = h.hit ? h.primitiveID : 0xffffffffu;
+            if (hitOut) {
+                float *o = hitOut + 8 * i;
+                o[0] = h.normal.x, o[1] = h.normal.y, o[2] = h.normal.z;
+                o[3] = float(h.materialID), o[4] = float(h.materialID2), o[5] = h.materialBlend;
+                o[6] = float(h.objectID), o[7] = h.frontFacing ? 1.0f : 0.0f;
+            }
         }
     }
     return 0;

@@ -90,6 +90,7 @@ typedef struct {
     float sampleRadianceClamp;
     float backgroundColor[4];
     uint32_t tileX, tileY, tileWidth, tileHeight;
+    uint32_t renderQuality;  // RenderSettings.shaderQualityLevel: 0 preview, 1 interactive, 2 final
 } or_settings;
 
 typedef struct {
@@ -144,9 +145,11 @@ int or_render(const or_scene *scene, const or_settings *settings,
               uint64_t *counters, int threads);
 
 /* Closest-hit queries for arbitrary rays (fp32 path): rays are 8 floats
- * (ox,oy,oz,_,dx,dy,dz,_); out: t (float), triangle index (u32, ~0 = miss). */
+ * (ox,oy,oz,_,dx,dy,dz,_); out: t (float), primitive index (u32, ~0 = miss),
+ * and, if hitOut is not NULL, 8 floats per ray: shading normal xyz,
+ * materialID, materialID2, materialBlend, objectID, frontFacing. */
 int or_intersect(const or_scene *scene, const float *rays, uint32_t count,
-                 int traversal, float *tOut, uint32_t *idOut);
+                 int traversal, float *tOut, uint32_t *idOut, float *hitOut);
 
 /* RNG restatement (PathTrace.metal:264-276). */
 uint32_t or_hash(uint32_t value);

@@ -1,0 +1,158 @@
+"""GPU parity of the sparse signed-distance path (brick DDA, march,
+bisection, cubic polish, normals, material blend) against the CPU oracle,
+through the C ABI.
+
+Tolerances:
+  * material / object / primitive IDs: bit-exact;
+  * depth, normal, albedo: |diff| <= 1e-5 (the same IEEE operations over
+    the same FP32 values: FP16-stored distances widen exactly);
+  * sparse-volume counters: exactly equal when every ray is a primary ray
+    (no lights, no environment texture, one bounce); within 1% otherwise
+    (shading transcendentals differ by ulps between libm and the GPU and can
+    move a rare secondary ray);
+  * beauty: as the triangle parity tests (at most 1% of channels outside
+    1e-3 * (1 + |ref|), mean |diff| < 2e-3).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from denrimrendererkit_amd import RenderQuality, RenderSettings, compile_scene, synthetic
+from denrimrendererkit_amd.settings import RenderOutput
+from test_sdf import aov_metrics, macro_skip_scene, no_crossing_scene, quality_probe, sdf_stats
+
+pytestmark = pytest.mark.gpu
+
+SDF_NAMES = ("cells", "macro", "tests", "invalid", "culls", "marches", "steps", "hits")
+
+
+def _render_pair(renderer, scene, W, H, spp=1, bounces=4, quality=RenderQuality.FINAL, seed=0):
+    build = compile_scene(scene)
+    settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=quality, frame_seed=seed,
+                              collects_sdf_traversal_stats=True)
+    sess = renderer.make_session(renderer.upload(build), settings, camera=scene.camera)
+    sess.render(spp)
+    ost = O.make_settings(build, W, H, max_bounces=bounces, frame_seed=seed, clamp=settings.resolved_sample_radiance_clamp,
+                          quality=quality.value)
+    ref = O.render(O.OracleScene(build), ost, scene.camera.gpu_camera(W, H), sample_count=spp)
+    return sess, ref
+
+
+def _gpu_sdf(sess):
+    s = sess.sdf_traversal_stats()
+    return dict(cells=s.sparse_grid_cells_visited, macro=s.sparse_grid_macro_skips, tests=s.sparse_brick_tests,
+                invalid=s.sparse_brick_invalid, culls=s.sparse_brick_range_culls, marches=s.sparse_brick_marches,
+                steps=s.sparse_brick_march_steps, hits=s.sparse_brick_hits, empty=s.sparse_grid_empty_cells)
+
+
+def _check_ids_and_aovs(sess, ref):
+    prim = sess.triangle_ids()
+    ref_prim = ref.aovs[6][..., 0].view(np.uint32)
+    assert np.array_equal(prim, ref_prim), f"primitive IDs differ at {int((prim != ref_prim).sum())} px"
+    for out, k in ((RenderOutput.MATERIAL_ID, 3), (RenderOutput.OBJECT_ID, 4)):
+        assert np.array_equal(sess.pixels(out), ref.aovs[k]), out
+    for out, k in ((RenderOutput.DEPTH, 0), (RenderOutput.NORMAL, 1), (RenderOutput.ALBEDO, 2)):
+        d = np.abs(sess.pixels(out) - ref.aovs[k]).max()
+        assert d <= 1e-5, (out, d)
+
+
+def _check_beauty(sess, ref, frac_tol=0.01, mean_tol=2e-3):
+    g, r = sess.pixels(RenderOutput.BEAUTY), ref.accumulation
+    assert np.isfinite(g).all()
+    diff = np.abs(g - r)
+    bad = (diff > 1e-3 * (1.0 + np.abs(r))).mean()
+    assert bad <= frac_tol, f"{bad:.4%} channels outside tolerance"
+    assert diff.mean() < mean_tol, diff.mean()
+
+
+def test_sdf_bricks_scene_parity(renderer):
+    # The sparse-SDF workload at a reduced virtual grid (96^3) and image.
+    scene = synthetic.sdf_bricks_scene(resolution=96, primitives=60, hdri=(256, 128))
+    sess, ref = _render_pair(renderer, scene, 96, 96, spp=1, bounces=4)
+    _check_ids_and_aovs(sess, ref)
+    _check_beauty(sess, ref)
+    g, o = _gpu_sdf(sess), sdf_stats(ref.counters)
+    assert g["hits"] > 0 and g["marches"] > 0
+    for k in SDF_NAMES:
+        assert abs(g[k] - o[k]) <= 0.01 * max(o[k], 1) + 2, (k, g[k], o[k])
+
+
+@pytest.mark.parametrize("quality", [RenderQuality.PREVIEW, RenderQuality.INTERACTIVE, RenderQuality.FINAL])
+def test_sdf_primary_counters_exact(renderer, quality):
+    # No lights and no environment texture, one bounce: every query is a
+    # primary ray, so the counters are the same sums of the same traversals.
+    from denrimrendererkit_amd import Camera, Material, RenderScene
+    from denrimrendererkit_amd.sdf import build_sparse_volume
+    scene = RenderScene(camera=Camera(origin=(0.5, 0.9, 2.0), target=(0.5, 0.45, 0.5)))
+    metal = scene.add_material(Material(base_color=(0.9, 0.75, 0.55), metallic=1.0, roughness=0.3))
+    matte = scene.add_material(Material(base_color=(0.7, 0.7, 0.7), specular=0.0, roughness=0.5))
+    prims = synthetic.sdf_primitives(synthetic.content_rng(3), 40, metal, matte)
+    scene.add_sparse_volume(build_sparse_volume(prims, dims=(64, 64, 64), narrow_band=2 / 63), metal)
+    sess, ref = _render_pair(renderer, scene, 80, 64, spp=1, bounces=1, quality=quality)
+    _check_ids_and_aovs(sess, ref)
+    g, o = _gpu_sdf(sess), sdf_stats(ref.counters)
+    for k in SDF_NAMES:
+        assert g[k] == o[k], (k, g[k], o[k])
+    assert sess.stats()[0] == ref.counters[O.CNT_PRIMARY]
+
+
+def test_sparse_brick_without_zero_crossing_is_culled_before_march(renderer):
+    # RenderSessionTests.swift:862-919; the samples (0.05) are not FP16-exact,
+    # so this also covers the FP32 distance storage.
+    scene = no_crossing_scene()
+    sess, ref = _render_pair(renderer, scene, 8, 8, bounces=1)
+    g = _gpu_sdf(sess)
+    assert g["cells"] > 0 and g["tests"] > 0 and g["cells"] >= g["empty"] + g["tests"]
+    assert g["culls"] > 0 and g["marches"] == 0 and g["steps"] == 0 and g["hits"] == 0
+    assert g == {**sdf_stats(ref.counters)}
+    sess.reset_sdf_traversal_stats()
+    assert _gpu_sdf(sess)["cells"] == 0
+
+
+def test_sparse_volume_uses_macro_grid_skips(renderer):
+    # RenderSessionTests.swift:1069-1120.
+    scene = macro_skip_scene()
+    sess, ref = _render_pair(renderer, scene, 32, 32, bounces=1)
+    g = _gpu_sdf(sess)
+    assert g["macro"] > 0 and g["cells"] > 0 and g["tests"] > 0
+    assert g == sdf_stats(ref.counters)
+    _check_ids_and_aovs(sess, ref)
+
+
+def test_sparse_quality_probe_matches_oracle_and_reference_bounds(renderer):
+    # RenderSessionTests.swift:1024-1067 on the GPU, and each probe equal to
+    # the oracle's.
+    def gpu_probe(scene, w, h, quality):
+        sess, ref = _render_pair(renderer, scene, w, h, bounces=1, quality=RenderQuality(quality))
+        _check_ids_and_aovs(sess, ref)
+        return sess.pixels(RenderOutput.NORMAL), sess.pixels(RenderOutput.DEPTH)
+    low = quality_probe(28, 2, gpu_probe)
+    ref = quality_probe(64, 2, gpu_probe)
+    quality_probe(28, 0, gpu_probe)
+    m = aov_metrics(low, ref)
+    assert m["coverage"] > 0.20 and m["normal_rmse"] < 0.30 and m["depth_rmse"] < 0.05 and m["silhouette"] < 0.02
+
+
+def test_two_volumes_with_transform_and_triangles(renderer):
+    # Mixed scene: a ground quad, an untransformed volume and a second,
+    # transformed volume (non-zero volume index), as in
+    # RenderSessionTests.swift:1398-1456.
+    from denrimrendererkit_amd import Camera, Material, Mesh, RenderScene, Transform
+    from denrimrendererkit_amd.sdf import SDFPrimitive, build_sparse_volume
+    scene = RenderScene(camera=Camera(origin=(0.2, 1.0, 3.0), target=(0.2, 0.3, 0.0)))
+    ground = scene.add_material(Material(base_color=(0.5, 0.5, 0.5), roughness=0.8))
+    red = scene.add_material(Material(base_color=(0.8, 0.2, 0.1), roughness=0.4))
+    blue = scene.add_material(Material(base_color=(0.1, 0.3, 0.9), metallic=1.0, roughness=0.25))
+    scene.add(Mesh.quad((-3, 0, 3), (3, 0, 3), (3, 0, -3), (-3, 0, -3)), ground)
+    v1 = build_sparse_volume([SDFPrimitive("sphere", red, (0.5, 0.5, 0.5), 0.3)], dims=(40, 40, 40),
+                             narrow_band=2 / 39)
+    v2 = build_sparse_volume([SDFPrimitive("torus", blue, (0.5, 0.5, 0.5), 0.3, axis=(0, 1, 0), minor_radius=0.1)],
+                             dims=(48, 48, 48), narrow_band=2 / 47)
+    scene.add_sparse_volume(v1, red, Transform.translation((-1.2, -0.2, 0.0)))
+    scene.add_sparse_volume(v2, blue, Transform.translation((0.6, 0.1, -0.3)) * Transform.rotation_x(0.6)
+                            * Transform.scale((1.4, 1.4, 1.4)))
+    sess, ref = _render_pair(renderer, scene, 96, 64, spp=2, bounces=3)
+    _check_ids_and_aovs(sess, ref)
+    _check_beauty(sess, ref)
+    objs = set(np.unique(sess.pixels(RenderOutput.OBJECT_ID)[..., 0]).tolist())
+    assert {1.0, 2.0, 3.0} <= objs  # ground (instance 0) and both volumes
