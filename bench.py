@@ -124,7 +124,9 @@ def main():
     cs.render(1)
     c = cs.stats().astype(np.float64)
     cs.close()
-    sdf_ext, sdf_sh = c[8:16], c[16:24]  # cells, macro skips, brick tests, invalid, culls, marches, steps, hits
+    # SDF counters in the reference's slot order: dense tests, dense steps, cells, macro skips, brick tests,
+    # invalid, culls, marches, march steps, hits; extension/subsurface queries, then shadow queries.
+    sdf_ext, sdf_sh = c[8:18], c[18:28]
     queries_per_step = c[0] + c[1] + c[2]
     log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
         f"shadow {c[2]:.0f}), "
@@ -156,7 +158,7 @@ def main():
     else:
         nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1], sdf_ext
     launches_per_step = max(1, st_n[dom] // max(1, args.steps))
-    sdf_bytes = CELL_BYTES * sdf[0] + BRICK_BYTES * sdf[2] + STEP_BYTES * sdf[6] + HIT_BYTES * sdf[7]
+    sdf_bytes = CELL_BYTES * sdf[2] + BRICK_BYTES * sdf[4] + STEP_BYTES * sdf[8] + HIT_BYTES * sdf[9]
     bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays + sdf_bytes) / launches_per_step
     avg_ms = st_ms[dom] / max(1, st_n[dom])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
@@ -174,9 +176,9 @@ def main():
     if len(build.volume_bricks):
         tot = sdf_ext + sdf_sh
         config.update({"bricks": int(len(build.volume_bricks)), "brick_samples": int(len(build.volume_brick_samples)),
-                       "sdf_per_step": {"grid_cells": int(tot[0]), "macro_skips": int(tot[1]),
-                                        "brick_tests": int(tot[2]), "marches": int(tot[5]),
-                                        "march_steps": int(tot[6]), "hits": int(tot[7])}})
+                       "sdf_per_step": {"grid_cells": int(tot[2]), "macro_skips": int(tot[3]),
+                                        "brick_tests": int(tot[4]), "marches": int(tot[7]),
+                                        "march_steps": int(tot[8]), "hits": int(tot[9])}})
     out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

@@ -24,6 +24,7 @@ EXPORTS = (
     "dr_hip_session_sample_count", "dr_hip_session_read_output", "dr_hip_session_stats",
     "dr_hip_session_reset_stats", "dr_hip_session_set_stage_timing", "dr_hip_session_timing",
     "dr_sdf_build_sparse", "dr_sdf_volume_info", "dr_sdf_volume_export", "dr_sdf_volume_free",
+    "dr_sdf_build_dense",
 )
 
 
@@ -39,6 +40,7 @@ class SceneDesc(C.Structure):
         ("environment_intensity", C.c_float), ("environment_rotation_y", C.c_float),
         ("environment_max_radiance", C.c_float), ("max_leaf_triangles", C.c_uint32),
         ("volumes", C.c_void_p), ("volume_count", C.c_uint32),
+        ("volume_samples", C.c_void_p), ("volume_sample_count", C.c_uint32),
         ("volume_bricks", C.c_void_p), ("volume_brick_count", C.c_uint32),
         ("volume_brick_samples", C.c_void_p), ("volume_brick_sample_count", C.c_uint32),
         ("volume_brick_grids", C.c_void_p), ("volume_brick_grid_count", C.c_uint32),
@@ -48,13 +50,16 @@ class SceneDesc(C.Structure):
 
 class SdfPrimitive(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("material", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float),
-                ("axis", C.c_float * 3), ("minor_radius", C.c_float)]
+                ("axis", C.c_float * 3), ("minor_radius", C.c_float), ("half_extents", C.c_float * 3),
+                ("corner_radius", C.c_float), ("half_height", C.c_float), ("smooth_radius", C.c_float),
+                ("operation", C.c_uint32), ("has_transform", C.c_uint32), ("world_to_primitive", C.c_float * 16),
+                ("distance_scale", C.c_float), ("lipschitz", C.c_float)]
 
 
 class SdfBuildSettings(C.Structure):
     _fields_ = [("dims", C.c_uint32 * 3), ("brick_size", C.c_uint32), ("apron", C.c_uint32),
                 ("narrow_band", C.c_float), ("smooth_radius", C.c_float), ("bounds_min", C.c_float * 3),
-                ("bounds_max", C.c_float * 3)]
+                ("bounds_max", C.c_float * 3), ("quantize", C.c_uint32)]
 
 
 class Settings(C.Structure):
@@ -115,6 +120,7 @@ def load():
     L.dr_sdf_volume_info.argtypes = [vp, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.dr_sdf_volume_export.argtypes = [vp] + [vp] * 8
     L.dr_sdf_volume_free.argtypes = [vp]
+    L.dr_sdf_build_dense.argtypes = [C.POINTER(SdfPrimitive), u32, C.POINTER(SdfBuildSettings)] + [vp] * 4
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

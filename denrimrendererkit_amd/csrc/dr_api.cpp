@@ -84,6 +84,7 @@ struct dr_scene {
     uint4 *texDesc = nullptr;
     float4 *texels = nullptr;
     DrVolume *volumes = nullptr;
+    DrVolumeSample *denseSamples = nullptr;
     DrBrick *bricks = nullptr;
     uint16_t *dist16 = nullptr;
     float *dist32 = nullptr;
@@ -157,6 +158,7 @@ static void freeScene(dr_scene *s) {
     devFree(s->texDesc);
     devFree(s->texels);
     devFree(s->volumes);
+    devFree(s->denseSamples);
     devFree(s->bricks);
     devFree(s->dist16);
     devFree(s->dist32);
@@ -199,7 +201,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
         return fail(DR_ERR_INVALID_SCENE, "Scene contains no renderable geometry.");
     if (d->material_count == 0 || !d->materials) return fail(DR_ERR_INVALID_SCENE, "Scene contains no materials.");
     if (d->volume_count &&
-        (!d->volumes || (d->volume_brick_count && !d->volume_bricks) ||
+        (!d->volumes || (d->volume_sample_count && !d->volume_samples) || (d->volume_brick_count && !d->volume_bricks) ||
          (d->volume_brick_sample_count && !d->volume_brick_samples) ||
          (d->volume_brick_grid_count && !d->volume_brick_grids) ||
          (d->volume_brick_grid_index_count && !d->volume_brick_grid_indices)))
@@ -321,6 +323,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count));
     chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count));
     chain(upload(&s->volumes, d->volumes, d->volume_count));
+    chain(upload(&s->denseSamples, d->volume_samples, d->volume_count ? d->volume_sample_count : 0u));
     chain(upload(&s->bricks, bricks.data(), bricks.size()));
     chain(upload(&s->dist16, dist16.data(), dist16.size()));
     chain(upload(&s->dist32, dist32.data(), dist32.size()));
@@ -356,6 +359,8 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     g.opaqueOnly = opaqueOnly ? 1u : 0u;
     DrSdfDev &v = g.sdf;
     v.volumes = s->volumes;
+    v.dense = s->denseSamples;
+    v.denseCount = d->volume_count ? d->volume_sample_count : 0u;
     v.bricks = s->bricks;
     v.dist16 = s->dist16;
     v.dist32 = s->dist32;

@@ -50,23 +50,28 @@ __device__ __forceinline__ void waveAddStat(unsigned long long *stats, int slot,
     if (v) atomicAdd(&stats[slot], (unsigned long long)v);
 }
 __device__ __forceinline__ void addSdfStats(unsigned long long *stats, const DrSdfCounters &c,
-                                            int base = DR_CNT_SDF_GRID_CELLS) {
-    waveAddStat(stats, base + 0, c.gridCells);
-    waveAddStat(stats, base + 1, c.macroSkips);
-    waveAddStat(stats, base + 2, c.brickTests);
-    waveAddStat(stats, base + 3, c.invalid);
-    waveAddStat(stats, base + 4, c.rangeCulls);
-    waveAddStat(stats, base + 5, c.marches);
-    waveAddStat(stats, base + 6, c.marchSteps);
-    waveAddStat(stats, base + 7, c.hits);
+                                            int base = DR_CNT_SDF_BASE) {
+    waveAddStat(stats, base + 0, c.denseTests);
+    waveAddStat(stats, base + 1, c.denseSteps);
+    waveAddStat(stats, base + 2, c.gridCells);
+    waveAddStat(stats, base + 3, c.macroSkips);
+    waveAddStat(stats, base + 4, c.brickTests);
+    waveAddStat(stats, base + 5, c.invalid);
+    waveAddStat(stats, base + 6, c.rangeCulls);
+    waveAddStat(stats, base + 7, c.marches);
+    waveAddStat(stats, base + 8, c.marchSteps);
+    waveAddStat(stats, base + 9, c.hits);
 }
 
 // A closest hit as passed between stages. Triangles: (t, u, v, bits(id)).
-// Sparse-volume samples: (t, bits(blend), bits(A | B << 16),
-// bits(DR_VOL_HIT | frontFacing << 30 | volume)) plus the normal in hitN.
-// Miss: id 0xffffffff.
+// Sparse volumes: (t, bits(blend), bits(A | B << 16),
+// bits(DR_VOL_HIT | frontFacing << 30 | volume)); dense volumes:
+// (t, bits(nearest sample), 0, bits(DR_VOL_HIT | DR_VOL_DENSE |
+// frontFacing << 30 | volume)); volume normals in hitN. Miss: id 0xffffffff.
 #define DR_VOL_HIT 0x80000000u
 #define DR_VOL_FRONT 0x40000000u
+#define DR_VOL_DENSE 0x20000000u
+#define DR_VOL_INDEX 0x1fffffffu
 #define DR_MISS 0xffffffffu
 
 // intersectScene (PathTrace.metal:3294-3421): closest triangle (lowest index
@@ -85,8 +90,12 @@ __device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d
     if constexpr (VOL) {
         DrVolHit vh;
         if (dr_trace_volumes(S.sdf, o, d, hr.t, sc, vh)) {
-            h = make_float4(vh.t, __uint_as_float(vh.material.y), __uint_as_float(vh.material.x),
-                            __uint_as_float(DR_VOL_HIT | (vh.frontFacing ? DR_VOL_FRONT : 0u) | vh.volume));
+            uint32_t id = DR_VOL_HIT | (vh.frontFacing ? DR_VOL_FRONT : 0u) | vh.volume;
+            if (vh.dense)
+                h = make_float4(vh.t, __uint_as_float(vh.denseSample), 0.0f, __uint_as_float(id | DR_VOL_DENSE));
+            else
+                h = make_float4(vh.t, __uint_as_float(vh.material.y), __uint_as_float(vh.material.x),
+                                __uint_as_float(id));
             hn = make_float4(vh.normal.x, vh.normal.y, vh.normal.z, 0.0f);
             hit = true;
         }
@@ -172,15 +181,16 @@ struct Surf {
     uint32_t materialID, objectID, primitiveID;
     uint32_t materialID2;
     float materialBlend;
+    uint32_t fieldSample;  // dense hits: sample whose material fields apply (~0: none)
     bool frontFacing;
 };
-// Sparse-volume hit record (closestVolumeHit's Hit fields, :3397-3418): the
-// tangent frame is built from the normal with the helper axis rule.
+// Volume hit record (closestVolumeHit's Hit fields, :3397-3418, and
+// intersectVolume's, :2560-2596): the tangent frame is built from the
+// normal with the helper axis rule.
 __device__ __forceinline__ Surf volumeSurfaceAt(const DrSceneDev &S, vec3 o, vec3 d, float4 h, float4 hn) {
     Surf s;
     uint32_t id = __float_as_uint(h.w);
-    uint32_t ab = __float_as_uint(h.z);
-    const DrVolume &vol = S.sdf.volumes[id & 0x3fffffffu];
+    const DrVolume &vol = S.sdf.volumes[id & DR_VOL_INDEX];
     s.t = h.x;
     s.position = o + d * h.x;
     s.normal = v3(hn.x, hn.y, hn.z);
@@ -188,9 +198,20 @@ __device__ __forceinline__ Surf volumeSurfaceAt(const DrSceneDev &S, vec3 o, vec
     vec3 helper = fabsf(s.normal.y) < 0.999f ? v3(0, 1, 0) : v3(1, 0, 0);
     s.tangent = normalize(cross(helper, s.normal));
     s.bitangent = normalize(cross(s.normal, s.tangent));
-    s.materialID = ab & 0xffffu;
-    s.materialID2 = ab >> 16;
-    s.materialBlend = clampf(__uint_as_float(__float_as_uint(h.y)), 0.0f, 1.0f);
+    if (id & DR_VOL_DENSE) {
+        uint32_t k = __float_as_uint(h.y);
+        const DrVolumeSample &ds = S.sdf.dense[k];
+        s.materialID = ds.materialA;
+        s.materialID2 = ds.materialB;
+        s.materialBlend = clampf(ds.materialBlend, 0.0f, 1.0f);
+        s.fieldSample = k;
+    } else {
+        uint32_t ab = __float_as_uint(h.z);
+        s.materialID = ab & 0xffffu;
+        s.materialID2 = ab >> 16;
+        s.materialBlend = clampf(h.y, 0.0f, 1.0f);
+        s.fieldSample = DR_MISS;
+    }
     s.objectID = vol.metadata.y;
     s.primitiveID = vol.metadata.z;
     s.frontFacing = (id & DR_VOL_FRONT) != 0u;
@@ -218,6 +239,7 @@ __device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, f
     s.materialID = tri.materialID;
     s.materialID2 = tri.materialID;
     s.materialBlend = 0.0f;
+    s.fieldSample = DR_MISS;
     s.objectID = tri.objectID;
     s.primitiveID = id;
     return s;
@@ -225,12 +247,13 @@ __device__ __forceinline__ Surf surfaceAt(const DrSceneDev &S, vec3 o, vec3 d, f
 
 __device__ __forceinline__ Mat materialFor(const DrSceneDev &S, const Surf &h) {  // materialForHit :1096-1137
     const DrMaterial *p = &S.materials[min(h.materialID, S.materialCount - 1u)];
-    Mat m{p, p, 0.0f, p->s[M_BASE]};
+    Mat m{p, p, 0.0f, p->s[M_BASE], 0u, {}, {}, {}};
     if (h.materialBlend > 0.0f) {
         m.q = &S.materials[min(h.materialID2, S.materialCount - 1u)];
         m.t = h.materialBlend;
         m.base = mix4(m.base, m.q->s[M_BASE], m.t);
     }
+    if (h.fieldSample != DR_MISS) applyVolumeMaterialFields(m, S.sdf.dense[h.fieldSample]);
     return m;
 }
 

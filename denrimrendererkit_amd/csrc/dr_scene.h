@@ -75,8 +75,20 @@ struct DrBrickGrid {
     uint4 dimensionsAndIndexOffset, brickSizeAndVolume, macroDimensionsAndIndexOffset, macroSizeAndReserved;
 };
 
+// GPUVolumeSample (PathTrace.metal:92-101): one dense-volume sample.
+struct DrVolumeSample {
+    float distance;
+    uint32_t materialA, materialB;
+    float materialBlend;
+    float4 baseColorOpacity, emissionTransmission, surface;
+    uint4 materialFieldFlags;
+};
+static_assert(sizeof(DrVolumeSample) == 80, "GPUVolumeSample layout");
+
 struct DrSdfDev {
     const DrVolume *volumes;
+    const DrVolumeSample *dense;  // dense-volume samples
+    uint32_t denseCount;
     const DrBrick *bricks;
     const uint16_t *dist16;     // FP16 bits (distHalf != 0)
     const float *dist32;        // FP32 (distHalf == 0)
@@ -137,17 +149,10 @@ enum {
     DR_CNT_SHADOW_NODES = 5,
     DR_CNT_SHADOW_TRIS = 6,
     DR_CNT_SSS = 7,  // subsurface-walk queries (also counted in BOUNCE)
-    // Sparse-volume counters, the reference's slots 2-9 (PathTrace.metal:193-200).
-    DR_CNT_SDF_GRID_CELLS = 8,
-    DR_CNT_SDF_MACRO_SKIPS = 9,
-    DR_CNT_SDF_BRICK_TESTS = 10,
-    DR_CNT_SDF_BRICK_INVALID = 11,
-    DR_CNT_SDF_RANGE_CULLS = 12,
-    DR_CNT_SDF_MARCHES = 13,
-    DR_CNT_SDF_MARCH_STEPS = 14,
-    DR_CNT_SDF_BRICK_HITS = 15,
-    // The same eight counters for NEE shadow queries (k_shadow) only; the
-    // reference's totals are the sums of both groups.
-    DR_CNT_SDF_SHADOW_BASE = 16,
-    DR_CNT_SLOTS = 24
+    // SDF counters, the reference's slots 0-9 (PathTrace.metal:191-200) at
+    // DR_CNT_SDF_BASE + slot; the same ten for NEE shadow queries (k_shadow)
+    // at DR_CNT_SDF_SHADOW_BASE. The reference's totals are the sums.
+    DR_CNT_SDF_BASE = 8,
+    DR_CNT_SDF_SHADOW_BASE = 18,
+    DR_CNT_SLOTS = 28
 };

@@ -20,15 +20,18 @@
 
 // Per-lane sparse-volume counters, added to DR_CNT_SDF_* at the end of a kernel.
 struct DrSdfCounters {
-    uint32_t gridCells, macroSkips, brickTests, invalid, rangeCulls, marches, marchSteps, hits;
+    uint32_t denseTests, denseSteps, gridCells, macroSkips, brickTests, invalid, rangeCulls, marches, marchSteps,
+        hits;
 };
 
 struct DrVolHit {
     float t;
     vec3 normal;
-    uint2 material;  // A | B << 16, bits(blend)
+    uint2 material;        // sparse hits: A | B << 16, bits(blend)
+    uint32_t denseSample;  // dense hits: index of the nearest sample (0xffffffff: none)
     uint32_t volume;
     bool frontFacing;
+    bool dense;
 };
 
 __device__ __forceinline__ float dr_half(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
@@ -293,6 +296,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
             hit.normal = n;
             hit.material = V.material[ctx.offset + nx + ny * ctx.strideY + nz * ctx.strideZ];
             hit.frontFacing = ff;
+            hit.dense = false;
             sc.hits++;
             return true;
         }
@@ -303,12 +307,126 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
     return false;
 }
 
-// Closest sparse-volume hit below closestT (closestVolumeHit, brick-grid path).
+// ------------------------------------------------------- dense volumes --
+// sampleVolumeDistance (PathTrace.metal:1915-1955).
+__device__ __forceinline__ float denseDistance(const DrSdfDev &V, const DrVolume &vol, vec3 lp) {
+    uint32_t dim[3] = {vol.dimensions.x, vol.dimensions.y, vol.dimensions.z};
+    uint32_t off = vol.metadata.x;
+    uint32_t n = dim[0] * dim[1] * dim[2];
+    if (off >= V.denseCount || off + n > V.denseCount) return __builtin_inff();
+    vec3 lmin = v3f(vol.localBoundsMin);
+    vec3 ext = vmax(v3f(vol.localBoundsMax) - lmin, v3s(1e-6f));
+    vec3 uvw = vclamp((lp - lmin) / ext, v3s(0.0f), v3s(1.0f));
+    float g[3] = {uvw.x * (float)(dim[0] - 1u), uvw.y * (float)(dim[1] - 1u), uvw.z * (float)(dim[2] - 1u)};
+    uint32_t b[3], nx[3];
+    float f[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b[a] = (uint32_t)floorf(g[a]);
+        nx[a] = min(b[a] + 1u, dim[a] - 1u);
+        f[a] = g[a] - (float)b[a];
+    }
+    uint32_t sy = dim[0], sz = dim[0] * dim[1];
+    const DrVolumeSample *s = V.dense + off;
+    float c00 = mixf(s[b[0] + b[1] * sy + b[2] * sz].distance, s[nx[0] + b[1] * sy + b[2] * sz].distance, f[0]);
+    float c10 = mixf(s[b[0] + nx[1] * sy + b[2] * sz].distance, s[nx[0] + nx[1] * sy + b[2] * sz].distance, f[0]);
+    float c01 = mixf(s[b[0] + b[1] * sy + nx[2] * sz].distance, s[nx[0] + b[1] * sy + nx[2] * sz].distance, f[0]);
+    float c11 = mixf(s[b[0] + nx[1] * sy + nx[2] * sz].distance, s[nx[0] + nx[1] * sy + nx[2] * sz].distance, f[0]);
+    float c0 = mixf(c00, c10, f[1]);
+    float c1 = mixf(c01, c11, f[1]);
+    return mixf(c0, c1, f[2]);
+}
+
+// volumeNormal (:2029-2055): central differences one sample apart.
+__device__ __forceinline__ vec3 denseNormal(const DrSdfDev &V, const DrVolume &vol, vec3 lp) {
+    vec3 ext = vmax(v3f(vol.localBoundsMax) - v3f(vol.localBoundsMin), v3s(1e-6f));
+    vec3 sp = ext / vmax(v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z) - v3s(1.0f),
+                         v3s(1.0f));
+    float dx = denseDistance(V, vol, lp + v3(sp.x, 0, 0)) - denseDistance(V, vol, lp - v3(sp.x, 0, 0));
+    float dy = denseDistance(V, vol, lp + v3(0, sp.y, 0)) - denseDistance(V, vol, lp - v3(0, sp.y, 0));
+    float dz = denseDistance(V, vol, lp + v3(0, 0, sp.z)) - denseDistance(V, vol, lp - v3(0, 0, sp.z));
+    vec3 ln = v3(dx, dy, dz);
+    if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
+    return normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
+}
+
+// intersectVolume (:2471-2606): world-t march, <= 512 steps, 10 bisections.
+__device__ __forceinline__ bool intersectDense(const DrSdfDev &V, const DrVolume &vol, vec3 ro, vec3 rd,
+                                               float closestT, DrSdfCounters &sc, DrVolHit &hit) {
+    float tNear, tFar;
+    if (!sdfAABB(ro, rd, v3f(vol.worldBoundsMin), v3f(vol.worldBoundsMax), closestT, tNear, tFar)) return false;
+    vec3 localStep = mulCols(vol.w2l, rd, 0.0f);
+    float lupw = fmaxf(length(localStep), 1e-6f);
+    vec3 ext = vmax(v3f(vol.localBoundsMax) - v3f(vol.localBoundsMin), v3s(1e-6f));
+    float cellSize = fminf(ext.x / fmaxf((float)vol.dimensions.x - 1.0f, 1.0f),
+                           fminf(ext.y / fmaxf((float)vol.dimensions.y - 1.0f, 1.0f),
+                                 ext.z / fmaxf((float)vol.dimensions.z - 1.0f, 1.0f)));
+    float minimumStep = fmaxf(cellSize * 0.10f / lupw, 0.0005f);
+    float prevT = tNear;
+    float prevD = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * prevT, 1.0f));
+    if (!isfinite(prevD)) return false;
+    float t = prevT + fmaxf(fabsf(prevD) / lupw * 0.9f, minimumStep);
+    for (uint32_t step = 0; step < 512u && t <= tFar && t < closestT; ++step) {
+        sc.denseSteps++;
+        float d = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * t, 1.0f));
+        if (!isfinite(d)) return false;
+        bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
+        if (crossed) {
+            float lowT = prevT, highT = t, lowD = prevD, refinedT = t;
+            for (uint32_t k = 0; k < 10u; ++k) {
+                float midT = 0.5f * (lowT + highT);
+                float md = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * midT, 1.0f));
+                if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
+                    lowT = midT;
+                    lowD = md;
+                } else {
+                    highT = midT;
+                }
+                refinedT = 0.5f * (lowT + highT);
+            }
+            vec3 rl = mulCols(vol.w2l, ro + rd * refinedT, 1.0f);
+            vec3 n = denseNormal(V, vol, rl);
+            bool ff = dot(n, rd) <= 0.0f;
+            if (!ff) n = -n;
+            // sampleVolumeMaterial (:1957-1990): the nearest sample.
+            uint32_t dim[3] = {vol.dimensions.x, vol.dimensions.y, vol.dimensions.z};
+            vec3 uvw = vclamp((rl - v3f(vol.localBoundsMin)) / ext, v3s(0.0f), v3s(1.0f));
+            uint32_t qx = min((uint32_t)roundf(uvw.x * (float)(dim[0] - 1u)), dim[0] - 1u);
+            uint32_t qy = min((uint32_t)roundf(uvw.y * (float)(dim[1] - 1u)), dim[1] - 1u);
+            uint32_t qz = min((uint32_t)roundf(uvw.z * (float)(dim[2] - 1u)), dim[2] - 1u);
+            hit.t = refinedT;
+            hit.normal = n;
+            hit.denseSample = vol.metadata.x + qx + qy * dim[0] + qz * dim[0] * dim[1];
+            hit.frontFacing = ff;
+            hit.dense = true;
+            return true;
+        }
+        prevT = t;
+        prevD = d;
+        t += fmaxf(fabsf(d) / lupw * 0.8f, minimumStep);
+    }
+    return false;
+}
+
+// closestVolumeHit (PathTrace.metal:3090-3163): dense volumes, then the
+// brick-grid branch for sparse volumes, below closestT.
 __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
                                               DrVolHit &out) {
     bool any = false;
     float best = closestT;
-    if (V.gridCount == 0u || V.gridIndexCount == 0u) return false;
+    for (uint32_t vi = 0; vi < V.volumeCount; ++vi) {
+        const DrVolume &vol = V.volumes[vi];
+        if (vol.metadata.w != 0u) continue;
+        sc.denseTests++;
+        DrVolHit cand;
+        if (intersectDense(V, vol, ro, rd, best, sc, cand) && cand.t < best) {
+            out = cand;
+            out.volume = vi;
+            best = cand.t;
+            any = true;
+        }
+    }
+    if (V.gridCount == 0u || V.gridIndexCount == 0u) return any;
     for (uint32_t vi = 0; vi < V.volumeCount; ++vi) {
         const DrVolume &vol = V.volumes[vi];
         if (vol.metadata.w != 1u) continue;

@@ -15,19 +15,57 @@
 // A hit's material: record p, or p blended toward q by t when t > 0
 // (blendMaterials, PathTrace.metal:549-569, as materialForHit applies it for
 // sparse-volume samples). base holds the (textured) base colour.
+// ovr marks slots replaced by dense-volume material fields (emis: 1,
+// par: 2, par2: 4; applyVolumeMaterialFields, PathTrace.metal:571-601).
 struct Mat {
     const DrMaterial *p;
     const DrMaterial *q;
     float t;
     float4 base;
+    uint32_t ovr;
+    float4 emis, par, par2;
 };
 template <int S> __device__ __forceinline__ float4 ms(const Mat &m) {
     if constexpr (S == M_BASE) return m.base;
     else {
+        if constexpr (S == M_EMISSION) {
+            if (m.ovr & 1u) return m.emis;
+        } else if constexpr (S == M_PARAMS) {
+            if (m.ovr & 2u) return m.par;
+        } else if constexpr (S == M_PARAMS2) {
+            if (m.ovr & 4u) return m.par2;
+        }
         float4 v = m.p->s[S];
         if (m.t > 0.0f) v = mix4(v, m.q->s[S], m.t);
         return v;
     }
+}
+
+// applyVolumeMaterialFields (PathTrace.metal:571-601) with the flag bits
+// volumeMaterialField* (:255-262).
+__device__ __forceinline__ void applyVolumeMaterialFields(Mat &m, const DrVolumeSample &s) {
+    const uint32_t f = s.materialFieldFlags.x;
+    if (f == 0u) return;
+    float4 e = ms<M_EMISSION>(m), pa = ms<M_PARAMS>(m), p2 = ms<M_PARAMS2>(m);
+    const float4 bc = s.baseColorOpacity, et = s.emissionTransmission, su = s.surface;
+    if (f & 1u) m.base.x = fmaxf(bc.x, 0.0f), m.base.y = fmaxf(bc.y, 0.0f), m.base.z = fmaxf(bc.z, 0.0f);
+    if (f & 2u) m.base.w = fminf(fmaxf(bc.w, 0.0f), 1.0f);
+    if (f & 4u) e.x = fmaxf(et.x, 0.0f), e.y = fmaxf(et.y, 0.0f), e.z = fmaxf(et.z, 0.0f);
+    if ((f & 4u) && (f & 128u)) {
+        float k = fmaxf(su.w, 0.0f);
+        e.x = fmaxf(et.x, 0.0f) * k, e.y = fmaxf(et.y, 0.0f) * k, e.z = fmaxf(et.z, 0.0f) * k;
+    } else if (f & 128u) {
+        float k = fmaxf(su.w, 0.0f);
+        e.x = fmaxf(e.x, 0.0f) * k, e.y = fmaxf(e.y, 0.0f) * k, e.z = fmaxf(e.z, 0.0f) * k;
+    }
+    if (f & 32u) e.w = fminf(fmaxf(et.w, 0.0f), 1.0f);
+    if (f & 8u) pa.x = fminf(fmaxf(su.x, 0.02f), 1.0f);
+    if (f & 16u) pa.y = fminf(fmaxf(su.y, 0.0f), 1.0f);
+    if (f & 64u) p2.x = fminf(fmaxf(su.z, 0.0f), 1.0f);
+    m.emis = e;
+    m.par = pa;
+    m.par2 = p2;
+    m.ovr = 7u;
 }
 __device__ __forceinline__ vec3 xyz(float4 f) { return vec3{f.x, f.y, f.z}; }
 

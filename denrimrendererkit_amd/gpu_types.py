@@ -73,6 +73,10 @@ BRICK = np.dtype([
     ("gridOriginAndVolume", *U4), ("dimensionsAndSampleOffset", *U4),
 ])
 BRICK_SAMPLE = np.dtype([("distance", "<f4"), ("materialA", "<u4"), ("materialB", "<u4"), ("materialBlend", "<f4")])
+VOLUME_SAMPLE = np.dtype([  # GPUVolumeSample (PathTrace.metal:92-101), dense volumes
+    ("distance", "<f4"), ("materialA", "<u4"), ("materialB", "<u4"), ("materialBlend", "<f4"),
+    ("baseColorOpacity", *F4), ("emissionTransmission", *F4), ("surface", *F4), ("materialFieldFlags", *U4),
+])
 BRICK_GRID = np.dtype([
     ("dimensionsAndIndexOffset", *U4), ("brickSizeAndVolume", *U4),
     ("macroDimensionsAndIndexOffset", *U4), ("macroSizeAndReserved", *U4),
@@ -81,4 +85,5 @@ BRICK_GRID = np.dtype([
 assert VOLUME.itemsize == 240
 assert BRICK.itemsize == 128
 assert BRICK_SAMPLE.itemsize == 16
+assert VOLUME_SAMPLE.itemsize == 80
 assert BRICK_GRID.itemsize == 64

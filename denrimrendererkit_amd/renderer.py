@@ -32,7 +32,7 @@ from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, 
 from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
 
 STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize")
-N_STATS = 24
+N_STATS = 28
 
 
 @dataclass(frozen=True)
@@ -116,12 +116,13 @@ class GpuScene:
                                                    build.environment_samples, build.texture_descriptors,
                                                    build.texture_pixels, build.volumes, build.volume_bricks,
                                                    build.volume_brick_samples, build.volume_brick_grids,
-                                                   build.volume_brick_grid_indices)]
-        t, m, li, e, td, tp, v, vb, vs, vg, vi = keep
+                                                   build.volume_brick_grid_indices, build.volume_samples)]
+        t, m, li, e, td, tp, v, vb, vs, vg, vi, ds = keep
         d = _lib.SceneDesc(_ptr(t), len(t), _ptr(m), len(m), _ptr(li), len(li), _ptr(e), len(e), _ptr(td), len(td),
                            _ptr(tp), len(tp), build.environment_texture_index_plus_one, build.environment_intensity,
                            build.environment_rotation_y, build.environment_max_radiance, max_leaf,
-                           _ptr(v), len(v), _ptr(vb), len(vb), _ptr(vs), len(vs), _ptr(vg), len(vg), _ptr(vi), len(vi))
+                           _ptr(v), len(v), _ptr(ds), len(ds), _ptr(vb), len(vb), _ptr(vs), len(vs), _ptr(vg), len(vg),
+                           _ptr(vi), len(vi))
         h = C.c_void_p()
         _lib.check(_lib.load().dr_hip_scene_upload(renderer._h, C.byref(d), C.byref(h)), "scene upload")
         self._h = h
@@ -230,13 +231,12 @@ class RenderSession:
         return np.array(list(out), dtype=np.uint64)
 
     def sdf_traversal_stats(self) -> SDFTraversalStats:
-        """sdfTraversalStats() (RenderSession.swift:807-831). The dense-volume
-        counters are zero: dense volumes are not traced by this backend."""
+        """sdfTraversalStats() (RenderSession.swift:807-831)."""
         s = [int(x) for x in self.stats()]
-        c = [s[8 + k] + s[16 + k] for k in range(8)]  # extension/subsurface + shadow queries
-        cells, tests = c[0], c[2]
-        return SDFTraversalStats(0, 0, cells, cells - min(cells, tests), c[1], tests, c[3], c[4], c[5], c[6],
-                                 c[7], s[0], s[1], s[2])
+        c = [s[8 + k] + s[18 + k] for k in range(10)]  # extension/subsurface/AOV + shadow queries
+        cells, tests = c[2], c[4]
+        return SDFTraversalStats(c[0], c[1], cells, cells - min(cells, tests), c[3], tests, c[5], c[6], c[7], c[8],
+                                 c[9], s[0], s[1], s[2])
 
     def reset_sdf_traversal_stats(self) -> None:
         self.reset_stats()

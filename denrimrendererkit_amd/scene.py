@@ -457,6 +457,7 @@ class RenderScene:
         self.materials: List[Material] = []
         self.mesh_instances: List[MeshInstance] = []
         self.sparse_volume_instances: list = []
+        self.volume_instances: list = []
 
     def add_material(self, material: Material) -> int:
         self.materials.append(material)
@@ -464,6 +465,11 @@ class RenderScene:
 
     def add(self, mesh: Mesh, material: int, transform: Optional[Transform] = None) -> None:
         self.mesh_instances.append(MeshInstance(mesh, material, transform or Transform.identity()))
+
+    def add_volume(self, volume, material: int, transform: Optional[Transform] = None) -> None:
+        """add(volume:material:transform:) (RenderScene.swift:140-152): a dense grid."""
+        from .sdf import VolumeInstance
+        self.volume_instances.append(VolumeInstance(volume, material, transform or Transform.identity()))
 
     def add_sparse_volume(self, volume, material: int, transform: Optional[Transform] = None) -> None:
         """add(sparseVolume:material:transform:) (RenderScene.swift:154-166)."""
@@ -489,6 +495,67 @@ class RenderScene:
         s.add(Mesh.quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1)), red)
         s.add(Mesh.quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)), green)
         s.add(Mesh.quad((-0.28, 1.99, -0.25), (0.28, 1.99, -0.25), (0.28, 1.99, 0.25), (-0.28, 1.99, 0.25)), light)
+        return s
+
+    @staticmethod
+    def _distance_volume_reference_contents():
+        """distanceVolumeReferenceContents (RenderScene.swift:644-768)."""
+        from .sdf import SDFPrimitive
+        s = RenderScene(camera=Camera(origin=(0, 1.35, 4.8), target=(0, 0.68, 0), vertical_field_of_view_degrees=38))
+        floor = s.add_material(Material(base_color=(0.58, 0.60, 0.56), roughness=0.74))
+        back = s.add_material(Material(base_color=(0.32, 0.37, 0.43), roughness=0.82))
+        red = s.add_material(Material(base_color=(0.92, 0.22, 0.14), roughness=0.62, clearcoat=0.18,
+                                      clearcoat_roughness=0.08))
+        transparent = s.add_material(Material(base_color=(0.18, 0.56, 0.95), roughness=0.08, opacity=0.42,
+                                              transmission=0.82, transmission_color=(0.62, 0.84, 1.0),
+                                              transmission_roughness=0.03, transmission_index_of_refraction=1.38,
+                                              transmission_absorption_color=(0.68, 0.84, 1.0),
+                                              transmission_absorption_distance=0.9))
+        metal = s.add_material(Material(base_color=(0.74, 0.86, 0.95), roughness=0.18, metallic=1,
+                                        specular_anisotropy=0.45, clearcoat=0.12, clearcoat_roughness=0.04))
+        yellow = s.add_material(Material(base_color=(0.95, 0.72, 0.18), roughness=0.38, sheen=0.25,
+                                         sheen_color=(1.0, 0.86, 0.42)))
+        rear = s.add_material(Material(base_color=(0.18, 0.84, 0.42), emission=(0.18, 0.84, 0.42),
+                                       emission_strength=0.7, roughness=0.65))
+        light = s.add_material(Material(base_color=(1, 1, 1), emission=(1, 0.9, 0.72), emission_strength=10))
+        s.add(Mesh.quad((-3.0, 0, 2.0), (3.0, 0, 2.0), (3.0, 0, -2.0), (-3.0, 0, -2.0)), floor)
+        s.add(Mesh.quad((-3.0, 0, -2.0), (3.0, 0, -2.0), (3.0, 2.5, -2.0), (-3.0, 2.5, -2.0)), back)
+        s.add(Mesh.quad((-0.8, 2.36, -0.75), (0.8, 2.36, -0.75), (0.8, 2.36, 0.25), (-0.8, 2.36, 0.25)), light)
+        s.add(Mesh.quad((-0.66, 0.16, -0.38), (0.36, 0.16, -0.38), (0.36, 1.16, -0.38), (-0.66, 1.16, -0.38)), rear)
+        T = Transform
+        prims = [
+            SDFPrimitive("sphere", red, radius=0.5,
+                         transform=T.translation((-1.25, 0.55, 0.1)) * T.scale((1.0, 1.1, 0.9))),
+            SDFPrimitive("sphere", transparent, radius=0.5,
+                         transform=T.translation((-0.2, 0.48, 0.0)) * T.scale((0.88, 0.96, 0.88)),
+                         smooth_union_radius=0.08),
+            SDFPrimitive("sphere", metal, radius=0.5,
+                         transform=T.translation((0.86, 0.42, 0.06)) * T.scale((0.84, 0.84, 0.84))),
+            SDFPrimitive("sphere", yellow, radius=0.5,
+                         transform=T.translation((1.55, 0.26, 0.38)) * T.scale((0.52, 0.52, 0.52))),
+            SDFPrimitive("sphere", yellow, radius=0.5,
+                         transform=T.translation((-1.85, 0.23, 0.62)) * T.scale((0.46, 0.46, 0.46))),
+        ]
+        settings = dict(dims=(48, 36, 32), bounds_min=(-2.35, -0.02, -0.72), bounds_max=(2.1, 1.2, 0.92))
+        return s, prims, settings, red
+
+    @staticmethod
+    def distance_volume_reference() -> "RenderScene":
+        """RenderScene.distanceVolumeReference (RenderScene.swift:770-779): the
+        SDF model built as a dense volume."""
+        from .sdf import build_volume
+        s, prims, st, red = RenderScene._distance_volume_reference_contents()
+        s.add_volume(build_volume(prims, **st), red)
+        return s
+
+    @staticmethod
+    def sparse_distance_volume_reference() -> "RenderScene":
+        """RenderScene.sparseDistanceVolumeReference (RenderScene.swift:782-797):
+        the same model as 8^3 bricks with narrow band 0.28 (reference format)."""
+        from .sdf import build_sparse_reference
+        s, prims, st, red = RenderScene._distance_volume_reference_contents()
+        s.add_sparse_volume(build_sparse_reference(prims, st["dims"], st["bounds_min"], st["bounds_max"],
+                                                   brick_size=8, narrow_band=0.28), red)
         return s
 
     @staticmethod
@@ -537,15 +604,19 @@ class AccelerationBuild:
     volume_brick_samples: Optional[np.ndarray] = None
     volume_brick_grids: Optional[np.ndarray] = None
     volume_brick_grid_indices: Optional[np.ndarray] = None
+    volume_samples: Optional[np.ndarray] = None  # dense GPUVolumeSample records
 
     def __post_init__(self):
-        from .gpu_types import BRICK, BRICK_GRID, BRICK_SAMPLE, VOLUME
+        from .gpu_types import BRICK, BRICK_GRID, BRICK_SAMPLE, VOLUME, VOLUME_SAMPLE
         if self.volumes is None:
             self.volumes = np.zeros(0, VOLUME)
+        if self.volume_bricks is None:
             self.volume_bricks = np.zeros(0, BRICK)
             self.volume_brick_samples = np.zeros(0, BRICK_SAMPLE)
             self.volume_brick_grids = np.zeros(0, BRICK_GRID)
             self.volume_brick_grid_indices = np.zeros(0, np.uint32)
+        if self.volume_samples is None:
+            self.volume_samples = np.zeros(0, VOLUME_SAMPLE)
 
 
 def _materials_and_textures(scene: RenderScene):
@@ -682,17 +753,24 @@ def environment_samples(env: Environment) -> np.ndarray:
 
 
 def compile_scene(scene: RenderScene) -> AccelerationBuild:
-    """LinearTriangleAccelerationBackend.build (AccelerationBackend.swift:7-67), triangle path."""
+    """LinearTriangleAccelerationBackend.build (AccelerationBackend.swift:7-67):
+    triangles, then dense volumes, then sparse volumes (gpuVolumes,
+    VolumeGPUEncoding.swift:5-65, with its objectID / primitiveID bases)."""
+    tris = materialize_triangles(scene)
+    from .gpu_types import VOLUME
+    from .sdf import encode_dense_volumes, encode_sparse_volumes
+    n_mesh, n_dense = len(scene.mesh_instances), len(scene.volume_instances)
+    dense, dense_samples = encode_dense_volumes(scene.volume_instances, n_mesh, len(scene.materials))
+    sparse = (None,) * 5
+    if scene.sparse_volume_instances:
+        sparse = encode_sparse_volumes(scene.sparse_volume_instances, n_dense, n_mesh + n_dense, n_dense,
+                                       len(scene.materials))
+    vol_list = [v for v in (dense, sparse[0]) if v is not None and len(v)]
+    volumes = np.concatenate(vol_list) if vol_list else np.zeros(0, VOLUME)
+    if len(tris) == 0 and len(volumes) == 0:  # RenderSession.swift:322-324
+        raise DenrimRendererError.invalid_scene("Scene contains no renderable geometry.")
     if not scene.materials:  # RenderSession.swift:325-327
         raise DenrimRendererError.invalid_scene("Scene contains no materials.")
-    tris = materialize_triangles(scene)
-    vols = (None,) * 5
-    if scene.sparse_volume_instances:
-        from .sdf import encode_sparse_volumes
-        vols = encode_sparse_volumes(scene.sparse_volume_instances, 0, len(scene.mesh_instances), 0,
-                                     len(scene.materials))
-    if len(tris) == 0 and vols[0] is None:  # RenderSession.swift:322-324
-        raise DenrimRendererError.invalid_scene("Scene contains no renderable geometry.")
     mats, descs, pixels, env_index = _materials_and_textures(scene)
     lights = light_records(tris, mats)
     env = environment_samples(scene.environment)
@@ -702,6 +780,6 @@ def compile_scene(scene: RenderScene) -> AccelerationBuild:
         environment_intensity=float(scene.environment.intensity),
         environment_rotation_y=float(scene.environment.rotation_y),
         environment_max_radiance=float(scene.environment.max_radiance),
-        volumes=vols[0], volume_bricks=vols[1], volume_brick_samples=vols[2], volume_brick_grids=vols[3],
-        volume_brick_grid_indices=vols[4],
+        volumes=volumes, volume_bricks=sparse[1], volume_brick_samples=sparse[2], volume_brick_grids=sparse[3],
+        volume_brick_grid_indices=sparse[4], volume_samples=dense_samples,
     )

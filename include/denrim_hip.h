@@ -86,15 +86,16 @@ typedef struct {
     float environment_rotation_y;
     float environment_max_radiance;
     uint32_t max_leaf_triangles;      /* 0 = default (4) */
-    /* Sparse signed-distance volumes: the volume and brick arrays of
-     * AccelerationBuild (AccelerationBuild.swift:5-20) in the reference
-     * layouts (Metal/GPUTypes.swift: GPUVolumeDescriptor 240 B,
-     * GPUVolumeBrickDescriptor 128 B, GPUVolumeBrickSample 16 B,
-     * GPUVolumeBrickGrid 64 B, grid + macro-grid indices u32). Distances are
-     * re-encoded as FP16 on upload and must be FP16-exact; material ids
-     * < 256; blends multiples of 1/65535 (dr_sdf_build_sparse produces such
-     * data). Dense volumes (metadata.w == 0) are not supported. */
+    /* Signed-distance volumes: the volume arrays of AccelerationBuild
+     * (AccelerationBuild.swift:5-20) in the reference layouts
+     * (Metal/GPUTypes.swift: GPUVolumeDescriptor 240 B, dense
+     * GPUVolumeSample 80 B, GPUVolumeBrickDescriptor 128 B,
+     * GPUVolumeBrickSample 16 B, GPUVolumeBrickGrid 64 B, grid + macro-grid
+     * indices u32). Brick distances are stored as FP16 on the device when
+     * every one of them is FP16-exact, else as FP32; scenes with volumes may
+     * hold at most 65536 materials. */
     const void *volumes;              uint32_t volume_count;
+    const void *volume_samples;       uint32_t volume_sample_count;
     const void *volume_bricks;        uint32_t volume_brick_count;
     const void *volume_brick_samples; uint32_t volume_brick_sample_count;
     const void *volume_brick_grids;   uint32_t volume_brick_grid_count;
@@ -129,42 +130,59 @@ typedef enum {
     DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
     DR_STAT_SHADOW_TRIANGLE_TESTS = 6,
     DR_STAT_SUBSURFACE_QUERIES = 7,   /* random-walk queries, also in BOUNCE_QUERIES */
-    /* Sparse-volume counters, sdfCounterSparse* (PathTrace.metal:193-200). */
-    DR_STAT_SPARSE_GRID_CELLS_VISITED = 8,
-    DR_STAT_SPARSE_GRID_MACRO_SKIPS = 9,
-    DR_STAT_SPARSE_BRICK_TESTS = 10,
-    DR_STAT_SPARSE_BRICK_INVALID = 11,
-    DR_STAT_SPARSE_BRICK_RANGE_CULLS = 12,
-    DR_STAT_SPARSE_BRICK_MARCHES = 13,
-    DR_STAT_SPARSE_BRICK_MARCH_STEPS = 14,
-    DR_STAT_SPARSE_BRICK_HITS = 15,
-    /* 16-23: the eight sparse-volume counters again, for NEE shadow queries
-     * only (the totals above exclude them; the reference's counters are the
-     * sums of both groups). */
-    DR_STAT_SHADOW_SPARSE_BASE = 16,
-    DR_STAT_COUNT = 24
+    /* 8-17: the SDF counters sdfCounterDense* / sdfCounterSparse*
+     * (PathTrace.metal:191-200, slot k at 8 + k) for extension, subsurface
+     * and AOV queries. */
+    DR_STAT_DENSE_VOLUME_TESTS = 8,
+    DR_STAT_DENSE_MARCH_STEPS = 9,
+    DR_STAT_SPARSE_GRID_CELLS_VISITED = 10,
+    DR_STAT_SPARSE_GRID_MACRO_SKIPS = 11,
+    DR_STAT_SPARSE_BRICK_TESTS = 12,
+    DR_STAT_SPARSE_BRICK_INVALID = 13,
+    DR_STAT_SPARSE_BRICK_RANGE_CULLS = 14,
+    DR_STAT_SPARSE_BRICK_MARCHES = 15,
+    DR_STAT_SPARSE_BRICK_MARCH_STEPS = 16,
+    DR_STAT_SPARSE_BRICK_HITS = 17,
+    /* 18-27: the same ten counters for NEE shadow queries only; the
+     * reference's counters are the sums of both groups. */
+    DR_STAT_SHADOW_SDF_BASE = 18,
+    DR_STAT_COUNT = 28
 } dr_stat;
 
-/* Sparse-field primitive for dr_sdf_build_sparse (SDFPrimitive,
- * Scene/SDFModel.swift:28-70, in world space; the torus is an addition). */
-enum { DR_SDF_SPHERE = 0, DR_SDF_TORUS = 1 };
+/* SDF primitive for the volume builders (SDFPrimitive / CompiledPrimitive,
+ * Scene/SDFModel.swift:4-70, 664-713; the torus is an addition). The shape
+ * is evaluated at world_to_primitive * p - center and scaled by
+ * distance_scale when has_transform is set. */
+enum { DR_SDF_SPHERE = 0, DR_SDF_TORUS = 1, DR_SDF_BOX = 2, DR_SDF_CYLINDER = 3 };
+enum { DR_SDF_UNION = 0, DR_SDF_SUBTRACT = 1 };
 typedef struct {
-    uint32_t kind;          /* DR_SDF_SPHERE or DR_SDF_TORUS */
+    uint32_t kind;                 /* DR_SDF_* */
     uint32_t material;
-    float center[3];
-    float radius;           /* sphere radius / torus major radius */
-    float axis[3];          /* torus axis */
-    float minor_radius;     /* torus tube radius */
+    float center[3];               /* shape origin in primitive space */
+    float radius;                  /* sphere radius / torus major radius / cylinder radius */
+    float axis[3];                 /* torus axis */
+    float minor_radius;            /* torus tube radius */
+    float half_extents[3];         /* box */
+    float corner_radius;           /* box */
+    float half_height;             /* cylinder (along local y) */
+    float smooth_radius;           /* smoothUnionRadius; < 0: the build settings' radius */
+    uint32_t operation;            /* DR_SDF_UNION or DR_SDF_SUBTRACT */
+    uint32_t has_transform;
+    float world_to_primitive[16];  /* column-major inverse of the primitive transform */
+    float distance_scale;          /* smallest axis scale of the transform (SDFModel.swift:708-713) */
+    float lipschitz;               /* bound on |grad d| of the scaled distance (1 without shear) */
 } dr_sdf_primitive;
 
-/* SparseDistanceVolumeBuildSettings (Scene/SDFModel.swift:256-275). */
+/* DistanceVolumeBuildSettings / SparseDistanceVolumeBuildSettings
+ * (Scene/SDFModel.swift:256-275). */
 typedef struct {
-    uint32_t dims[3];       /* virtual sample grid */
-    uint32_t brick_size;    /* core samples per brick side */
-    uint32_t apron;         /* stored samples beyond the core per side */
-    float narrow_band;      /* keep bricks whose samples reach |d| <= band */
-    float smooth_radius;    /* smooth-union radius of every primitive */
+    uint32_t dims[3];       /* sample grid */
+    uint32_t brick_size;    /* sparse: core samples per brick side */
+    uint32_t apron;         /* sparse: stored samples beyond the core per side (the reference uses 2) */
+    float narrow_band;      /* sparse: keep bricks whose samples reach |d| <= band */
+    float smooth_radius;    /* default smooth-union radius */
     float bounds_min[3], bounds_max[3];
+    uint32_t quantize;      /* 1: FP16 distances, blends on a 1/65535 grid, exact skipping */
 } dr_sdf_build_settings;
 
 typedef struct dr_sdf_volume dr_sdf_volume;
@@ -241,6 +259,10 @@ dr_status dr_hip_session_timing(dr_session *session, double *total_ms, double *s
 dr_status dr_sdf_build_sparse(const dr_sdf_primitive *prims, uint32_t count, const dr_sdf_build_settings *settings,
                               dr_sdf_volume **out);
 dr_status dr_sdf_volume_info(const dr_sdf_volume *volume, uint32_t *bricks, uint64_t *samples, uint64_t *candidates);
+/* DistanceVolumeBuilder.build (Scene/SDFModel.swift:220-254): every sample
+ * of the dense grid (x fastest) into caller arrays of dims.x*dims.y*dims.z. */
+dr_status dr_sdf_build_dense(const dr_sdf_primitive *prims, uint32_t count, const dr_sdf_build_settings *settings,
+                             float *distance, uint32_t *material_a, uint32_t *material_b, float *blend);
 /* Per brick (z, y, x order): stored origin / dims and core origin / dims
  * (3 x int32 each); per stored sample (x fastest): distance, material A / B,
  * blend. Any pointer may be NULL. */

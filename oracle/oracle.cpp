@@ -151,6 +151,9 @@ template <class R> struct Hit {  // PathTrace.metal:229-253 (triangle and sparse
     R materialBlend;
     uint32_t objectID, primitiveID;
     bool frontFacing;
+    // Dense-volume material fields (volumeBaseColorOpacity ... materialFieldFlags).
+    V4<R> fieldBaseColorOpacity, fieldEmissionTransmission, fieldSurface;
+    uint32_t fieldFlags;
 };
 template <class R> Hit<R> emptyHit() {  // :1883-1909
     Hit<R> h;
@@ -167,6 +170,8 @@ template <class R> Hit<R> emptyHit() {  // :1883-1909
     h.objectID = 0;
     h.primitiveID = 0;
     h.frontFacing = true;
+    h.fieldBaseColorOpacity = h.fieldEmissionTransmission = h.fieldSurface = V4<R>{0, 0, 0, 0};
+    h.fieldFlags = 0u;
     return h;
 }
 
@@ -684,6 +689,7 @@ struct or_scene {
     std::vector<or_volume> volumes;
     std::vector<or_brick> bricks;
     std::vector<or_brick_sample> brickSamples;
+    std::vector<or_volume_sample> volumeSamples;
     std::vector<or_brick_grid> brickGrids;
     std::vector<uint32_t> brickGridIndices;
 };
@@ -768,6 +774,8 @@ template <class R> void initCtx(Ctx<R> &c) {
     c.sdf.volumeCount = uint32_t(s.volumes.size());
     c.sdf.bricks = s.bricks.data();
     c.sdf.brickCount = uint32_t(s.bricks.size());
+    c.sdf.dense = s.volumeSamples.data();
+    c.sdf.denseCount = uint32_t(s.volumeSamples.size());
     c.sdf.samples = s.brickSamples.data();
     c.sdf.sampleCount = uint32_t(s.brickSamples.size());
     c.sdf.grids = s.brickGrids.data();
@@ -837,6 +845,12 @@ template <class R> Hit<R> intersectScene(Ctx<R> &c, const Ray<R> &ray) {
             h.objectID = vh.objectID;
             h.primitiveID = vh.primitiveID;
             h.frontFacing = vh.frontFacing;
+            if (vh.fields) {
+                h.fieldBaseColorOpacity = v4<R>(vh.fields->baseColorOpacity);
+                h.fieldEmissionTransmission = v4<R>(vh.fields->emissionTransmission);
+                h.fieldSurface = v4<R>(vh.fields->surface);
+                h.fieldFlags = vh.fields->materialFieldFlags[0];
+            }
         }
     }
     return h;
@@ -850,6 +864,29 @@ template <class R> Mat<R> blendMaterials(Mat<R> a, const Mat<R> &b, R blend) {  
     return a;
 }
 
+// applyVolumeMaterialFields (:571-601); flag bits volumeMaterialField*
+// (:255-262).
+template <class R> void applyVolumeMaterialFields(Mat<R> &m, const Hit<R> &h) {
+    const uint32_t f = h.fieldFlags;
+    if (f == 0u) return;
+    const V4<R> &bc = h.fieldBaseColorOpacity, &et = h.fieldEmissionTransmission, &su = h.fieldSurface;
+    if (f & 1u) m.baseColor.x = fmx(bc.x, R(0)), m.baseColor.y = fmx(bc.y, R(0)), m.baseColor.z = fmx(bc.z, R(0));
+    if (f & 2u) m.baseColor.w = clampv(bc.w, R(0), R(1));
+    if (f & 4u) m.emission.x = fmx(et.x, R(0)), m.emission.y = fmx(et.y, R(0)), m.emission.z = fmx(et.z, R(0));
+    if ((f & 4u) && (f & 128u)) {
+        R s = fmx(su.w, R(0));
+        m.emission.x = fmx(et.x, R(0)) * s, m.emission.y = fmx(et.y, R(0)) * s, m.emission.z = fmx(et.z, R(0)) * s;
+    } else if (f & 128u) {
+        R s = fmx(su.w, R(0));
+        m.emission.x = fmx(m.emission.x, R(0)) * s, m.emission.y = fmx(m.emission.y, R(0)) * s,
+        m.emission.z = fmx(m.emission.z, R(0)) * s;
+    }
+    if (f & 32u) m.emission.w = clampv(et.w, R(0), R(1));
+    if (f & 8u) m.parameters.x = clampv(su.x, R(0.02), R(1));
+    if (f & 16u) m.parameters.y = clampv(su.y, R(0), R(1));
+    if (f & 64u) m.parameters2.x = clampv(su.z, R(0), R(1));
+}
+
 template <class R> Mat<R> materialForHit(const Ctx<R> &c, const Hit<R> &h) {  // :1096-1137
     uint32_t count = uint32_t(c.s->materials.size());
     uint32_t a = std::min<uint32_t>(h.materialID, count - 1u);
@@ -858,7 +895,8 @@ template <class R> Mat<R> materialForHit(const Ctx<R> &c, const Hit<R> &h) {  //
         uint32_t b = std::min<uint32_t>(h.materialID2, count - 1u);
         m = blendMaterials(m, toMat<R>(c.s->materials[b]), h.materialBlend);
     }
-    return m;  // no material-field flags or programs on this path
+    applyVolumeMaterialFields(m, h);
+    return m;  // no material programs on this path
 }
 
 template <class R> V4<R> texel(const or_scene &s, uint32_t i) { return v4<R>(s.texels[i]); }
@@ -1529,6 +1567,7 @@ int or_scene_create(const or_scene_desc *d, uint32_t maxLeaf, or_scene **out) {
     if (d->volumeCount) s->volumes.assign(d->volumes, d->volumes + d->volumeCount);
     if (d->brickCount) s->bricks.assign(d->bricks, d->bricks + d->brickCount);
     if (d->brickSampleCount) s->brickSamples.assign(d->brickSamples, d->brickSamples + d->brickSampleCount);
+    if (d->volumeSampleCount) s->volumeSamples.assign(d->volumeSamples, d->volumeSamples + d->volumeSampleCount);
     if (d->brickGridCount) s->brickGrids.assign(d->brickGrids, d->brickGrids + d->brickGridCount);
     if (d->brickGridIndexCount)
         s->brickGridIndices.assign(d->brickGridIndices, d->brickGridIndices + d->brickGridIndexCount);

@@ -67,6 +67,12 @@ typedef struct {
 
 /* GPUVolumeBrickSample (PathTrace.metal:103-108), 16 B. */
 typedef struct { float distance; uint32_t materialA, materialB; float materialBlend; } or_brick_sample;
+/* GPUVolumeSample (PathTrace.metal:92-101), dense volumes: 80 B. */
+typedef struct {
+    float distance; uint32_t materialA, materialB; float materialBlend;
+    or_float4 baseColorOpacity, emissionTransmission, surface;
+    uint32_t materialFieldFlags[4];
+} or_volume_sample;
 
 /* GPUVolumeBrickGrid (PathTrace.metal:143-148), 64 B. */
 typedef struct {
@@ -101,6 +107,7 @@ typedef struct {
     const or_texture_desc *textureDescs; uint32_t textureCount;
     const or_float4 *texturePixels; uint64_t texturePixelCount;
     const or_volume *volumes; uint32_t volumeCount;
+    const or_volume_sample *volumeSamples; uint32_t volumeSampleCount;
     const or_brick *bricks; uint32_t brickCount;
     const or_brick_sample *brickSamples; uint32_t brickSampleCount;
     const or_brick_grid *brickGrids; uint32_t brickGridCount;

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 CNT_PRIMARY, CNT_BOUNCE, CNT_SHADOW, CNT_NODES, CNT_TRIS = 0, 1, 2, 3, 4
-CNT_SDF = 8  # + the reference's SDF counter slot (2 grid cells ... 9 brick hits)
+CNT_SDF = 8  # + the reference's SDF counter slot (0 dense tests, 1 dense steps, 2 grid cells ... 9 brick hits)
 N_COUNTERS = 24
 
 
@@ -36,6 +36,7 @@ class _SceneDesc(C.Structure):
         ("textureDescs", C.c_void_p), ("textureCount", C.c_uint32),
         ("texturePixels", C.c_void_p), ("texturePixelCount", C.c_uint64),
         ("volumes", C.c_void_p), ("volumeCount", C.c_uint32),
+        ("volumeSamples", C.c_void_p), ("volumeSampleCount", C.c_uint32),
         ("bricks", C.c_void_p), ("brickCount", C.c_uint32),
         ("brickSamples", C.c_void_p), ("brickSampleCount", C.c_uint32),
         ("brickGrids", C.c_void_p), ("brickGridCount", C.c_uint32),
@@ -94,11 +95,11 @@ class OracleScene:
                                                          build.environment_samples, build.texture_descriptors,
                                                          build.texture_pixels, build.volumes, build.volume_bricks,
                                                          build.volume_brick_samples, build.volume_brick_grids,
-                                                         build.volume_brick_grid_indices)]
-        t, m, li, e, td, tp, v, vb, vs, vg, vi = self._keep
+                                                         build.volume_brick_grid_indices, build.volume_samples)]
+        t, m, li, e, td, tp, v, vb, vs, vg, vi, ds = self._keep
         d = _SceneDesc(_ptr(t), len(t), _ptr(m), len(m), _ptr(li), len(li), _ptr(e), len(e),
-                       _ptr(td), len(td), _ptr(tp), len(tp), _ptr(v), len(v), _ptr(vb), len(vb), _ptr(vs), len(vs),
-                       _ptr(vg), len(vg), _ptr(vi), len(vi))
+                       _ptr(td), len(td), _ptr(tp), len(tp), _ptr(v), len(v), _ptr(ds), len(ds),
+                       _ptr(vb), len(vb), _ptr(vs), len(vs), _ptr(vg), len(vg), _ptr(vi), len(vi))
         h = C.c_void_p()
         rc = L.or_scene_create(C.byref(d), max_leaf, C.byref(h))
         if rc != 0:

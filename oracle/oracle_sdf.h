@@ -8,7 +8,9 @@
 //   volumeBrickNormal                 :2420-2469
 //   intersectVolumeBrickPrepared      :2608-2807
 //   closestSparseVolumeBrickDDAHit    :2881-3088
-//   closestVolumeHit (brick-grid path) :3090-3163
+//   closestVolumeHit (dense loop, brick-grid path) :3090-3163
+//   dense volumes: sampleVolumeDistance :1915-1955, sampleVolumeMaterial
+//   :1957-1990, volumeNormal :2029-2055, intersectVolume :2471-2606
 // Included by oracle.cpp inside its anonymous namespace; uses its vector
 // types, the or_* records and Hit<R>.
 #pragma once
@@ -47,6 +49,8 @@ template <class R> struct BrickCtx {  // VolumeBrickSampleContext
 struct SdfScene {
     const or_volume *volumes = nullptr;
     uint32_t volumeCount = 0;
+    const or_volume_sample *dense = nullptr;
+    uint32_t denseCount = 0;
     const or_brick *bricks = nullptr;
     uint32_t brickCount = 0;
     const or_brick_sample *samples = nullptr;
@@ -219,10 +223,11 @@ template <class R> struct VolHit {
     R blend;
     uint32_t objectID, primitiveID;
     bool frontFacing;
+    const or_volume_sample *fields;  // dense hits: the nearest sample (material fields)
 };
 
 // Counter slots (the reference's sdfCounter* constants, PathTrace.metal:191-200).
-enum { SC_GRID_CELLS = 2, SC_MACRO_SKIPS = 3, SC_BRICK_TESTS = 4, SC_BRICK_INVALID = 5, SC_RANGE_CULLS = 6,
+enum { SC_DENSE_TESTS = 0, SC_DENSE_STEPS = 1, SC_GRID_CELLS = 2, SC_MACRO_SKIPS = 3, SC_BRICK_TESTS = 4, SC_BRICK_INVALID = 5, SC_RANGE_CULLS = 6,
        SC_MARCHES = 7, SC_MARCH_STEPS = 8, SC_BRICK_HITS = 9 };
 
 template <class R>
@@ -457,11 +462,138 @@ VolHit<R> closestSparseVolumeBrickDDAHit(const SdfScene &S, uint32_t volumeIndex
     return closest;
 }
 
-// closestVolumeHit, brick-grid branch (:3135-3163): sparse volumes only.
+// ------------------------------------------------------- dense volumes --
+// sampleVolumeDistance (:1915-1955): trilinear over the dense grid.
+template <class R> R denseDistance(const SdfScene &S, const or_volume &vol, V3<R> lp) {
+    uint32_t dim[3] = {vol.dimensions.x, vol.dimensions.y, vol.dimensions.z};
+    uint32_t off = vol.metadata.x;
+    uint32_t n = dim[0] * dim[1] * dim[2];
+    if (off >= S.denseCount || off + n > S.denseCount) return kInf<R>();
+    V3<R> lmin = v3<R>(vol.localBoundsMin);
+    V3<R> ext = vmax(v3<R>(vol.localBoundsMax) - lmin, V3<R>(R(1e-6)));
+    V3<R> uvw = vclamp((lp - lmin) / ext, V3<R>(0), V3<R>(1));
+    R g[3] = {uvw.x * R(dim[0] - 1u), uvw.y * R(dim[1] - 1u), uvw.z * R(dim[2] - 1u)};
+    uint32_t b[3], nx[3];
+    R f[3];
+    for (int a = 0; a < 3; ++a) {
+        b[a] = uint32_t(std::floor(g[a]));
+        nx[a] = std::min(b[a] + 1u, dim[a] - 1u);
+        f[a] = g[a] - R(b[a]);
+    }
+    uint32_t sy = dim[0], sz = dim[0] * dim[1];
+    auto d = [&](uint32_t x, uint32_t y, uint32_t z) { return R(S.dense[off + x + y * sy + z * sz].distance); };
+    R c00 = mixs(d(b[0], b[1], b[2]), d(nx[0], b[1], b[2]), f[0]);
+    R c10 = mixs(d(b[0], nx[1], b[2]), d(nx[0], nx[1], b[2]), f[0]);
+    R c01 = mixs(d(b[0], b[1], nx[2]), d(nx[0], b[1], nx[2]), f[0]);
+    R c11 = mixs(d(b[0], nx[1], nx[2]), d(nx[0], nx[1], nx[2]), f[0]);
+    R c0 = mixs(c00, c10, f[1]);
+    R c1 = mixs(c01, c11, f[1]);
+    return mixs(c0, c1, f[2]);
+}
+
+// sampleVolumeMaterial (:1957-1990): the nearest sample, or the volume's
+// material when the sample range is invalid (nullptr here).
+template <class R> const or_volume_sample *denseNearest(const SdfScene &S, const or_volume &vol, V3<R> lp) {
+    uint32_t dim[3] = {vol.dimensions.x, vol.dimensions.y, vol.dimensions.z};
+    uint32_t off = vol.metadata.x;
+    uint32_t n = dim[0] * dim[1] * dim[2];
+    if (off >= S.denseCount || off + n > S.denseCount) return nullptr;
+    V3<R> ext = vmax(v3<R>(vol.localBoundsMax) - v3<R>(vol.localBoundsMin), V3<R>(R(1e-6)));
+    V3<R> uvw = vclamp((lp - v3<R>(vol.localBoundsMin)) / ext, V3<R>(0), V3<R>(1));
+    R u[3] = {uvw.x, uvw.y, uvw.z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = std::min(uint32_t(std::round(u[a] * R(dim[a] - 1u))), dim[a] - 1u);
+    return &S.dense[off + q[0] + q[1] * dim[0] + q[2] * dim[0] * dim[1]];
+}
+
+// volumeNormal (:2029-2055): central differences one sample apart.
+template <class R> V3<R> denseNormal(const SdfScene &S, const or_volume &vol, V3<R> lp) {
+    V3<R> ext = vmax(v3<R>(vol.localBoundsMax) - v3<R>(vol.localBoundsMin), V3<R>(R(1e-6)));
+    V3<R> sp = ext / vmax(V3<R>(R(vol.dimensions.x), R(vol.dimensions.y), R(vol.dimensions.z)) - V3<R>(1), V3<R>(1));
+    R dx = denseDistance(S, vol, lp + V3<R>(sp.x, 0, 0)) - denseDistance(S, vol, lp - V3<R>(sp.x, 0, 0));
+    R dy = denseDistance(S, vol, lp + V3<R>(0, sp.y, 0)) - denseDistance(S, vol, lp - V3<R>(0, sp.y, 0));
+    R dz = denseDistance(S, vol, lp + V3<R>(0, 0, sp.z)) - denseDistance(S, vol, lp - V3<R>(0, 0, sp.z));
+    V3<R> ln(dx, dy, dz);
+    if (dot(ln, ln) <= R(1e-10)) ln = V3<R>(0, 1, 0);
+    Mat4<R> nt = mat4<R>(&vol.normalTransform0);
+    return normalize(mul(nt, normalize(ln), R(0)));
+}
+
+// intersectVolume (:2471-2606): march in world t, <= 512 steps, 10 bisections.
+template <class R>
+bool intersectDenseVolume(const SdfScene &S, const or_volume &vol, const Ray<R> &ray, R closestT, uint64_t *sc,
+                          VolHit<R> &hit) {
+    R tNear, tFar;
+    if (!intersectAABB(ray, v3<R>(vol.worldBoundsMin), v3<R>(vol.worldBoundsMax), closestT, tNear, tFar)) return false;
+    Mat4<R> w2l = mat4<R>(&vol.worldToLocal0);
+    V3<R> localStep = mul(w2l, ray.direction, R(0));
+    R lupw = fmx(length(localStep), R(1e-6));
+    V3<R> ext = vmax(v3<R>(vol.localBoundsMax) - v3<R>(vol.localBoundsMin), V3<R>(R(1e-6)));
+    R cellSize = fmn(ext.x / fmx(R(vol.dimensions.x) - R(1), R(1)),
+                     fmn(ext.y / fmx(R(vol.dimensions.y) - R(1), R(1)), ext.z / fmx(R(vol.dimensions.z) - R(1), R(1))));
+    R minimumStep = fmx(cellSize * R(0.10) / lupw, R(0.0005));
+    R prevT = tNear;
+    V3<R> prevW = ray.origin + ray.direction * prevT;
+    R prevD = denseDistance(S, vol, mul(w2l, prevW, R(1)));
+    if (!std::isfinite(prevD)) return false;
+    R t = prevT + fmx(std::fabs(prevD) / lupw * R(0.9), minimumStep);
+    for (uint32_t step = 0; step < 512u && t <= tFar && t < closestT; ++step) {
+        sc[SC_DENSE_STEPS]++;
+        V3<R> w = ray.origin + ray.direction * t;
+        R d = denseDistance(S, vol, mul(w2l, w, R(1)));
+        if (!std::isfinite(d)) return false;
+        bool crossed = (prevD > R(0) && d <= R(0)) || (prevD < R(0) && d >= R(0));
+        if (crossed) {
+            R lowT = prevT, highT = t, lowD = prevD, refinedT = t;
+            for (uint32_t k = 0; k < 10u; ++k) {
+                R midT = R(0.5) * (lowT + highT);
+                R md = denseDistance(S, vol, mul(w2l, ray.origin + ray.direction * midT, R(1)));
+                if ((lowD > R(0) && md > R(0)) || (lowD < R(0) && md < R(0))) {
+                    lowT = midT;
+                    lowD = md;
+                } else {
+                    highT = midT;
+                }
+                refinedT = R(0.5) * (lowT + highT);
+            }
+            V3<R> rw = ray.origin + ray.direction * refinedT;
+            V3<R> rl = mul(w2l, rw, R(1));
+            V3<R> n = denseNormal(S, vol, rl);
+            bool ff = dot(n, ray.direction) <= R(0);
+            if (!ff) n = -n;
+            const or_volume_sample *ms = denseNearest(S, vol, rl);
+            hit.hit = true;
+            hit.t = refinedT;
+            hit.position = rw;
+            hit.normal = n;
+            hit.materialA = ms ? ms->materialA : vol.dimensions.w;
+            hit.materialB = ms ? ms->materialB : vol.dimensions.w;
+            hit.blend = ms ? clampv(R(ms->materialBlend), R(0), R(1)) : R(0);
+            hit.fields = ms;
+            hit.objectID = vol.metadata.y;
+            hit.primitiveID = vol.metadata.z;
+            hit.frontFacing = ff;
+            return true;
+        }
+        prevT = t;
+        prevD = d;
+        t += fmx(std::fabs(d) / lupw * R(0.8), minimumStep);
+    }
+    return false;
+}
+
+// closestVolumeHit (:3090-3163): dense volumes, then the brick-grid branch
+// for sparse volumes.
 template <class R> VolHit<R> closestVolumeHit(const SdfScene &S, const Ray<R> &ray, R closestT, uint64_t *sc) {
     VolHit<R> closest{};
     closest.hit = false;
     closest.t = closestT;
+    for (uint32_t v = 0; v < S.volumeCount; ++v) {
+        if (S.volumes[v].metadata.w != 0u) continue;
+        sc[SC_DENSE_TESTS]++;
+        VolHit<R> cand{};
+        if (intersectDenseVolume(S, S.volumes[v], ray, closest.t, sc, cand) && cand.t < closest.t) closest = cand;
+    }
     if (S.gridCount == 0u || S.gridIndexCount == 0u) return closest;
     for (uint32_t v = 0; v < S.volumeCount; ++v) {
         if (S.volumes[v].metadata.w != 1u) continue;

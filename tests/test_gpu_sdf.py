@@ -1,12 +1,13 @@
-"""GPU parity of the sparse signed-distance path (brick DDA, march,
-bisection, cubic polish, normals, material blend) against the CPU oracle,
-through the C ABI.
+"""GPU parity of the signed-distance paths against the CPU oracle, through
+the C ABI: sparse bricks (DDA, march, bisection, cubic polish, normals,
+material blend) and dense grids (march, bisection, central-difference
+normals, material fields).
 
 Tolerances:
   * material / object / primitive IDs: bit-exact;
   * depth, normal, albedo: |diff| <= 1e-5 (the same IEEE operations over
     the same FP32 values: FP16-stored distances widen exactly);
-  * sparse-volume counters: exactly equal when every ray is a primary ray
+  * SDF counters: exactly equal when every ray is a primary ray
     (no lights, no environment texture, one bounce); within 1% otherwise
     (shading transcendentals differ by ulps between libm and the GPU and can
     move a rare secondary ray);
@@ -156,3 +157,49 @@ def test_two_volumes_with_transform_and_triangles(renderer):
     _check_beauty(sess, ref)
     objs = set(np.unique(sess.pixels(RenderOutput.OBJECT_ID)[..., 0]).tolist())
     assert {1.0, 2.0, 3.0} <= objs  # ground (instance 0) and both volumes
+
+
+@pytest.mark.parametrize("name", ["distance_volume_reference", "sparse_distance_volume_reference"])
+def test_reference_volume_scene_parity(renderer, name):
+    # RenderScene.distanceVolumeReference / sparseDistanceVolumeReference
+    # (RenderScene.swift:770-797): triangles, lights, a dielectric and the SDF
+    # model as a dense grid or as reference-format FP32 bricks.
+    from denrimrendererkit_amd import RenderScene
+    scene = getattr(RenderScene, name)()
+    sess, ref = _render_pair(renderer, scene, 96, 64, spp=2, bounces=4)
+    _check_ids_and_aovs(sess, ref)
+    _check_beauty(sess, ref)
+
+
+def test_dense_volume_primary_counters_exact(renderer):
+    # Dense march (intersectVolume): every query a primary ray, so the dense
+    # counters are the same sums of the same marches.
+    from denrimrendererkit_amd import Camera, Material, RenderScene, Transform
+    from denrimrendererkit_amd.sdf import build_volume, SDFPrimitive
+    scene = RenderScene(camera=Camera(origin=(0.2, 0.6, 2.6), target=(0, 0.1, 0)))
+    a = scene.add_material(Material(base_color=(0.9, 0.3, 0.2)))
+    b = scene.add_material(Material(base_color=(0.2, 0.4, 0.9), metallic=1, roughness=0.3))
+    prims = [SDFPrimitive("box", a, half_extents=(0.4, 0.2, 0.3), corner_radius=0.05),
+             SDFPrimitive("sphere", b, radius=0.3, transform=Transform.translation((0.3, 0.2, 0.1)),
+                          smooth_union_radius=0.1)]
+    scene.add_volume(build_volume(prims, (40, 30, 34), (-0.7, -0.5, -0.6), (0.7, 0.6, 0.6)), a,
+                     Transform.rotation_y(0.4))
+    sess, ref = _render_pair(renderer, scene, 64, 48, spp=1, bounces=1)
+    _check_ids_and_aovs(sess, ref)
+    s = sess.sdf_traversal_stats()
+    assert s.dense_volume_tests == int(ref.counters[O.CNT_SDF + 0]) > 0
+    assert s.dense_march_steps == int(ref.counters[O.CNT_SDF + 1]) > 0
+
+
+def test_volume_material_fields_override_albedo(renderer):
+    # RenderSessionTests.swift:337-376 on the GPU, equal to the oracle.
+    from denrimrendererkit_amd.sdf import MaterialFields
+    from test_sdf import dense_only_scene
+    scene = dense_only_scene(MaterialFields(base_color=(0.08, 0.82, 0.24), opacity=0.64, roughness=0.3,
+                                            emission=(0.2, 0.1, 0.0), emission_strength=2.0))
+    sess, ref = _render_pair(renderer, scene, 20, 20, bounces=2)
+    _check_ids_and_aovs(sess, ref)
+    _check_beauty(sess, ref)
+    a = sess.pixels(RenderOutput.ALBEDO)
+    assert np.any((a[..., 0] < 0.2) & (a[..., 1] > 0.7) & (a[..., 2] > 0.15) & (a[..., 2] < 0.35) & (a[..., 3] > 0.55)
+                  & (a[..., 3] < 0.75))

@@ -259,3 +259,118 @@ def test_oracle_sparse_quality_close_to_denser_reference():
     ref = quality_probe(64, 2, _oracle_probe)
     m = aov_metrics(low, ref)
     assert m["coverage"] > 0.20 and m["normal_rmse"] < 0.30 and m["depth_rmse"] < 0.05 and m["silhouette"] < 0.02
+
+
+# ------------------------------------------------------ model and dense --
+def _np_primitive_distance(p, prim, xf):
+    """CompiledPrimitive.distance (SDFModel.swift:685-713) in float32 numpy."""
+    if xf is not None:
+        inv = np.linalg.inv(xf.matrix.astype(np.float64)).astype(f32)
+        l = [((inv[r, 0] * p[..., 0] + inv[r, 1] * p[..., 1]) + inv[r, 2] * p[..., 2]) + inv[r, 3] for r in range(3)]
+        l = np.stack(l, -1).astype(f32)
+        cols = xf.matrix[:3, :3].T.astype(f32)
+        scale = f32(max(float(np.sqrt((cols * cols).sum(1)).min()), 1e-6))
+    else:
+        l, scale = p.astype(f32), f32(1)
+    if prim.kind == "sphere":
+        d = np.sqrt((l * l).sum(-1)) - f32(prim.radius)
+    elif prim.kind == "box":
+        q = np.abs(l) - np.asarray(prim.half_extents, f32)
+        d = (np.sqrt((np.maximum(q, 0) ** 2).sum(-1)) + np.minimum(q.max(-1), 0) - f32(max(prim.corner_radius, 0)))
+    else:  # cylinder
+        dx = np.sqrt(l[..., 0] ** 2 + l[..., 2] ** 2) - f32(prim.radius)
+        dy = np.abs(l[..., 1]) - f32(prim.half_height)
+        d = np.minimum(np.maximum(dx, dy), 0) + np.sqrt(np.maximum(dx, 0) ** 2 + np.maximum(dy, 0) ** 2)
+    return (d * scale).astype(f32)
+
+
+def test_dense_builder_shapes_transforms_and_subtract():
+    # DistanceVolumeBuilder.build over a box, a capped cylinder and a
+    # non-uniformly scaled, rotated sphere, then a subtracted cylinder
+    # (sampleField, SDFModel.swift:480-505), against numpy on every sample.
+    from denrimrendererkit_amd.sdf import build_volume
+    T = Transform
+    box = SDFPrimitive("box", 1, half_extents=(0.3, 0.2, 0.25), corner_radius=0.03,
+                       transform=T.translation((-0.3, 0.0, 0.1)) * T.rotation_z(0.3))
+    cyl = SDFPrimitive("cylinder", 2, radius=0.18, half_height=0.3, transform=T.translation((0.35, 0.05, 0.0)))
+    sph = SDFPrimitive("sphere", 3, radius=0.25,
+                       transform=T.translation((0.0, -0.3, -0.2)) * T.rotation_y(0.5) * T.scale((1.2, 0.8, 1.0)))
+    cut = SDFPrimitive("cylinder", 4, radius=0.08, half_height=0.9, operation="subtract",
+                       transform=T.translation((-0.3, 0.0, 0.1)) * T.rotation_x(1.5707964))
+    dims, bmin, bmax = (21, 18, 19), (-0.8, -0.7, -0.6), (0.8, 0.7, 0.6)
+    vol = build_volume([box, cyl, sph, cut], dims, bmin, bmax)
+    ext = np.asarray(bmax, f32) - np.asarray(bmin, f32)
+    axes = [f32(bmin[a]) + ext[a] * (np.arange(dims[a], dtype=f32) / f32(dims[a] - 1)) for a in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    p = np.stack([x, y, z], -1).reshape(-1, 3)
+    ds = [_np_primitive_distance(p, q, q.transform) for q in (box, cyl, sph, cut)]
+    union = np.minimum(np.minimum(ds[0], ds[1]), ds[2])
+    expect = np.maximum(union, -ds[3])
+    np.testing.assert_allclose(vol.distances, expect, atol=3e-6, rtol=0)
+    owner = np.argmin(np.stack(ds[:3]), 0) + 1
+    assert np.mean(vol.material_a == owner) > 0.99 and np.all(vol.blend == 0)
+
+
+@pytest.mark.parametrize("res", [32, 64])
+def test_oracle_dense_hits_match_closed_form_sphere(res):
+    from denrimrendererkit_amd.sdf import sphere_volume
+    s = RenderScene(camera=Camera(origin=(0, 0, 3), target=(0, 0, 0)))
+    m = s.add_material(Material(base_color=(0.85, 0.18, 0.12)))
+    s.add_volume(sphere_volume(res, 0.55), m, Transform.translation((0.2, -0.1, 0.3)))
+    sc = O.OracleScene(compile_scene(s))
+    C, R = np.array([0.2, -0.1, 0.3]), 0.55
+    rays = _rays_at_sphere(np.random.default_rng(7 + res), 3000, C, R)
+    t, ids, hit = sc.intersect(rays, with_hit=True)
+    ta, na, disc = _analytic_sphere(rays, C, R)
+    h = ids != 0xFFFFFFFF
+    assert np.all(h[disc > (0.05 * R) ** 2])
+    voxel = 2.0 / (res - 1)
+    dt = np.abs(t[h] - ta[h])
+    assert np.quantile(dt, 0.99) < 0.1 * voxel and dt.max() < 0.5 * voxel
+    dn = np.linalg.norm(hit[h, :3] - na[h], axis=1)
+    assert np.quantile(dn, 0.99) < voxel / R and dn.max() < 2 * voxel / R
+
+
+def dense_only_scene(fields=None):
+    # RenderSessionTests.swift:158-200 (and :337-376 with material fields).
+    from denrimrendererkit_amd.sdf import sphere_volume
+    s = RenderScene(camera=Camera(origin=(0, 0, 3), target=(0, 0, 0), orthographic_scale=2.4))
+    m = s.add_material(Material(base_color=(0.85, 0.18, 0.12), roughness=0.7))
+    v = sphere_volume(20, 0.55)
+    if fields is not None:
+        v.fields = fields.samples(len(v.distances))
+    s.add_volume(v, m)
+    return s
+
+
+def test_oracle_volume_only_scene_renders():
+    s = dense_only_scene()
+    b = compile_scene(s)
+    assert len(b.triangles) == 0 and len(b.volume_samples) == 20 ** 3
+    r = O.render(O.OracleScene(b), O.make_settings(b, 20, 20, max_bounces=1), s.camera.gpu_camera(20, 20))
+    beauty, alb, nrm = r.accumulation, r.aovs[2], r.aovs[1]
+    assert np.any(np.isfinite(beauty[..., :3]).all(-1) & (beauty[..., :3] > 0).any(-1))
+    assert np.any((alb[..., 0] > 0.5) & (alb[..., 1] < 0.3) & (alb[..., 2] < 0.25) & (alb[..., 3] > 0.9))
+    assert np.any((nrm[..., 3] > 0.9) & (np.abs(nrm[..., :3] - 0.5) > 0.01).any(-1))
+    assert r.counters[SDF + 0] > 0 and r.counters[SDF + 1] > 0
+
+
+def test_oracle_volume_material_fields_override_albedo():
+    from denrimrendererkit_amd.sdf import MaterialFields
+    s = dense_only_scene(MaterialFields(base_color=(0.08, 0.82, 0.24), opacity=0.64))
+    b = compile_scene(s)
+    r = O.render(O.OracleScene(b), O.make_settings(b, 20, 20, max_bounces=1), s.camera.gpu_camera(20, 20))
+    a = r.aovs[2]
+    assert np.any((a[..., 0] < 0.2) & (a[..., 1] > 0.7) & (a[..., 2] > 0.15) & (a[..., 2] < 0.35) & (a[..., 3] > 0.55)
+                  & (a[..., 3] < 0.75))
+
+
+@pytest.mark.parametrize("name", ["distance_volume_reference", "sparse_distance_volume_reference"])
+def test_reference_volume_scenes_compile_and_render(name):
+    s = getattr(RenderScene, name)()
+    b = compile_scene(s)
+    assert len(b.volumes) == 1 and len(b.triangles) == 8
+    assert int(b.volumes["metadata"][0, 1]) == 4  # objectID after the four quads
+    r = O.render(O.OracleScene(b), O.make_settings(b, 48, 32, max_bounces=2), s.camera.gpu_camera(48, 32))
+    assert set(np.unique(r.aovs[4][..., 0]).tolist()) >= {1.0, 2.0, 5.0}
+    assert np.isfinite(r.accumulation).all()
