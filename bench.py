@@ -97,7 +97,21 @@ def main():
     ap.add_argument("--workload", default="terrain_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 256 MiB (5 times): the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20  # float32 elements: 256 MiB
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        assert bool((dst[-4:] == src[-4:]).all())
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)

@@ -143,21 +143,27 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0;
     DrSdfCounters sc{};
-    while (true) {
-        uint32_t base = 0;
-        if (laneId() == 0) base = atomicAdd(steal, 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        uint32_t qi = base + laneId();
-        if (qi < count) {
-            uint32_t slot = queue[qi];
-            float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
-            if constexpr (!VOL) {
-                DrRayPre r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
-                DrHitRec h{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
-                dr_traverse<false>(S, r, h, stk, nv, nt);
-                P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-            } else {
+    if constexpr (!VOL) {
+        uint32_t slot = 0;
+        dr_trace_queue<false>(
+            S, count, steal, stk, nv, nt,
+            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+                slot = queue[qi];
+                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+            },
+            [&](const DrHitRec &h, bool) { P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id)); });
+    } else {
+        while (true) {
+            uint32_t base = 0;
+            if (laneId() == 0) base = atomicAdd(steal, 64u);
+            base = __shfl(base, 0);
+            if (base >= count) break;
+            uint32_t qi = base + laneId();
+            if (qi < count) {
+                uint32_t slot = queue[qi];
+                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
                 float4 h, hn;
                 sceneClosest<true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, nv, nt, sc, h, hn);
                 P.hit[slot] = h;
@@ -725,7 +731,24 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
     DrSdfCounters sc{};
-    while (true) {
+    if (!VOL && S.opaqueOnly) {
+        // No material lets light through: occluded iff any triangle lies
+        // below maxDistance (the limit is exclusive: id 0 never wins a tie).
+        uint32_t k = 0;
+        dr_trace_queue<true>(
+            S, count, steal, stk, nv, nt,
+            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+                k = P.shadowQueue[qi];
+                float4 o4 = P.shO[k], d4 = P.shD[k];
+                r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                best = DrHitRec{o4.w, 0.0f, 0.0f, 0u};
+                ++nq;
+            },
+            [&](const DrHitRec &, bool occluded) {
+                P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            });
+    }
+    while (VOL || !S.opaqueOnly) {
         uint32_t base = 0;
         if (laneId() == 0) base = atomicAdd(steal, 64u);
         base = __shfl(base, 0);
@@ -738,12 +761,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         float maxDistance = o4.w;
         vec3 tr = v3s(1.0f);
         bool occluded;
-        if (!VOL && S.opaqueOnly) {
-            ++nq;
-            DrRayPre r = dr_prep(o, d);
-            DrHitRec h{maxDistance, 0.0f, 0.0f, 0u};  // id 0: the limit is exclusive
-            occluded = dr_traverse<true>(S, r, h, stk, nv, nt);
-        } else {
+        {
             occluded = true;
             float traveled = 0.0f;
             for (uint32_t step = 0; step < 8u; ++step) {

@@ -159,7 +159,7 @@ __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, co
         return brickDistanceGradient(V, c, p);
     vec3 f = v3(g[0] - (float)b[0], g[1] - (float)b[1], g[2] - (float)b[2]);
     float yr[4], dyr[4], dxyr[4];
-#pragma unroll
+#pragma unroll 1
     for (uint32_t z = 0; z < 4; ++z) {
         float xr[4], dxr[4];
 #pragma unroll
@@ -237,6 +237,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
     if (prevD <= 0.0f) {
         float backStep = fmaxf(cellSize / lupw * 0.5f, minimumStep);
         float searchT = prevT;
+        #pragma unroll 1
         for (uint32_t bt = 0; bt < 12u; ++bt) {
             searchT = fmaxf(searchT - backStep, 0.0005f);
             float sd = brickDistance(V, ctx, lo + ld * searchT);
@@ -250,6 +251,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
         }
     }
     float t = fmaxf(originalEntryT, prevT + fmaxf(fabsf(prevD) / lupw * 0.9f, minimumStep));
+    #pragma unroll 1
     for (uint32_t step = 0; step < 192u && t <= tFar && t < closestT; ++step) {
         sc.marchSteps++;
         float d = brickDistance(V, ctx, lo + ld * t);
@@ -258,6 +260,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
         if (crossed) {
             float lowT = prevT, highT = t, lowD = prevD;
             float refinedT = t;
+            #pragma unroll 1
             for (uint32_t k = 0; k < 8u; ++k) {
                 float midT = 0.5f * (lowT + highT);
                 float md = brickDistance(V, ctx, lo + ld * midT);
@@ -272,10 +275,19 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
             bool cubicPolish = cellSize <= 0.03f;
             bool coarse = V.renderQuality > 0u && cellSize > 0.03f;
             vec3 rp = lo + ld * refinedT;
+            vec3 n;
             if (cubicPolish) {
+                // Newton polish steps and the cubic normal share one
+                // evaluation site: the last pass (or the pass that stops the
+                // polish) evaluates the gradient at the final point, which is
+                // what volumeBrickNormal computes there.
                 uint32_t polishCount = V.renderQuality > 1u ? 2u : 1u;
-                for (uint32_t k = 0; k < polishCount; ++k) {
+                vec3 ln;
+                #pragma unroll 1
+                for (uint32_t k = 0;; ++k) {
                     DrDG pg = brickDistanceGradientCubic(V, ctx, rp);
+                    ln = pg.gradient;
+                    if (k == polishCount) break;
                     float slope = dot(pg.gradient, ld);
                     if (!isfinite(pg.distance) || fabsf(slope) <= 1e-6f) break;
                     float projected = refinedT - pg.distance / slope;
@@ -283,8 +295,11 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
                     refinedT = projected;
                     rp = lo + ld * refinedT;
                 }
+                if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
+                n = normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
+            } else {
+                n = brickNormal(V, vol, ctx, rp, false, coarse);
             }
-            vec3 n = brickNormal(V, vol, ctx, rp, cubicPolish, coarse);
             bool ff = dot(n, rd) <= 0.0f;
             if (!ff) n = -n;
             // Nearest sample's material (sampleVolumeBrickMaterial, :2350-2356).
@@ -366,6 +381,7 @@ __device__ __forceinline__ bool intersectDense(const DrSdfDev &V, const DrVolume
     float prevD = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * prevT, 1.0f));
     if (!isfinite(prevD)) return false;
     float t = prevT + fmaxf(fabsf(prevD) / lupw * 0.9f, minimumStep);
+    #pragma unroll 1
     for (uint32_t step = 0; step < 512u && t <= tFar && t < closestT; ++step) {
         sc.denseSteps++;
         float d = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * t, 1.0f));
@@ -373,6 +389,7 @@ __device__ __forceinline__ bool intersectDense(const DrSdfDev &V, const DrVolume
         bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
         if (crossed) {
             float lowT = prevT, highT = t, lowD = prevD, refinedT = t;
+            #pragma unroll 1
             for (uint32_t k = 0; k < 10u; ++k) {
                 float midT = 0.5f * (lowT + highT);
                 float md = denseDistance(V, vol, mulCols(vol.w2l, ro + rd * midT, 1.0f));
@@ -484,6 +501,7 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
         bool volHit = false;
         DrVolHit closest;
         closest.t = best;
+        #pragma unroll 1
         for (uint32_t step = 0; step < maxSteps && t <= tFar && t < closest.t; ++step) {
             sc.gridCells++;
             if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] >= (int32_t)gd[0] || cell[1] >= (int32_t)gd[1] ||

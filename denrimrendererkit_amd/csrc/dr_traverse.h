@@ -3,8 +3,9 @@
 // CDNA4 has no ray-tracing units, so traversal is plain VALU work:
 //   * one node fetch (4 x dwordx4, 64 B) tests both child boxes with a
 //     fused-multiply-add slab test against precomputed 1/d and o/d;
-//   * leaf children are intersected as soon as they are hit, nearer inner
-//     child first, the farther one pushed;
+//   * while-while order: inner nodes (nearer child first, the farther one
+//     pushed) until every lane of the wave holds a hit leaf, then the
+//     leaves' triangles, converged;
 //   * the traversal stack lives in LDS (DR_LDS_STACK entries per lane,
 //     lane-interleaved so a wave's pushes hit 64 distinct banks) and spills
 //     to a per-lane global area only past that depth.
@@ -109,72 +110,200 @@ struct DrHitRec {
     uint32_t id;  // original triangle index, 0xffffffff = miss
 };
 
-// Closest hit (ANY = false) or any hit below tmax (ANY = true).
-// On entry best.t is the search limit; best.id must be 0xffffffff for a
-// closest-hit query and the result is the lowest-index triangle among those
-// with minimal t < limit.
+// Traversal state of one ray in the while-while form: inner nodes are
+// visited until the lane holds a pending leaf (or finishes); leaves are
+// tested when every lane of the wave holds one, so triangle tests run
+// converged instead of serializing the wave once per lane.
+// Stack entries: an inner node index, or DR_LEAF_REF | node << 1 | child for
+// a hit leaf child waiting to be tested (its range is read back from the
+// node). The tree depth cap bounds the stack: one push per visited node.
+#define DR_POP 0xffffffffu
+#define DR_LEAF_REF 0x80000000u
+
+struct DrTrav {
+    uint32_t cur;        // inner node to visit next, or DR_POP
+    uint32_t sp;
+    uint32_t leafFirst;  // pending leaf: next triangle (leaf order)
+    uint32_t leafCount;  // triangles left in the pending leaf
+    bool done;
+};
+
+__device__ __forceinline__ void dr_trav_init(DrTrav &s) {
+    s.cur = 0;
+    s.sp = 0;
+    s.leafFirst = 0;
+    s.leafCount = 0;
+    s.done = false;
+}
+
+// One inner-node step; requires !s.done && s.leafCount == 0.
+__device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre &r, float tmax, DrTrav &s,
+                                             DrStack &stk, uint32_t &nv) {
+    if (s.cur == DR_POP) {
+        if (s.sp == 0) {
+            s.done = true;
+            return;
+        }
+        uint32_t e = stk.get(--s.sp);
+        if (e & DR_LEAF_REF) {
+            uint32_t node = (e & ~DR_LEAF_REF) >> 1;
+            uint4 d = S.nodes[node].d;
+            bool c1 = (e & 1u) != 0u;
+            s.leafFirst = c1 ? d.y : d.x;
+            s.leafCount = c1 ? d.w : d.z;
+            return;
+        }
+        s.cur = e;
+    }
+    const uint32_t ni = s.cur;
+    const DrNode n = S.nodes[ni];
+    ++nv;
+    float t0 = dr_box(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmax);
+    float t1 = (n.d.y == DR_EMPTY_CHILD) ? __builtin_inff() : dr_box(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmax);
+    bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
+    bool l0 = h0 && n.d.z != 0u, l1 = h1 && n.d.w != 0u;
+    bool i0 = h0 && n.d.z == 0u, i1 = h1 && n.d.w == 0u;
+    bool near0 = t0 <= t1;
+    if (l0 && l1) {
+        s.leafFirst = near0 ? n.d.x : n.d.y;
+        s.leafCount = near0 ? n.d.z : n.d.w;
+        stk.put(s.sp++, DR_LEAF_REF | ni << 1 | (near0 ? 1u : 0u));
+    } else if (l0) {
+        s.leafFirst = n.d.x;
+        s.leafCount = n.d.z;
+    } else if (l1) {
+        s.leafFirst = n.d.y;
+        s.leafCount = n.d.w;
+    }
+    if (i0 && i1) {
+        stk.put(s.sp++, near0 ? n.d.y : n.d.x);
+        s.cur = near0 ? n.d.x : n.d.y;
+    } else if (i0) {
+        s.cur = n.d.x;
+    } else if (i1) {
+        s.cur = n.d.y;
+    } else {
+        s.cur = DR_POP;
+    }
+}
+
+// One triangle of the pending leaf; requires s.leafCount > 0. Returns true
+// when the test hits and improves the record.
+__device__ __forceinline__ bool dr_leaf_step(const DrSceneDev &S, const DrRayPre &r, DrTrav &s, DrHitRec &best,
+                                             uint32_t &nt) {
+    const float4 *tp = S.triPos + size_t(s.leafFirst) * 3;
+    float4 A = tp[0], B = tp[1], C = tp[2];
+    ++nt;
+    ++s.leafFirst;
+    --s.leafCount;
+    float t, u, v;
+    if (dr_tri(r, A, B, C, t, u, v)) {
+        uint32_t id = __float_as_uint(A.w);
+        if (t < best.t || (t == best.t && id < best.id)) {
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            best.id = id;
+            return true;
+        }
+    }
+    return false;
+}
+
+// Closest hit (ANY = false) or any hit below tmax (ANY = true) of one ray
+// per lane. On entry best.t is the search limit; best.id must be 0xffffffff
+// for a closest-hit query and the result is the lowest-index triangle among
+// those with minimal t < limit (boxes are tested with t_near <= limit, so
+// equal-t candidates stay reachable in any visiting order).
 // Node fetches and triangle tests are added to nodeVisits / triTests.
+// Every active lane of the wave must call it (the phases use ballots).
 template <bool ANY>
 __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre &r, DrHitRec &best, DrStack &stk,
                                             uint32_t &nodeVisits, uint32_t &triTests) {
-    if (S.triangleCount == 0) return false;
-    uint32_t sp = 0;
-    uint32_t cur = 0;
+    DrTrav s;
+    dr_trav_init(s);
+    if (S.triangleCount == 0) s.done = true;
     uint32_t nv = 0, nt = 0;
     bool found = false;
     while (true) {
-        const DrNode n = S.nodes[cur];
-        ++nv;
-        float t0 = dr_box(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, best.t);
-        float t1 = (n.d.y == DR_EMPTY_CHILD) ? __builtin_inff()
-                                             : dr_box(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, best.t);
-        bool h0 = t0 != __builtin_inff();
-        bool h1 = t1 != __builtin_inff();
-        // Leaf children: intersect now.
-        for (int c = 0; c < 2; ++c) {
-            bool hit = c == 0 ? h0 : h1;
-            uint32_t cnt = c == 0 ? n.d.z : n.d.w;
-            if (!hit || cnt == 0) continue;
-            uint32_t first = c == 0 ? n.d.x : n.d.y;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tp = S.triPos + size_t(first + k) * 3;
-                float4 A = tp[0], B = tp[1], C = tp[2];
-                ++nt;
-                float t, u, v;
-                if (dr_tri(r, A, B, C, t, u, v)) {
-                    uint32_t id = __float_as_uint(A.w);
-                    if (t < best.t || (t == best.t && id < best.id)) {
-                        best.t = t;
-                        best.u = u;
-                        best.v = v;
-                        best.id = id;
-                        found = true;
-                        if (ANY) {
-                            nodeVisits += nv;
-                            triTests += nt;
-                            return true;
-                        }
-                    }
+        while (__any(!s.done && s.leafCount == 0)) {
+            if (!s.done && s.leafCount == 0) dr_node_step(S, r, best.t, s, stk, nv);
+        }
+        if (!__any(s.leafCount != 0u)) break;
+        while (__any(s.leafCount != 0u)) {
+            if (s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
+                found = true;
+                if (ANY) {
+                    s.done = true;
+                    s.leafCount = 0;
                 }
             }
-        }
-        bool g0 = h0 && n.d.z == 0;
-        bool g1 = h1 && n.d.w == 0;
-        if (g0 && g1) {
-            uint32_t nearC = t0 <= t1 ? n.d.x : n.d.y;
-            uint32_t farC = t0 <= t1 ? n.d.y : n.d.x;
-            stk.put(sp++, farC);
-            cur = nearC;
-        } else if (g0) {
-            cur = n.d.x;
-        } else if (g1) {
-            cur = n.d.y;
-        } else {
-            if (sp == 0) break;
-            cur = stk.get(--sp);
         }
     }
     nodeVisits += nv;
     triTests += nt;
     return found;
+}
+
+// Persistent tracing of a queue with per-lane refill: a lane whose ray is
+// finished takes the next queue entry as soon as at least DR_REFILL_MIN
+// lanes of the wave are idle (one atomic per refill), so a wave is not held
+// by its longest ray. fetch(qi, r, best) sets up queue entry qi;
+// retire(best, found) stores the result. All 64 lanes of the wave must
+// enter; the loop exits when the queue is drained and every lane is idle.
+#ifndef DR_REFILL_MIN
+#define DR_REFILL_MIN 16
+#endif
+template <bool ANY, class Fetch, class Retire>
+__device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t count, uint32_t *claim, DrStack &stk,
+                                               uint32_t &nv, uint32_t &nt, Fetch fetch, Retire retire) {
+    const uint32_t lane = __lane_id();
+    bool active = false, found = false, more = true;
+    DrRayPre r;
+    DrHitRec best;
+    DrTrav s;
+    dr_trav_init(s);
+    s.done = true;
+    while (true) {
+        if (more) {
+            uint64_t idle = __ballot(!active);
+            uint32_t nIdle = __popcll(idle);
+            if (nIdle >= DR_REFILL_MIN) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(claim, nIdle);
+                base = __shfl(base, 0);
+                if (base + nIdle >= count) more = false;
+                if (!active) {
+                    uint32_t qi = base + __popcll(idle & ((1ull << lane) - 1ull));
+                    if (qi < count) {
+                        fetch(qi, r, best);
+                        dr_trav_init(s);
+                        if (S.triangleCount == 0) s.done = true;
+                        found = false;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (!more) break;
+            continue;
+        }
+        while (__any(active && !s.done && s.leafCount == 0u)) {
+            if (active && !s.done && s.leafCount == 0u) dr_node_step(S, r, best.t, s, stk, nv);
+        }
+        while (__any(active && s.leafCount != 0u)) {
+            if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
+                found = true;
+                if (ANY) {
+                    s.done = true;
+                    s.leafCount = 0;
+                }
+            }
+        }
+        if (active && s.done) {
+            retire(best, found);
+            active = false;
+        }
+    }
 }
