@@ -143,8 +143,8 @@ def main():
     sdf_ext, sdf_sh = c[8:18], c[18:28]
     queries_per_step = c[0] + c[1] + c[2]
     log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
-        f"shadow {c[2]:.0f}), "
-        f"node fetches {c[3]:.0f} + shadow {c[5]:.0f}, tri tests {c[4]:.0f} + shadow {c[6]:.0f}")
+        f"shadow {c[2]:.0f}), node fetches {c[3]:.0f} + shadow {c[5]:.0f} + subsurface {c[28]:.0f}, "
+        f"tri tests {c[4]:.0f} + shadow {c[6]:.0f} + subsurface {c[29]:.0f}")
 
     sess = renderer.make_session(gscene, settings, camera=scene.camera, samples_per_launch=1)
     sess.encode_next_sample(args.warmup)
@@ -166,11 +166,13 @@ def main():
 
     # Roofline of the dominant kernel (largest share of stage time).
     st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
-    dom = max(("extend", "shadow", "shade"), key=lambda k: st_ms[k])
+    dom = max(("extend", "shadow", "shade", "sss"), key=lambda k: st_ms[k])
     if dom == "shadow":
         nodes, tris, rays, sdf = c[5], c[6], c[2], sdf_sh
+    elif dom == "sss":  # subsurface-walk queries
+        nodes, tris, rays, sdf = c[28], c[29], c[7], sdf_ext * 0.0
     else:
-        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1], sdf_ext
+        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
     launches_per_step = max(1, st_n[dom] // max(1, args.steps))
     sdf_bytes = CELL_BYTES * sdf[2] + BRICK_BYTES * sdf[4] + STEP_BYTES * sdf[8] + HIT_BYTES * sdf[9]
     bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays + sdf_bytes) / launches_per_step

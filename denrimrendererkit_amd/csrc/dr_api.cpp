@@ -451,9 +451,9 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     size_t oRayO = take(N * 16), oRayD = take(N * 16), oThr = take(N * 16), oRad = take(N * 16), oMeta = take(N * 16),
            oHit = take(N * 16), oHitN = take(N * 16), oNeeT = take(N * 16);
     size_t oShO = take(3 * N * 16), oShD = take(3 * N * 16), oShC = take(3 * N * 16), oShR = take(3 * N * 16);
-    size_t oQ0 = take(N * 4), oQ1 = take(N * 4), oSQ = take(3 * N * 4);
+    size_t oQ0 = take(N * 4), oQ1 = take(N * 4), oSQ = take(3 * N * 4), oSss = take(N * 4), oRes = take(N * 4);
     size_t oCtr = take(DR_CTR_WORDS * 4), oStats = take(DR_CNT_SLOTS * 8);
-    size_t oOvf = take(size_t(stride) * (DR_MAX_DEPTH - DR_LDS_STACK) * 4);
+    size_t oOvf = take(size_t(stride) * (DR_STACK_MAX - DR_LDS_STACK) * 4);
     if (e == hipSuccess) e = hipMalloc(&s->pathBlock, bytes);
     if (e == hipSuccess) e = hipMemset(s->pathBlock, 0, bytes);
     const size_t plane = pixels * 16;
@@ -482,6 +482,8 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     P.queue[0] = reinterpret_cast<uint32_t *>(b + oQ0);
     P.queue[1] = reinterpret_cast<uint32_t *>(b + oQ1);
     P.shadowQueue = reinterpret_cast<uint32_t *>(b + oSQ);
+    P.sssQueue = reinterpret_cast<uint32_t *>(b + oSss);
+    P.resumeQueue = reinterpret_cast<uint32_t *>(b + oRes);
     P.counters = reinterpret_cast<uint32_t *>(b + oCtr);
     P.stats = reinterpret_cast<unsigned long long *>(b + oStats);
     P.overflow = reinterpret_cast<uint32_t *>(b + oOvf);
@@ -560,6 +562,8 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
     for (uint32_t b = 0; b < L.maxBounces; ++b) {
         DR_HIP(run(DR_STAGE_EXTEND, b), "extend launch");
         DR_HIP(run(DR_STAGE_SHADE, b), "shade launch");
+        DR_HIP(run(DR_STAGE_SSS, b), "subsurface launch");
+        DR_HIP(run(DR_STAGE_RESUME, b), "shade resume launch");
         DR_HIP(run(DR_STAGE_SHADOW, b), "shadow launch");
         DR_HIP(run(DR_STAGE_RESOLVE, b), "resolve launch");
     }
@@ -585,6 +589,8 @@ static void collectStageTimes(dr_session *s) {
         for (uint32_t b = 0; b < mb; ++b) {
             add(DR_STAGE_EXTEND);
             add(DR_STAGE_SHADE);
+            add(DR_STAGE_SSS);
+            add(DR_STAGE_RESUME);
             add(DR_STAGE_SHADOW);
             add(DR_STAGE_RESOLVE);
         }

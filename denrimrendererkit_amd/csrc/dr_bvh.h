@@ -31,4 +31,7 @@ void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vec
 }  // namespace dr
 
 #define DR_MAX_DEPTH 64
+// Traversal stack capacity: one entry per inner level plus two leaf
+// references pushed at the last level.
+#define DR_STACK_MAX (DR_MAX_DEPTH + 2)
 #define DR_EMPTY_CHILD 0xffffffffu

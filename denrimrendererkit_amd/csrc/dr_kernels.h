@@ -10,7 +10,7 @@
 #define DR_SHADE_BLOCK 128
 #define DR_MAX_BOUNCES 64
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
-// per-lane global area of (DR_MAX_DEPTH - DR_LDS_STACK) words.
+// per-lane global area of (DR_STACK_MAX - DR_LDS_STACK) words.
 #define DR_LDS_STACK 24
 
 // Device counter words: queue lengths per bounce (index b = queue consumed
@@ -21,7 +21,12 @@
 #define DR_CTR_EXTEND (DR_CTR_QUEUE + DR_MAX_BOUNCES + 1)
 #define DR_CTR_SHADOWQ (DR_CTR_EXTEND + DR_MAX_BOUNCES)
 #define DR_CTR_SHADOW (DR_CTR_SHADOWQ + DR_MAX_BOUNCES)
-#define DR_CTR_WORDS (DR_CTR_SHADOW + DR_MAX_BOUNCES)
+// Subsurface-walk queue lengths and claim cursors per bounce, and the queue
+// of paths whose walk failed and resume surface shading.
+#define DR_CTR_SSSQ (DR_CTR_SHADOW + DR_MAX_BOUNCES)
+#define DR_CTR_SSS (DR_CTR_SSSQ + DR_MAX_BOUNCES)
+#define DR_CTR_RESUMEQ (DR_CTR_SSS + DR_MAX_BOUNCES)
+#define DR_CTR_WORDS (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
 
 struct DrPaths {
     float4 *rayO;   // origin.xyz, previousBSDFPDF
@@ -38,6 +43,8 @@ struct DrPaths {
     float4 *shR;    // 3 per slot: transmittance.xyz, visible
     uint32_t *queue[2];
     uint32_t *shadowQueue;
+    uint32_t *sssQueue;     // paths entering a subsurface walk this bounce
+    uint32_t *resumeQueue;  // paths whose walk failed: surface shading resumes
     uint32_t *counters;  // DR_CTR_WORDS
     unsigned long long *stats;  // DR_CNT_SLOTS, accumulated over launches
     uint32_t *overflow;  // traversal stack spill, overflowStride lanes
@@ -56,12 +63,15 @@ enum DrStage {
     DR_STAGE_SHADOW = 3,
     DR_STAGE_RESOLVE = 4,
     DR_STAGE_FINALIZE = 5,
-    DR_STAGE_COUNT = 6
+    DR_STAGE_SSS = 6,     // subsurface random walks of the bounce
+    DR_STAGE_RESUME = 7,  // surface shading after a failed walk
+    DR_STAGE_COUNT = 8
 };
 
 namespace dr {
 // Enqueues one stage of a sample batch on `st`. One batch is RAYGEN, then
-// for b in [0, maxBounces): EXTEND, SHADE, SHADOW, RESOLVE; then FINALIZE.
+// for b in [0, maxBounces): EXTEND, SHADE, SSS, RESUME, SHADOW, RESOLVE;
+// then FINALIZE.
 hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
                        uint32_t traceGrid, hipStream_t st);

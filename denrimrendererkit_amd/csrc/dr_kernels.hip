@@ -7,10 +7,15 @@
 //   extend   -- closest-hit traversal for every live path (persistent,
 //               wave-granular work stealing)
 //   shade    -- one bounce of the reference loop body (:5279-5517): miss and
-//               emission terms, pass-through, volume / subsurface / dielectric
+//               emission terms, pass-through, volume / dielectric
 //               continuation, NEE sample generation, lobe sampling, RR. Live
 //               paths are appended to the next queue with a wave ballot and
 //               one atomic per wave; NEE shadow rays go to a shadow queue.
+//               Paths that pass the subsurface gate go to a walk queue.
+//   sss      -- the subsurface random walks of the bounce (:3525-3621), all
+//               lanes of a wave walking, so their nested traversals run
+//               converged; a failed walk sends the path to
+//   resume   -- shade again from the dielectric step on (:5365-5517).
 //   shadow   -- occlusion / transmittance of each NEE ray (any-hit when no
 //               material can pass light, else the reference's 8-step chain)
 //   resolve  -- sums a path's NEE terms in the reference order and adds the
@@ -153,7 +158,10 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths
                 r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
                 best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
             },
-            [&](const DrHitRec &h, bool) { P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id)); });
+            [&](const DrHitRec &h, bool, DrRayPre &, DrHitRec &) {
+                P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+                return false;
+            });
     } else {
         while (true) {
             uint32_t base = 0;
@@ -317,6 +325,12 @@ __device__ __forceinline__ float boundaryPDF(vec3 sigmaT, float d) {
 }
 
 
+// The walk returns false before drawing anything unless the surface is
+// entered from the front (:3534-3537); only then is it handed to k_sss.
+__device__ __forceinline__ bool subsurfaceWalkStarts(const Mat &m, const Surf &entry) {
+    return materialSubsurface(m) > 0.0f && entry.frontFacing;
+}
+
 template <bool VOL>
 __device__ __forceinline__ bool continueSubsurfaceRandomWalk(const DrSceneDev &S, const Surf &entry, const Mat &m, vec3 &ro,
                                                           vec3 &rd, vec3 &T, float &prevPdf, uint32_t &state,
@@ -457,28 +471,43 @@ __device__ __forceinline__ bool continueTransmissiveSurface(const Surf &h, const
 __device__ __forceinline__ void writeAOV(float4 *plane, uint32_t pix, float4 v) { plane[pix] = v; }
 
 // ----------------------------------------------------------------- shade --
-template <bool VOL>
+// The ray, throughput and prevPdf a path carries into its subsurface walk
+// are stashed in two of its shadow records (free until NEE of the bounce),
+// since the volume step before the gate may have moved them while the hit
+// surface is rebuilt from the ray in rayO / rayD.
+__device__ __forceinline__ void stashWalkRay(const DrPaths &P, uint32_t slot, vec3 ro, vec3 rd, float prevPdf) {
+    P.shR[slot * 3u] = make_float4(ro.x, ro.y, ro.z, prevPdf);
+    P.shR[slot * 3u + 1u] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+}
+
+// RESUME = false: a bounce from the hit in P.hit. RESUME = true: paths from
+// resumeQueue continue at continueTransmissiveSurface with the throughput
+// and RNG state their failed walk left.
+template <bool VOL, bool RESUME>
 __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
                                                          DrPaths P, DrOutputs O, uint32_t bounce) {
-    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
-    const uint32_t *queue = P.queue[bounce & 1u];
+    const uint32_t *queue = RESUME ? P.resumeQueue : P.queue[bounce & 1u];
     uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
-    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    const uint32_t count = P.counters[(RESUME ? DR_CTR_RESUMEQ : DR_CTR_QUEUE) + bounce];
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= count) return;  // whole waves past the end leave together
-    uint32_t gl = qi;
-    DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + gl, P.overflowStride};
-    uint32_t cPrimary = 0u, cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
-    DrSdfCounters sc{};
+    uint32_t cPrimary = 0u, cBounce = 0u;
 
     const uint32_t slot = queue[qi];
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
     uint4 meta = P.meta[slot];
     float4 h4 = P.hit[slot];
     vec3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
+    const vec3 hitO = ro, hitD = rd;  // the ray of the hit in P.hit
     vec3 T = v3(t4.x, t4.y, t4.z), radiance = v3(l4.x, l4.y, l4.z);
     float outputAlpha = t4.w;
     float prevPdf = o4.w;
+    if (RESUME) {
+        float4 a = P.shR[slot * 3u], b = P.shR[slot * 3u + 1u];
+        ro = v3(a.x, a.y, a.z);
+        prevPdf = a.w;
+        rd = v3(b.x, b.y, b.z);
+    }
     uint32_t state = meta.z;
     uint32_t flags = meta.w & DR_FLAG_PRIMARY;
     const bool hasPrimary = (flags & DR_FLAG_PRIMARY) != 0u;
@@ -486,8 +515,10 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     const uint32_t pix = meta.x;
     const float clampValue = L.sampleRadianceClamp;
     bool alive = false;
-    if (bounce == 0u) cPrimary++;
-    else cBounce++;
+    if (!RESUME) {
+        if (bounce == 0u) cPrimary++;
+        else cBounce++;
+    }
 
     // NEE shadow records are written straight to the slot's three records;
     // neeMask holds which of them are valid.
@@ -510,11 +541,11 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     } else {
         float4 hn4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         if (VOL && (id & DR_VOL_HIT)) hn4 = P.hitN[slot];
-        Surf h = surfaceAt<VOL>(S, ro, rd, h4, hn4);
+        Surf h = surfaceAt<VOL>(S, hitO, hitD, h4, hn4);
         Mat m = materialFor(S, h);
-        applyNormalMap(S, h, m, rd);
+        applyNormalMap(S, h, m, hitD);
         applyBaseColorTexture(S, m, h);
-        if (m.base.w <= 0.001f) {
+        if (!RESUME && m.base.w <= 0.001f) {
             // Pass-through: consumes this bounce (:5318-5321).
             ro = h.position + rd * 0.001f;
             alive = true;
@@ -542,18 +573,28 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                 }
             }
             bool handled = false;
-            if (continueTransmissionVolumeRandomWalk(h, m, ro, rd, T, prevPdf, state)) {
-                handled = true;
-                alive = !terminatePathWithRussianRoulette(bounce, T, state);
-            }
-            if (!handled) {
-                float subsurface = materialSubsurface(m);
-                if (subsurface > 0.0f) {
-                    float gate = randomFloat(state);
-                    if (gate < subsurface &&
-                        continueSubsurfaceRandomWalk<VOL>(S, h, m, ro, rd, T, prevPdf, state, stk, cBounce, cNodes, cTris, cSss, sc)) {
-                        handled = true;
-                        alive = !terminatePathWithRussianRoulette(bounce, T, state);
+            if (!RESUME) {
+                if (continueTransmissionVolumeRandomWalk(h, m, ro, rd, T, prevPdf, state)) {
+                    handled = true;
+                    alive = !terminatePathWithRussianRoulette(bounce, T, state);
+                }
+                if (!handled) {
+                    float subsurface = materialSubsurface(m);
+                    if (subsurface > 0.0f) {
+                        float gate = randomFloat(state);
+                        if (gate < subsurface && subsurfaceWalkStarts(m, h)) {
+                            // The walk runs in k_sss; hand the path over.
+                            stashWalkRay(P, slot, ro, rd, prevPdf);
+                            P.thr[slot] = make_float4(T.x, T.y, T.z, outputAlpha);
+                            P.meta[slot] = make_uint4(meta.x, meta.y, state, flags);
+                            uint32_t pos = waveAppend(true, &P.counters[DR_CTR_SSSQ + bounce]);
+                            P.sssQueue[pos] = slot;
+                            if (L.statsEnabled) {
+                                waveAddStat(P.stats, DR_CNT_PRIMARY, cPrimary);
+                                waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
+                            }
+                            return;
+                        }
                     }
                 }
             }
@@ -712,8 +753,160 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     if (L.statsEnabled) {
         waveAddStat(P.stats, DR_CNT_PRIMARY, cPrimary);
         waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
-        waveAddStat(P.stats, DR_CNT_NODES, cNodes);
-        waveAddStat(P.stats, DR_CNT_TRIS, cTris);
+    }
+}
+
+// ------------------------------------------------------------------- sss --
+// continueSubsurfaceRandomWalk for the paths k_shade queued this bounce.
+// Persistent waves take 64 paths at a time; every lane of a wave walks, so
+// the nested closest-hit queries run converged. A successful walk exits the
+// surface (then RR, :5359-5363); a failed one goes to k_shade<RESUME> with
+// its throughput and RNG state.
+template <bool VOL>
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L, DrPaths P, uint32_t bounce) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
+    const uint32_t count = P.counters[DR_CTR_SSSQ + bounce];
+    uint32_t *claim = &P.counters[DR_CTR_SSS + bounce];
+    uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
+    uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    uint32_t cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
+    DrSdfCounters sc{};
+    if constexpr (!VOL) {
+        // continueSubsurfaceRandomWalk (PathTrace.metal:3525-3621) as a
+        // chain of closest-hit queries, one per walk segment.
+        uint32_t slot = 0, state = 0, step = 0;
+        vec3 mo, md, T, sigmaT, albedo;
+        float g = 0.0f, alpha = 0.0f;
+        uint4 meta;
+        auto finish = [&](bool exited, vec3 ro, vec3 rd, float prevPdf) {
+            bool alive = false;
+            if (exited) {
+                alive = !terminatePathWithRussianRoulette(bounce, T, state) && (bounce + 1u < L.maxBounces);
+                P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
+                P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+            }
+            P.thr[slot] = make_float4(T.x, T.y, T.z, alpha);
+            P.meta[slot] = make_uint4(meta.x, meta.y, state, meta.w);
+            uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+            if (alive) queueOut[pos] = slot;
+            bool resume = !exited;
+            uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
+            if (resume) P.resumeQueue[rpos] = slot;
+        };
+        dr_trace_queue<false>(
+            S, count, claim, stk, cNodes, cTris,
+            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+                slot = P.sssQueue[qi];
+                float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
+                meta = P.meta[slot];
+                float4 h4 = P.hit[slot];
+                vec3 hitO = v3(o4.x, o4.y, o4.z), hitD = v3(d4.x, d4.y, d4.z);
+                Surf h = surfaceAt<false>(S, hitO, hitD, h4, h4);
+                Mat m = materialFor(S, h);
+                applyNormalMap(S, h, m, hitD);
+                applyBaseColorTexture(S, m, h);
+                T = v3(t4.x, t4.y, t4.z);
+                alpha = t4.w;
+                state = meta.z;
+                // Entry (:3534-3548): cosine-weighted into the surface.
+                float e0 = randomFloat(state);
+                float e1 = randomFloat(state);
+                vec3 entryDir = orientHemisphere(cosineHemisphere(e0, e1), -h.normal);
+                mo = h.position + entryDir * 0.001f;
+                md = entryDir;
+                sigmaT = v3s(1.0f) / materialSubsurfaceRadius(m);
+                albedo = materialSubsurfaceColor(m);
+                g = materialSubsurfaceAnisotropy(m);
+                step = 0;
+                cBounce++;
+                cSss++;
+                r = dr_prep(mo, md);
+                best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+            },
+            [&](const DrHitRec &hr, bool hit, DrRayPre &r, DrHitRec &best) {
+                if (!hit) {
+                    finish(false, mo, md, 0.0f);
+                    return false;
+                }
+                float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
+                Surf b = surfaceAt<false>(S, mo, md, h4, h4);
+                float f0 = randomFloat(state);
+                float f1 = randomFloat(state);
+                float ff = sampleFreeFlight(sigmaT, f0, f1);
+                if (ff < b.t) {
+                    mo += md * ff;
+                    float h0 = randomFloat(state);
+                    float h1 = randomFloat(state);
+                    md = sampleHenyeyGreenstein(h0, h1, md, g);
+                    vec3 tr = vexp((-sigmaT) * ff);
+                    T *= albedo * sigmaT * tr / collisionPDF(sigmaT, ff);
+                    if (maxComponent(T) <= 1e-4f || ++step >= 8u) {
+                        finish(false, mo, md, 0.0f);
+                        return false;
+                    }
+                    cBounce++;
+                    cSss++;
+                    r = dr_prep(mo, md);
+                    best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+                    return true;
+                }
+                vec3 exitN = normalize(b.frontFacing ? b.normal : -b.normal);
+                float x0 = randomFloat(state);
+                float x1 = randomFloat(state);
+                vec3 rd = orientHemisphere(cosineHemisphere(x0, x1), exitN);
+                vec3 ro = b.position + exitN * 0.001f;
+                vec3 tr = vexp((-sigmaT) * b.t);
+                T *= tr / boundaryPDF(sigmaT, b.t);
+                finish(true, ro, rd, fmaxf(dot(exitN, rd), 0.0f) * 0.31830988618f);
+                return false;
+            });
+    }
+    while (VOL) {
+        uint32_t base = 0;
+        if (laneId() == 0) base = atomicAdd(claim, 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        uint32_t qi = base + laneId();
+        bool have = qi < count;
+        uint32_t slot = have ? P.sssQueue[qi] : 0u;
+        bool alive = false, resume = false;
+        if (have) {
+            float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
+            uint4 meta = P.meta[slot];
+            float4 h4 = P.hit[slot];
+            vec3 hitO = v3(o4.x, o4.y, o4.z), hitD = v3(d4.x, d4.y, d4.z);
+            float4 a = P.shR[slot * 3u], b = P.shR[slot * 3u + 1u];
+            vec3 ro = v3(a.x, a.y, a.z), rd = v3(b.x, b.y, b.z);
+            float prevPdf = a.w;
+            vec3 T = v3(t4.x, t4.y, t4.z);
+            uint32_t state = meta.z;
+            float4 hn4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+            if (VOL && (__float_as_uint(h4.w) & DR_VOL_HIT)) hn4 = P.hitN[slot];
+            Surf h = surfaceAt<VOL>(S, hitO, hitD, h4, hn4);
+            Mat m = materialFor(S, h);
+            applyNormalMap(S, h, m, hitD);
+            applyBaseColorTexture(S, m, h);
+            if (continueSubsurfaceRandomWalk<VOL>(S, h, m, ro, rd, T, prevPdf, state, stk, cBounce, cNodes, cTris,
+                                                  cSss, sc)) {
+                alive = !terminatePathWithRussianRoulette(bounce, T, state) && (bounce + 1u < L.maxBounces);
+                P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
+                P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+            } else {
+                resume = true;
+            }
+            P.thr[slot] = make_float4(T.x, T.y, T.z, t4.w);
+            P.meta[slot] = make_uint4(meta.x, meta.y, state, meta.w);
+        }
+        uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+        if (alive) queueOut[pos] = slot;
+        uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
+        if (resume) P.resumeQueue[rpos] = slot;
+    }
+    if (L.statsEnabled) {
+        waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
+        waveAddStat(P.stats, DR_CNT_SSS_NODES, cNodes);
+        waveAddStat(P.stats, DR_CNT_SSS_TRIS, cTris);
         waveAddStat(P.stats, DR_CNT_SSS, cSss);
         if constexpr (VOL) addSdfStats(P.stats, sc);
     }
@@ -744,11 +937,78 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
                 best = DrHitRec{o4.w, 0.0f, 0.0f, 0u};
                 ++nq;
             },
-            [&](const DrHitRec &, bool occluded) {
+            [&](const DrHitRec &, bool occluded, DrRayPre &, DrHitRec &) {
                 P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+                return false;
+            });
+    } else if (!VOL) {
+        // The chain of closest hits, one query per step; a lane continues
+        // its chain without waiting for the wave.
+        uint32_t k = 0, step = 0;
+        vec3 o, d, tr;
+        float maxDistance = 0.0f, traveled = 0.0f;
+        // Hits at or beyond maxDistance - traveled end the chain, so each
+        // query is limited there: exactly (exclusive limit) on the first
+        // step, with a relative margin afterwards where the reference test
+        // traveled + t >= maxDistance decides.
+        auto limitFor = [&](uint32_t st) {
+            return st == 0u ? DrHitRec{maxDistance, 0.0f, 0.0f, 0u}
+                            : DrHitRec{(maxDistance - traveled) * 1.000001f + 1e-30f, 0.0f, 0.0f, DR_MISS};
+        };
+        dr_trace_queue<false>(
+            S, count, steal, stk, nv, nt,
+            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+                k = P.shadowQueue[qi];
+                float4 o4 = P.shO[k], d4 = P.shD[k];
+                o = v3(o4.x, o4.y, o4.z);
+                d = v3(d4.x, d4.y, d4.z);
+                maxDistance = o4.w;
+                traveled = 0.0f;
+                tr = v3s(1.0f);
+                step = 0;
+                r = dr_prep(o, d);
+                best = limitFor(0);
+                ++nq;
+            },
+            [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
+                bool occluded = true, next = false;
+                if (!hit || traveled + h.t >= maxDistance) {
+                    occluded = false;
+                } else {
+                    float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+                    Surf sh = surfaceAt<false>(S, o, d, h4, h4);
+                    Mat m = materialFor(S, sh);
+                    if (m.base.w <= 0.001f) {
+                        traveled += sh.t;
+                        o = sh.position + d * 0.001f;
+                        next = true;
+                    } else {
+                        float transmission = materialTransmission(m);
+                        if (transmission > 0.001f) {
+                            bool inside = !sh.frontFacing && !materialThinWalled(m);
+                            vec3 medium = materialHasVolumeScattering(m) && inside
+                                              ? materialVolumeTransmittance(m, sh.t)
+                                              : materialTransmissionAbsorption(m, sh.t, inside);
+                            tr = tr * materialTransmissionTint(m, 0.35f) * medium * transmission;
+                            if (maxComponent(tr) >= 0.02f) {
+                                traveled += sh.t;
+                                o = sh.position + d * 0.001f;
+                                next = true;
+                            }
+                        }
+                    }
+                }
+                if (next && ++step < 8u) {
+                    r = dr_prep(o, d);
+                    best = limitFor(step);
+                    ++nq;
+                    return true;
+                }
+                P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
+                return false;
             });
     }
-    while (VOL || !S.opaqueOnly) {
+    while (VOL) {
         uint32_t base = 0;
         if (laneId() == 0) base = atomicAdd(steal, 64u);
         base = __shfl(base, 0);
@@ -950,12 +1210,24 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
             hipLaunchKernelGGL(k_extend<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         break;
     case DR_STAGE_SHADE:
-        if (vol)
-            hipLaunchKernelGGL(k_shade<true>, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK), 0,
-                               st, S, L, cam, prevCam, P, O, bounce);
+    case DR_STAGE_RESUME: {
+        const dim3 grid((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), block(DR_SHADE_BLOCK);
+        const bool resume = stage == DR_STAGE_RESUME;
+        if (vol && resume)
+            hipLaunchKernelGGL((k_shade<true, true>), grid, block, 0, st, S, L, cam, prevCam, P, O, bounce);
+        else if (vol)
+            hipLaunchKernelGGL((k_shade<true, false>), grid, block, 0, st, S, L, cam, prevCam, P, O, bounce);
+        else if (resume)
+            hipLaunchKernelGGL((k_shade<false, true>), grid, block, 0, st, S, L, cam, prevCam, P, O, bounce);
         else
-            hipLaunchKernelGGL(k_shade<false>, dim3((nSlots + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), dim3(DR_SHADE_BLOCK),
-                               0, st, S, L, cam, prevCam, P, O, bounce);
+            hipLaunchKernelGGL((k_shade<false, false>), grid, block, 0, st, S, L, cam, prevCam, P, O, bounce);
+        break;
+    }
+    case DR_STAGE_SSS:
+        if (vol)
+            hipLaunchKernelGGL(k_sss<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
+        else
+            hipLaunchKernelGGL(k_sss<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
         break;
     case DR_STAGE_SHADOW:
         if (vol)

@@ -154,5 +154,9 @@ enum {
     // at DR_CNT_SDF_SHADOW_BASE. The reference's totals are the sums.
     DR_CNT_SDF_BASE = 8,
     DR_CNT_SDF_SHADOW_BASE = 18,
-    DR_CNT_SLOTS = 28
+    // Node fetches and triangle tests of subsurface-walk queries (k_sss);
+    // NODES / TRIS count extension and AOV queries.
+    DR_CNT_SSS_NODES = 28,
+    DR_CNT_SSS_TRIS = 29,
+    DR_CNT_SLOTS = 30
 };

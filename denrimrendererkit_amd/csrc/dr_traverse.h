@@ -136,7 +136,8 @@ __device__ __forceinline__ void dr_trav_init(DrTrav &s) {
     s.done = false;
 }
 
-// One inner-node step; requires !s.done && s.leafCount == 0.
+// One inner-node step; requires !s.done and either no pending leaf, or a
+// known next node (speculative step: leaves found then go to the stack).
 __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre &r, float tmax, DrTrav &s,
                                              DrStack &stk, uint32_t &nv) {
     if (s.cur == DR_POP) {
@@ -164,7 +165,11 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
     bool l0 = h0 && n.d.z != 0u, l1 = h1 && n.d.w != 0u;
     bool i0 = h0 && n.d.z == 0u, i1 = h1 && n.d.w == 0u;
     bool near0 = t0 <= t1;
-    if (l0 && l1) {
+    if (s.leafCount != 0u) {
+        // Speculative: keep the pending leaf, stack the new ones (nearer on top).
+        if (l0 && l1) stk.put(s.sp++, DR_LEAF_REF | ni << 1 | (near0 ? 1u : 0u));
+        if (l0 || l1) stk.put(s.sp++, DR_LEAF_REF | ni << 1 | ((l0 && l1) ? (near0 ? 0u : 1u) : (l0 ? 0u : 1u)));
+    } else if (l0 && l1) {
         s.leafFirst = near0 ? n.d.x : n.d.y;
         s.leafCount = near0 ? n.d.z : n.d.w;
         stk.put(s.sp++, DR_LEAF_REF | ni << 1 | (near0 ? 1u : 0u));
@@ -226,8 +231,10 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
     uint32_t nv = 0, nt = 0;
     bool found = false;
     while (true) {
+        // Node phase: until every lane holds a leaf; lanes that already hold
+        // one keep descending (speculatively) while they know a next node.
         while (__any(!s.done && s.leafCount == 0)) {
-            if (!s.done && s.leafCount == 0) dr_node_step(S, r, best.t, s, stk, nv);
+            if (!s.done && (s.leafCount == 0 || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
         }
         if (!__any(s.leafCount != 0u)) break;
         while (__any(s.leafCount != 0u)) {
@@ -245,12 +252,15 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
     return found;
 }
 
-// Persistent tracing of a queue with per-lane refill: a lane whose ray is
-// finished takes the next queue entry as soon as at least DR_REFILL_MIN
-// lanes of the wave are idle (one atomic per refill), so a wave is not held
-// by its longest ray. fetch(qi, r, best) sets up queue entry qi;
-// retire(best, found) stores the result. All 64 lanes of the wave must
-// enter; the loop exits when the queue is drained and every lane is idle.
+// Persistent tracing of a queue with per-lane refill: a lane whose work
+// item is finished takes the next queue entry as soon as at least
+// DR_REFILL_MIN lanes of the wave are idle (one atomic per refill), so a
+// wave is not held by its longest item. fetch(qi, r, best) sets up the
+// first query of queue entry qi; retire(res, found, r, best) consumes a
+// query's result and either sets up the item's next query in r / best and
+// returns true (a shadow chain step, a subsurface-walk segment), or
+// returns false when the item is done. All 64 lanes of the wave must enter;
+// the loop exits when the queue is drained and every lane is idle.
 #ifndef DR_REFILL_MIN
 #define DR_REFILL_MIN 16
 #endif
@@ -258,6 +268,12 @@ template <bool ANY, class Fetch, class Retire>
 __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t count, uint32_t *claim, DrStack &stk,
                                                uint32_t &nv, uint32_t &nt, Fetch fetch, Retire retire) {
     const uint32_t lane = __lane_id();
+    // A wave claims at most its fair share of a short queue, so that short
+    // queues (subsurface walks, late bounces) spread over every CU instead
+    // of filling a few waves whose chains of dependent fetches then set the
+    // kernel's duration.
+    const uint32_t waves = gridDim.x * ((blockDim.x + 63u) >> 6);
+    const uint32_t share = max(4u, (count + waves - 1u) / waves);
     bool active = false, found = false, more = true;
     DrRayPre r;
     DrHitRec best;
@@ -268,14 +284,16 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
         if (more) {
             uint64_t idle = __ballot(!active);
             uint32_t nIdle = __popcll(idle);
-            if (nIdle >= DR_REFILL_MIN) {
+            if (nIdle >= min(DR_REFILL_MIN, share)) {
+                uint32_t take = min(nIdle, share);
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(claim, nIdle);
+                if (lane == 0) base = atomicAdd(claim, take);
                 base = __shfl(base, 0);
-                if (base + nIdle >= count) more = false;
+                if (base + take >= count) more = false;
                 if (!active) {
-                    uint32_t qi = base + __popcll(idle & ((1ull << lane) - 1ull));
-                    if (qi < count) {
+                    uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+                    uint32_t qi = base + rank;
+                    if (rank < take && qi < count) {
                         fetch(qi, r, best);
                         dr_trav_init(s);
                         if (S.triangleCount == 0) s.done = true;
@@ -290,7 +308,7 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
             continue;
         }
         while (__any(active && !s.done && s.leafCount == 0u)) {
-            if (active && !s.done && s.leafCount == 0u) dr_node_step(S, r, best.t, s, stk, nv);
+            if (active && !s.done && (s.leafCount == 0u || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
         }
         while (__any(active && s.leafCount != 0u)) {
             if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
@@ -302,8 +320,14 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
             }
         }
         if (active && s.done) {
-            retire(best, found);
-            active = false;
+            const DrHitRec res = best;
+            if (retire(res, found, r, best)) {
+                dr_trav_init(s);
+                if (S.triangleCount == 0) s.done = true;
+                found = false;
+            } else {
+                active = false;
+            }
         }
     }
 }

@@ -31,8 +31,8 @@ from . import _lib
 from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, compile_scene
 from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
 
-STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize")
-N_STATS = 28
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize", "sss", "shade_resume")
+N_STATS = 30
 
 
 @dataclass(frozen=True)
@@ -249,8 +249,8 @@ class RenderSession:
 
     def timing(self) -> dict:
         total = C.c_double()
-        ms = (C.c_double * 6)()
-        n = (C.c_uint64 * 6)()
-        _lib.check(_lib.load().dr_hip_session_timing(self._h, C.byref(total), ms, n, 6), "timing")
+        ms = (C.c_double * len(STAGES))()
+        n = (C.c_uint64 * len(STAGES))()
+        _lib.check(_lib.load().dr_hip_session_timing(self._h, C.byref(total), ms, n, len(STAGES)), "timing")
         return {"total_ms": total.value, "stage_ms": dict(zip(STAGES, list(ms))),
                 "stage_launches": dict(zip(STAGES, [int(x) for x in n]))}

@@ -125,7 +125,7 @@ typedef enum {
     DR_STAT_PRIMARY_QUERIES = 0,
     DR_STAT_BOUNCE_QUERIES = 1,
     DR_STAT_SHADOW_QUERIES = 2,
-    DR_STAT_NODE_VISITS = 3,          /* extension + subsurface queries: node fetches (2 boxes each) */
+    DR_STAT_NODE_VISITS = 3,          /* extension queries: node fetches (2 boxes each) */
     DR_STAT_TRIANGLE_TESTS = 4,
     DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
     DR_STAT_SHADOW_TRIANGLE_TESTS = 6,
@@ -146,7 +146,9 @@ typedef enum {
     /* 18-27: the same ten counters for NEE shadow queries only; the
      * reference's counters are the sums of both groups. */
     DR_STAT_SHADOW_SDF_BASE = 18,
-    DR_STAT_COUNT = 28
+    DR_STAT_SUBSURFACE_NODE_VISITS = 28, /* subsurface-walk queries */
+    DR_STAT_SUBSURFACE_TRIANGLE_TESTS = 29,
+    DR_STAT_COUNT = 30
 } dr_stat;
 
 /* SDF primitive for the volume builders (SDFPrimitive / CompiledPrimitive,
@@ -247,8 +249,8 @@ dr_status dr_hip_session_reset_stats(dr_session *session);
 
 /* GPU time of the session's work, measured with HIP events on the session
  * stream: `total_ms` of the last render/encode call; per-stage sums since the
- * last reset_stats (raygen, extend, shade, shadow, resolve, finalize) when
- * stage timing is enabled. */
+ * last reset_stats (raygen, extend, shade, shadow, resolve, finalize,
+ * subsurface walk, shade resume) when stage timing is enabled. */
 dr_status dr_hip_session_set_stage_timing(dr_session *session, int enabled);
 dr_status dr_hip_session_timing(dr_session *session, double *total_ms, double *stage_ms, uint64_t *stage_launches,
                                 size_t stages);
