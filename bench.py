@@ -41,6 +41,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload_name, kernel):
+    """HBM bytes per dispatch of `kernel` on this workload from the committed
+    rocprofv3 --pmc summary (profiles/traffic.json): FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, each scaled by the factor the same counter
+    shows on a plain device copy of known size. None when not profiled."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["workloads"][workload_name]["kernels"][kernel]
+        cal = t["copy_calibration"]
+        return int((k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]) * 1024)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def workload(name):
     from denrimrendererkit_amd import synthetic
     if name == "terrain_1080p":
@@ -59,6 +75,113 @@ def workload(name):
             "workload": "room 320x180, 8,334 tris, Lambertian + GGX metal, 1 area light, max depth 4",
             "width": 320, "height": 180, "max_bounces": 4, "triangles": 8334}
     raise SystemExit(f"unknown workload {name}")
+
+
+SERVE_RES = ((96, 96), (256, 256), (512, 512), (1280, 720))
+SERVE_RES_P = (0.5, 0.25, 0.15, 0.1)
+SERVE_SPP = (16, 64, 256)
+
+
+def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load=1.25):
+    """The material-preview request stream: sphere (20k tris) on a plane
+    with seeded layered materials, 1024x512 HDRI; resolution from {96^2,
+    256^2, 512^2, 1280x720} w.p. {.5, .25, .15, .1}; spp from {16, 64, 256};
+    Poisson arrivals at `load` x the one-at-a-time service rate (from a
+    pilot). Scenes are resident in HBM before timing. The same arrival
+    schedule is served (a) one request at a time on a plain session and (b)
+    by the render server (continuous batching, graph replays). Both render
+    the same samples, so their images are identical (equal relMSE)."""
+    from denrimrendererkit_amd import RenderQuality, RenderSettings, compile_scene, synthetic
+    from denrimrendererkit_amd.renderer import RenderServer
+    rng = np.random.default_rng(0x5EED0004 + seed)
+    scenes = [synthetic.material_preview_scene(k) for k in range(n_scenes)]
+    gpu = [renderer.upload(compile_scene(sc)) for sc in scenes]
+    reqs = []
+    for _ in range(n_requests):
+        W, H = SERVE_RES[rng.choice(4, p=SERVE_RES_P)]
+        reqs.append((int(rng.integers(n_scenes)), W, H, int(SERVE_SPP[rng.integers(3)])))
+
+    def settings(W, H):
+        return RenderSettings(width=W, height=H, max_bounces=max_bounces, quality=RenderQuality.FINAL, frame_seed=0)
+
+    def plain(i, W, H, spp):
+        sess = renderer.make_session(gpu[i], settings(W, H), camera=scenes[i].camera)
+        sess.render(spp)
+        sess.pixels()
+        sess.close()
+
+    pilot = [reqs[k] for k in range(min(6, n_requests))]
+    t0 = time.perf_counter()
+    for r in pilot:
+        plain(*r)
+    service = (time.perf_counter() - t0) / len(pilot)
+    rate = load / max(service, 1e-6)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    # (a) one at a time, in arrival order
+    lat_plain = []
+    t0 = time.perf_counter()
+    for r, a in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        plain(*r)
+        lat_plain.append(time.perf_counter() - t0 - a)
+    span_plain = time.perf_counter() - t0
+
+    # (b) served: submit at the arrival times; a collector thread reads each
+    # finished request's beauty plane and releases it (as plain does).
+    import threading
+    server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=16)
+    outstanding, lat_served, errors = {}, [], []
+    lock = threading.Lock()
+    submitted = threading.Event()
+
+    def collect():
+        while True:
+            with lock:
+                items = list(outstanding.items())
+            if not items and submitted.is_set():
+                return
+            for t, a in items:
+                st = server.state(t)
+                if st == "failed":
+                    errors.append(t)
+                if st in ("done", "failed"):
+                    if st == "done":
+                        server.result(t)
+                    server.release(t)
+                    lat_served.append(time.perf_counter() - t0 - a)
+                    with lock:
+                        del outstanding[t]
+            time.sleep(2e-4)
+
+    t0 = time.perf_counter()
+    collector = threading.Thread(target=collect)
+    collector.start()
+    for (i, W, H, spp), a in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        with lock:
+            outstanding[server.submit(gpu[i], settings(W, H), scenes[i].camera, spp)] = a
+    submitted.set()
+    collector.join()
+    span_served = time.perf_counter() - t0
+    server.close()
+    if errors:
+        raise RuntimeError("served requests failed: %s" % errors)
+
+    def pct(x, q):
+        return round(float(np.percentile(np.asarray(x) * 1e3, q)), 2)
+    return {"requests": n_requests, "arrival_rate_per_s": round(rate, 3), "load_vs_plain": load,
+            "images_per_s": round(n_requests / span_served, 3), "p50_ms": pct(lat_served, 50),
+            "p99_ms": pct(lat_served, 99),
+            "plain": {"images_per_s": round(n_requests / span_plain, 3), "p50_ms": pct(lat_plain, 50),
+                      "p99_ms": pct(lat_plain, 99)},
+            "speedup_images_per_s": round(span_plain / span_served, 3),
+            "mix": "res {96^2:.5, 256^2:.25, 512^2:.15, 1280x720:.1}, spp {16,64,256}, max depth %d" % max_bounces}
 
 
 def cpu_baseline(build, camera, W, H, bounces, target_seconds):
@@ -97,6 +220,8 @@ def main():
     ap.add_argument("--workload", default="terrain_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests in the material-preview serving run (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -178,10 +303,18 @@ def main():
     bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays + sdf_bytes) / launches_per_step
     avg_ms = st_ms[dom] / max(1, st_n[dom])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-    roofline = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    kname = {"extend": "k_extend", "shadow": "k_shadow", "shade": "k_shade", "sss": "k_sss"}[dom]
+    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, kname),
                 "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
                 "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in st_ms.items()}}
+
+    serving = None
+    if args.serving_requests > 0:
+        sess.close()
+        serving = serving_bench(renderer, args.serving_requests)
+        log(f"serving: {serving}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -200,6 +333,9 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "spp_per_s": round(spp_s, 1), "wall_s": round(wall, 4), "gpu_event_s": round(gpu_s, 4),
            "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    if serving is not None:
+        out["preview_p50_ms"], out["preview_p99_ms"] = serving["p50_ms"], serving["p99_ms"]
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
 
 

@@ -12,7 +12,7 @@ from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, Rende
 def __getattr__(name):
     # The renderer needs the built library; import it lazily so the scene
     # model stays usable (and testable) without a GPU build.
-    if name in ("DenrimRenderer", "RenderSession", "GpuScene"):
+    if name in ("DenrimRenderer", "RenderSession", "GpuScene", "RenderServer"):
         from . import renderer
         return getattr(renderer, name)
     raise AttributeError(name)
@@ -22,5 +22,5 @@ __all__ = [
     "AccelerationBuild", "Camera", "CameraLens", "DenrimRendererError", "Environment", "Material", "Mesh",
     "QuadLight", "RenderScene", "Texture2D", "TextureSamplingMode", "Transform", "compile_scene",
     "RenderAccelerationMode", "RenderOutput", "RenderQuality", "RenderSettings",
-    "DenrimRenderer", "RenderSession", "GpuScene",
+    "DenrimRenderer", "RenderSession", "GpuScene", "RenderServer",
 ]

@@ -24,7 +24,10 @@ EXPORTS = (
     "dr_hip_session_sample_count", "dr_hip_session_read_output", "dr_hip_session_stats",
     "dr_hip_session_reset_stats", "dr_hip_session_set_stage_timing", "dr_hip_session_timing",
     "dr_sdf_build_sparse", "dr_sdf_volume_info", "dr_sdf_volume_export", "dr_sdf_volume_free",
-    "dr_sdf_build_dense",
+    "dr_sdf_build_dense", "dr_hip_session_estimate_bytes", "dr_hip_session_set_graphs", "dr_hip_session_query",
+    "dr_hip_server_create", "dr_hip_server_destroy", "dr_hip_server_submit", "dr_hip_server_poll",
+    "dr_hip_server_wait", "dr_hip_server_result", "dr_hip_server_timing", "dr_hip_server_release",
+    "dr_hip_server_usage", "dr_hip_session_retarget",
 )
 
 
@@ -72,6 +75,11 @@ class Settings(C.Structure):
     ]
 
 
+class ServerSettings(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("samples_per_batch", C.c_uint32),
+                ("use_graphs", C.c_uint32)]
+
+
 class GpuCamera(C.Structure):
     _fields_ = [("origin", C.c_float * 4), ("lower_left", C.c_float * 4), ("horizontal", C.c_float * 4),
                 ("vertical", C.c_float * 4), ("lens", C.c_float * 4)]
@@ -103,6 +111,7 @@ def load():
     L.dr_hip_session_create.argtypes = [vp, C.POINTER(Settings), C.POINTER(GpuCamera), C.POINTER(GpuCamera),
                                         C.POINTER(vp)]
     L.dr_hip_session_destroy.argtypes = [vp]
+    L.dr_hip_session_retarget.argtypes = [vp, vp, C.POINTER(Settings), C.POINTER(GpuCamera), C.POINTER(GpuCamera)]
     L.dr_hip_session_render.argtypes = [vp, u32]
     L.dr_hip_session_encode.argtypes = [vp, u32]
     L.dr_hip_session_synchronize.argtypes = [vp]
@@ -121,6 +130,19 @@ def load():
     L.dr_sdf_volume_export.argtypes = [vp] + [vp] * 8
     L.dr_sdf_volume_free.argtypes = [vp]
     L.dr_sdf_build_dense.argtypes = [C.POINTER(SdfPrimitive), u32, C.POINTER(SdfBuildSettings)] + [vp] * 4
+    u64, dbl = C.c_uint64, C.c_double
+    L.dr_hip_session_estimate_bytes.argtypes = [vp, C.POINTER(Settings), C.POINTER(u64)]
+    L.dr_hip_session_set_graphs.argtypes = [vp, i32]
+    L.dr_hip_session_query.argtypes = [vp, C.POINTER(i32)]
+    L.dr_hip_server_create.argtypes = [vp, C.POINTER(ServerSettings), C.POINTER(vp)]
+    L.dr_hip_server_destroy.argtypes = [vp]
+    L.dr_hip_server_submit.argtypes = [vp, vp, C.POINTER(Settings), C.POINTER(GpuCamera), u32, C.POINTER(u64)]
+    L.dr_hip_server_poll.argtypes = [vp, u64, C.POINTER(i32)]
+    L.dr_hip_server_wait.argtypes = [vp, u64, u32, C.POINTER(i32)]
+    L.dr_hip_server_result.argtypes = [vp, u64, i32, vp, C.c_size_t]
+    L.dr_hip_server_timing.argtypes = [vp, u64, C.POINTER(dbl), C.POINTER(dbl)]
+    L.dr_hip_server_release.argtypes = [vp, u64]
+    L.dr_hip_server_usage.argtypes = [vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

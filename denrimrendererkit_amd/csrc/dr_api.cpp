@@ -116,7 +116,52 @@ struct dr_session {
     double stageMs[DR_STAGE_COUNT] = {0};
     uint64_t stageLaunches[DR_STAGE_COUNT] = {0};
     double lastMs = 0.0;
+    // Batch graphs: full-frame batches captured once per batch size and
+    // replayed; the batch's first sample index lives in sampleBase.
+    bool useGraphs = false;
+    uint32_t *sampleBase = nullptr;
+    size_t pathCapacity = 0, outCapacity = 0;  // bytes of the two blocks
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
 };
+
+// Path-state block of a session: 8 float4 arrays of N slots, 4 float4
+// arrays of 3N (shadow records), u32 queues (2 x N ping-pong, 3N shadow,
+// N walk, N resume), counters, stats, the sample base word and the
+// per-lane traversal stack spill.
+struct PathLayout {
+    size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, stats, base, ovf;
+    size_t bytes;
+};
+static PathLayout pathLayout(size_t N, size_t stride) {
+    PathLayout l{};
+    size_t bytes = 0;
+    auto take = [&](size_t n) {
+        size_t off = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return off;
+    };
+    l.rayO = take(N * 16), l.rayD = take(N * 16), l.thr = take(N * 16), l.rad = take(N * 16), l.meta = take(N * 16);
+    l.hit = take(N * 16), l.hitN = take(N * 16), l.neeT = take(N * 16);
+    l.shO = take(3 * N * 16), l.shD = take(3 * N * 16), l.shC = take(3 * N * 16), l.shR = take(3 * N * 16);
+    l.q0 = take(N * 4), l.q1 = take(N * 4), l.sq = take(3 * N * 4), l.sss = take(N * 4), l.res = take(N * 4);
+    l.ctr = take(DR_CTR_WORDS * 4), l.stats = take(DR_CNT_SLOTS * 8), l.base = take(4);
+    l.ovf = take(stride * (DR_STACK_MAX - DR_LDS_STACK) * 4);
+    l.bytes = bytes;
+    return l;
+}
+
+// Samples of one pixel in flight together when the caller leaves it open:
+// about 2 M path slots per batch, at most 64 samples.
+static uint32_t autoSamplesPerLaunch(uint64_t pixels) {
+    return uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (2u << 20) / std::max<uint64_t>(pixels, 1))));
+}
+
+static void dropGraphs(dr_session *s) {
+    for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second);
+    s->graphs.clear();
+}
+
+int drDeviceOrdinal(const dr_device *device) { return device->ordinal; }
 
 extern "C" {
 
@@ -399,6 +444,7 @@ dr_status dr_hip_scene_info(const dr_scene *s, uint32_t *nodes, uint32_t *leaves
 
 static void freeSession(dr_session *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    dropGraphs(s);
     devFree(s->pathBlock);
     devFree(s->outBlock);
     if (s->evBegin) (void)hipEventDestroy(s->evBegin);
@@ -407,87 +453,72 @@ static void freeSession(dr_session *s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
-dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr_gpu_camera *camera,
-                                const dr_gpu_camera *prev, dr_session **out) {
-    if (!scene || !st || !camera || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
-    *out = nullptr;
+// Shape of a session for settings: samples per batch, path slots, lanes of
+// the persistent kernels (at most 16 x 128 per CU, no more than the slots
+// need), stack-spill stride, and the bytes of both blocks.
+struct SessionShape {
+    uint32_t k, nSlots, traceGrid, stride;
+    PathLayout lay;
+    size_t outBytes;
+};
+static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, SessionShape &sh) {
     if (st->width == 0 || st->height == 0)  // RenderSession.swift:304-306
         return fail(DR_ERR_INVALID_SCENE, "Render dimensions must be positive.");
     if (st->quality > 2) return fail(DR_ERR_INVALID_ARGUMENT, "quality must be 0 (preview), 1 (interactive) or 2 (final)");
     if (st->max_bounces > DR_MAX_BOUNCES) return fail(DR_ERR_INVALID_ARGUMENT, "max_bounces exceeds 64");
-    DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
-    auto *s = new dr_session;
+    const uint64_t pixels = uint64_t(st->width) * st->height;
+    sh.k = st->samples_per_launch ? st->samples_per_launch : autoSamplesPerLaunch(pixels);
+    if (pixels * sh.k > 0xFFFFFFFFull / 3) return fail(DR_ERR_INVALID_ARGUMENT, "image too large");
+    sh.nSlots = uint32_t(pixels * sh.k);
+    const uint32_t full = uint32_t(std::max(1, scene->device->cuCount)) * 16u;
+    sh.traceGrid = std::max(1u, std::min(full, (sh.nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
+    sh.stride = std::max(sh.nSlots, sh.traceGrid * DR_TRACE_BLOCK);
+    sh.lay = pathLayout(sh.nSlots, sh.stride);
+    sh.outBytes = size_t(pixels) * 16 * 8;
+    return DR_OK;
+}
+
+// Points the session's arrays into its blocks for `sh` and clears the
+// outputs and counters (on the session stream): a fresh session's state.
+static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *st, const dr_gpu_camera *camera,
+                             const dr_gpu_camera *prev, const SessionShape &sh) {
     s->scene = scene;
     s->settings = *st;
     s->sceneDev = scene->dev;
     s->sceneDev.sdf.renderQuality = st->quality;  // constants.renderQuality (RenderSettings.swift:106-115)
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : s->camera;
-    const uint64_t pixels = uint64_t(st->width) * st->height;
-    uint32_t k = st->samples_per_launch;
-    if (k == 0) k = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (2u << 20) / std::max<uint64_t>(pixels, 1))));
-    s->samplesPerLaunch = k;
-    if (pixels * k > 0xFFFFFFFFull / 3) {
-        delete s;
-        return fail(DR_ERR_INVALID_ARGUMENT, "image too large");
-    }
-    s->nSlots = uint32_t(pixels * k);
-    s->traceGrid = uint32_t(std::max(1, scene->device->cuCount)) * 16u;  // 16 x 128 lanes per CU
-    const uint32_t traceLanes = s->traceGrid * DR_TRACE_BLOCK;
-    const uint32_t stride = std::max(s->nSlots, traceLanes);
-    const size_t N = s->nSlots;
-
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
-    if (e == hipSuccess) e = hipEventCreate(&s->evEnd);
-    // Path state: 8 float4 arrays of N, 4 float4 arrays of 3N, 3 u32 queues
-    // (2 x N, 3N), counters, stats, stack overflow.
-    size_t bytes = 0;
-    auto take = [&](size_t n) {
-        size_t off = bytes;
-        bytes += (n + 255) & ~size_t(255);
-        return off;
-    };
-    size_t oRayO = take(N * 16), oRayD = take(N * 16), oThr = take(N * 16), oRad = take(N * 16), oMeta = take(N * 16),
-           oHit = take(N * 16), oHitN = take(N * 16), oNeeT = take(N * 16);
-    size_t oShO = take(3 * N * 16), oShD = take(3 * N * 16), oShC = take(3 * N * 16), oShR = take(3 * N * 16);
-    size_t oQ0 = take(N * 4), oQ1 = take(N * 4), oSQ = take(3 * N * 4), oSss = take(N * 4), oRes = take(N * 4);
-    size_t oCtr = take(DR_CTR_WORDS * 4), oStats = take(DR_CNT_SLOTS * 8);
-    size_t oOvf = take(size_t(stride) * (DR_STACK_MAX - DR_LDS_STACK) * 4);
-    if (e == hipSuccess) e = hipMalloc(&s->pathBlock, bytes);
-    if (e == hipSuccess) e = hipMemset(s->pathBlock, 0, bytes);
-    const size_t plane = pixels * 16;
-    if (e == hipSuccess) e = hipMalloc(&s->outBlock, plane * 8);
-    if (e == hipSuccess) e = hipMemset(s->outBlock, 0, plane * 8);
-    if (e != hipSuccess) {
-        freeSession(s);
-        delete s;
-        return fail(e == hipErrorOutOfMemory ? DR_ERR_OUT_OF_MEMORY : DR_ERR_COMMAND_BUFFER_FAILED,
-                    std::string("session buffers: ") + hipGetErrorString(e));
-    }
+    s->samplesPerLaunch = sh.k;
+    s->nSlots = sh.nSlots;
+    s->traceGrid = sh.traceGrid;
+    s->sampleCount = 0;
+    dropGraphs(s);
+    const PathLayout &l = sh.lay;
     char *b = static_cast<char *>(s->pathBlock);
     DrPaths &P = s->paths;
-    P.rayO = reinterpret_cast<float4 *>(b + oRayO);
-    P.rayD = reinterpret_cast<float4 *>(b + oRayD);
-    P.thr = reinterpret_cast<float4 *>(b + oThr);
-    P.rad = reinterpret_cast<float4 *>(b + oRad);
-    P.meta = reinterpret_cast<uint4 *>(b + oMeta);
-    P.hit = reinterpret_cast<float4 *>(b + oHit);
-    P.hitN = reinterpret_cast<float4 *>(b + oHitN);
-    P.neeT = reinterpret_cast<float4 *>(b + oNeeT);
-    P.shO = reinterpret_cast<float4 *>(b + oShO);
-    P.shD = reinterpret_cast<float4 *>(b + oShD);
-    P.shC = reinterpret_cast<float4 *>(b + oShC);
-    P.shR = reinterpret_cast<float4 *>(b + oShR);
-    P.queue[0] = reinterpret_cast<uint32_t *>(b + oQ0);
-    P.queue[1] = reinterpret_cast<uint32_t *>(b + oQ1);
-    P.shadowQueue = reinterpret_cast<uint32_t *>(b + oSQ);
-    P.sssQueue = reinterpret_cast<uint32_t *>(b + oSss);
-    P.resumeQueue = reinterpret_cast<uint32_t *>(b + oRes);
-    P.counters = reinterpret_cast<uint32_t *>(b + oCtr);
-    P.stats = reinterpret_cast<unsigned long long *>(b + oStats);
-    P.overflow = reinterpret_cast<uint32_t *>(b + oOvf);
-    P.overflowStride = stride;
+    P.rayO = reinterpret_cast<float4 *>(b + l.rayO);
+    P.rayD = reinterpret_cast<float4 *>(b + l.rayD);
+    P.thr = reinterpret_cast<float4 *>(b + l.thr);
+    P.rad = reinterpret_cast<float4 *>(b + l.rad);
+    P.meta = reinterpret_cast<uint4 *>(b + l.meta);
+    P.hit = reinterpret_cast<float4 *>(b + l.hit);
+    P.hitN = reinterpret_cast<float4 *>(b + l.hitN);
+    P.neeT = reinterpret_cast<float4 *>(b + l.neeT);
+    P.shO = reinterpret_cast<float4 *>(b + l.shO);
+    P.shD = reinterpret_cast<float4 *>(b + l.shD);
+    P.shC = reinterpret_cast<float4 *>(b + l.shC);
+    P.shR = reinterpret_cast<float4 *>(b + l.shR);
+    P.queue[0] = reinterpret_cast<uint32_t *>(b + l.q0);
+    P.queue[1] = reinterpret_cast<uint32_t *>(b + l.q1);
+    P.shadowQueue = reinterpret_cast<uint32_t *>(b + l.sq);
+    P.sssQueue = reinterpret_cast<uint32_t *>(b + l.sss);
+    P.resumeQueue = reinterpret_cast<uint32_t *>(b + l.res);
+    P.counters = reinterpret_cast<uint32_t *>(b + l.ctr);
+    P.stats = reinterpret_cast<unsigned long long *>(b + l.stats);
+    s->sampleBase = reinterpret_cast<uint32_t *>(b + l.base);
+    P.overflow = reinterpret_cast<uint32_t *>(b + l.ovf);
+    P.overflowStride = sh.stride;
+    const size_t plane = sh.outBytes / 8;
     char *o = static_cast<char *>(s->outBlock);
     DrOutputs &O = s->out;
     O.accumulation = reinterpret_cast<float4 *>(o + 0 * plane);
@@ -498,8 +529,77 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     O.objectID = reinterpret_cast<float4 *>(o + 5 * plane);
     O.motion = reinterpret_cast<float4 *>(o + 6 * plane);
     O.triangleID = reinterpret_cast<float4 *>(o + 7 * plane);
+    DR_HIP(hipMemsetAsync(s->outBlock, 0, sh.outBytes, s->stream), "clear outputs");
+    DR_HIP(hipMemsetAsync(P.stats, 0, DR_CNT_SLOTS * 8, s->stream), "clear stats");
+    for (int i = 0; i < DR_STAGE_COUNT; ++i) {
+        s->stageMs[i] = 0.0;
+        s->stageLaunches[i] = 0;
+    }
+    return DR_OK;
+}
+
+dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr_gpu_camera *camera,
+                                const dr_gpu_camera *prev, dr_session **out) {
+    if (!scene || !st || !camera || !out) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    SessionShape sh;
+    dr_status rc = sessionShape(scene, st, sh);
+    if (rc != DR_OK) return rc;
+    DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
+    auto *s = new dr_session;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
+    if (e == hipSuccess) e = hipEventCreate(&s->evEnd);
+    if (e == hipSuccess) e = hipMalloc(&s->pathBlock, sh.lay.bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(s->pathBlock, 0, sh.lay.bytes, s->stream);
+    if (e == hipSuccess) e = hipMalloc(&s->outBlock, sh.outBytes);
+    if (e != hipSuccess) {
+        freeSession(s);
+        delete s;
+        return fail(e == hipErrorOutOfMemory ? DR_ERR_OUT_OF_MEMORY : DR_ERR_COMMAND_BUFFER_FAILED,
+                    std::string("session buffers: ") + hipGetErrorString(e));
+    }
+    s->pathCapacity = sh.lay.bytes;
+    s->outCapacity = sh.outBytes;
+    rc = bindSession(s, scene, st, camera, prev, sh);
+    if (rc != DR_OK) {
+        freeSession(s);
+        delete s;
+        return rc;
+    }
     scene->sessions++;
     *out = s;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_retarget(dr_session *s, dr_scene *scene, const dr_settings *st, const dr_gpu_camera *camera,
+                                  const dr_gpu_camera *prev) {
+    if (!s || !scene || !st || !camera) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (scene->device != s->scene->device) return fail(DR_ERR_INVALID_ARGUMENT, "scene is on another device");
+    SessionShape sh;
+    dr_status rc = sessionShape(scene, st, sh);
+    if (rc != DR_OK) return rc;
+    if (sh.lay.bytes > s->pathCapacity || sh.outBytes > s->outCapacity)
+        return fail(DR_ERR_OUT_OF_MEMORY, "settings need more memory than this session holds");
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    dr_scene *old = s->scene;
+    rc = bindSession(s, scene, st, camera, prev, sh);  // s->scene is `scene` from here on
+    if (old != scene) {
+        scene->sessions++;
+        if (--old->sessions == 0 && old->released) {
+            freeScene(old);
+            delete old;
+        }
+    }
+    return rc;
+}
+
+dr_status dr_hip_session_estimate_bytes(const dr_scene *scene, const dr_settings *st, uint64_t *bytes) {
+    if (!scene || !st || !bytes) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    SessionShape sh;
+    dr_status rc = sessionShape(scene, st, sh);
+    if (rc != DR_OK) return rc;
+    *bytes = sh.lay.bytes + sh.outBytes;
     return DR_OK;
 }
 
@@ -600,15 +700,51 @@ static void collectStageTimes(dr_session *s) {
     s->stageEvents.clear();
 }
 
+// The graph of one full-frame batch of k samples (captured on first use):
+// the stage launches of the batch, then the advance of sampleBase by k.
+static dr_status batchGraph(dr_session *s, uint32_t k, hipGraphExec_t *out) {
+    for (auto &g : s->graphs)
+        if (g.first == k) {
+            *out = g.second;
+            return DR_OK;
+        }
+    DrLaunch L = makeLaunch(s, 0, k, 0, 0, s->settings.width, s->settings.height);
+    L.sampleBase = s->sampleBase;
+    DR_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    dr_status r = enqueueBatch(s, L);
+    hipError_t e = r == DR_OK ? dr::launchAdvanceSampleBase(s->sampleBase, k, s->stream) : hipSuccess;
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(s->stream, &graph);
+    if (r != DR_OK) return r;
+    DR_HIP(e, "advance launch");
+    DR_HIP(ec, "end capture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DR_HIP(e, "graph instantiate");
+    s->graphs.emplace_back(k, exec);
+    *out = exec;
+    return DR_OK;
+}
+
 static dr_status encodeSamples(dr_session *s, uint32_t samples) {
     DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
     DR_HIP(hipEventRecord(s->evBegin, s->stream), "event");
+    const bool graphs = s->useGraphs && !s->stageTiming;
+    if (graphs && samples > 0) DR_HIP(dr::launchSetSampleBase(s->sampleBase, s->sampleCount, s->stream), "set base");
     uint32_t left = samples;
     while (left > 0) {
         uint32_t k = std::min(left, s->samplesPerLaunch);
-        DrLaunch L = makeLaunch(s, s->sampleCount, k, 0, 0, s->settings.width, s->settings.height);
-        dr_status r = enqueueBatch(s, L);
-        if (r != DR_OK) return r;
+        if (graphs) {
+            hipGraphExec_t exec = nullptr;
+            dr_status r = batchGraph(s, k, &exec);
+            if (r != DR_OK) return r;
+            DR_HIP(hipGraphLaunch(exec, s->stream), "graph launch");
+        } else {
+            DrLaunch L = makeLaunch(s, s->sampleCount, k, 0, 0, s->settings.width, s->settings.height);
+            dr_status r = enqueueBatch(s, L);
+            if (r != DR_OK) return r;
+        }
         s->sampleCount += k;
         left -= k;
     }
@@ -669,6 +805,8 @@ dr_status dr_hip_session_update_camera(dr_session *s, const dr_gpu_camera *camer
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : old;
     s->sampleCount = 0;
+    (void)hipStreamSynchronize(s->stream);
+    dropGraphs(s);  // the graphs hold the old camera
     return DR_OK;
 }
 
@@ -713,6 +851,28 @@ dr_status dr_hip_session_reset_stats(dr_session *s) {
 dr_status dr_hip_session_set_stage_timing(dr_session *s, int enabled) {
     if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
     s->stageTiming = enabled != 0;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_set_graphs(dr_session *s, int enabled) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (!enabled && s->useGraphs) {
+        (void)hipStreamSynchronize(s->stream);
+        dropGraphs(s);
+    }
+    s->useGraphs = enabled != 0;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_query(dr_session *s, int *done) {
+    if (!s || !done) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    hipError_t e = hipStreamQuery(s->stream);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return DR_OK;
+    }
+    DR_HIP(e, "stream query");
+    *done = 1;
     return DR_OK;
 }
 

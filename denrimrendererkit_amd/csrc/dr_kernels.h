@@ -75,4 +75,8 @@ namespace dr {
 hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
                        uint32_t traceGrid, hipStream_t st);
+// Device word of the first sample index of the next batch: set it, or add
+// the batch's sample count after a batch (the last node of a batch graph).
+hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st);
+hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t st);
 }

@@ -37,6 +37,13 @@
 
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
+// Sample index of the batch's first sample: a launch argument, or a device
+// word advanced by the batch graph itself when batches are replayed.
+__device__ __forceinline__ uint32_t batchBegin(const DrLaunch &L) { return L.sampleBase ? *L.sampleBase : L.sampleBegin; }
+
+__global__ void k_set_base(uint32_t *base, uint32_t value) { *base = value; }
+__global__ void k_advance_base(uint32_t *base, uint32_t samples) { *base += samples; }
+
 // Wave-aggregated append: returns the position for this lane (valid only if
 // keep). All lanes of the wave that are still active must call it.
 __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t *counter) {
@@ -121,7 +128,7 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
     uint32_t p = i - sLocal * tilePixels;
     uint32_t px = L.tileX + p % L.tileWidth;
     uint32_t py = L.tileY + p / L.tileWidth;
-    uint32_t sampleIndex = L.sampleBegin + sLocal;
+    uint32_t sampleIndex = batchBegin(L) + sLocal;
     uint32_t state = L.frameSeed ^ dr_hash(px + py * L.width) ^ dr_hash(sampleIndex + 1u);
     float jx = randomFloat(state);
     float jy = randomFloat(state);
@@ -511,7 +518,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
     uint32_t state = meta.z;
     uint32_t flags = meta.w & DR_FLAG_PRIMARY;
     const bool hasPrimary = (flags & DR_FLAG_PRIMARY) != 0u;
-    const bool lastSample = meta.y == L.sampleBegin + L.samplesInBatch - 1u;
+    const bool lastSample = meta.y == batchBegin(L) + L.samplesInBatch - 1u;
     const uint32_t pix = meta.x;
     const float clampValue = L.sampleRadianceClamp;
     bool alive = false;
@@ -1140,11 +1147,12 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLau
     uint32_t pix = px + py * L.width;
     float4 acc = O.accumulation[pix];
     uint32_t lastFlags = 0;
+    const uint32_t sampleBegin = batchBegin(L);
     for (uint32_t s = 0; s < L.samplesInBatch; ++s) {
         uint32_t slot = s * tilePixels + p;
         float4 l = P.rad[slot];
         float alpha = P.thr[slot].w;
-        float w = 1.0f / ((float)(L.sampleBegin + s) + 1.0f);
+        float w = 1.0f / ((float)(sampleBegin + s) + 1.0f);
         acc.x = mixf(acc.x, l.x, w);
         acc.y = mixf(acc.y, l.y, w);
         acc.z = mixf(acc.z, l.z, w);
@@ -1193,6 +1201,15 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLau
 
 // --------------------------------------------------------------- launches --
 namespace dr {
+
+hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, st, base, value);
+    return hipGetLastError();
+}
+hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(1), 0, st, base, samples);
+    return hipGetLastError();
+}
 
 hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,

@@ -128,6 +128,7 @@ struct DrLaunch {
     float backgroundColor[4];
     uint32_t tileX, tileY, tileWidth, tileHeight;
     uint32_t sampleBegin;      // sample index of the first sample in this batch
+    const uint32_t *sampleBase;  // if set, the device word holding sampleBegin (graph replays)
     uint32_t samplesInBatch;   // samples per pixel in flight together
     uint32_t statsEnabled;
 };

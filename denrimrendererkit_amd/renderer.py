@@ -142,25 +142,31 @@ class GpuScene:
         self.release()
 
 
+def settings_struct(settings: RenderSettings, samples_per_launch: int = 0) -> _lib.Settings:
+    """The C ABI's dr_settings for RenderSettings (RenderSettings.swift:29-115)."""
+    s = _lib.Settings()
+    s.width, s.height = settings.width, settings.height
+    s.max_bounces = max(1, settings.max_bounces)
+    s.quality = settings.quality.value
+    s.sample_radiance_clamp = settings.resolved_sample_radiance_clamp
+    s.transparent_background = int(settings.transparent_background)
+    s.shows_environment_background = int(settings.shows_environment_background)
+    for k in range(3):
+        s.background_color[k] = float(settings.background_color[k])
+    s.denoiser_enabled = int(bool(settings.denoise))
+    s.collects_stats = int(settings.collects_sdf_traversal_stats)
+    s.frame_seed = settings.frame_seed & 0xFFFFFFFF
+    s.samples_per_launch = samples_per_launch
+    return s
+
+
 class RenderSession:
     def __init__(self, scene: GpuScene, settings: RenderSettings, camera: Camera, samples_per_launch: int = 0):
         self.scene = scene
         self.settings = settings
         self.camera = camera
         self.previous_camera = settings.previous_camera or camera
-        s = _lib.Settings()
-        s.width, s.height = settings.width, settings.height
-        s.max_bounces = max(1, settings.max_bounces)
-        s.quality = settings.quality.value
-        s.sample_radiance_clamp = settings.resolved_sample_radiance_clamp
-        s.transparent_background = int(settings.transparent_background)
-        s.shows_environment_background = int(settings.shows_environment_background)
-        for k in range(3):
-            s.background_color[k] = float(settings.background_color[k])
-        s.denoiser_enabled = int(bool(settings.denoise))
-        s.collects_stats = int(settings.collects_sdf_traversal_stats)
-        s.frame_seed = settings.frame_seed & 0xFFFFFFFF
-        s.samples_per_launch = samples_per_launch
+        s = settings_struct(settings, samples_per_launch)
         self._settings = s
         cam = gpu_camera_struct(camera.gpu_camera(settings.width, settings.height))
         prev = gpu_camera_struct(self.previous_camera.gpu_camera(settings.width, settings.height))
@@ -207,6 +213,19 @@ class RenderSession:
         p = gpu_camera_struct(self.previous_camera.gpu_camera(W, H))
         _lib.check(_lib.load().dr_hip_session_update_camera(self._h, C.byref(c), C.byref(p)), "camera")
 
+    def retarget(self, scene: GpuScene, settings: RenderSettings, camera: Camera, samples_per_launch: int = 0) -> None:
+        """Rebinds this session to another scene/settings/camera in its own
+        device memory (dr_hip_session_retarget); raises outOfMemory when the
+        new settings need larger buffers than the session was created with."""
+        prev_cam = settings.previous_camera or camera
+        s = settings_struct(settings, samples_per_launch)
+        cam = gpu_camera_struct(camera.gpu_camera(settings.width, settings.height))
+        prev = gpu_camera_struct(prev_cam.gpu_camera(settings.width, settings.height))
+        _lib.check(_lib.load().dr_hip_session_retarget(self._h, scene._h, C.byref(s), C.byref(cam), C.byref(prev)),
+                   "retarget")
+        self.scene, self.settings, self.camera, self.previous_camera = scene, settings, camera, prev_cam
+        self._settings = s
+
     @property
     def sample_count(self) -> int:
         v = C.c_uint32()
@@ -244,6 +263,16 @@ class RenderSession:
     def reset_stats(self) -> None:
         _lib.check(_lib.load().dr_hip_session_reset_stats(self._h), "reset stats")
 
+    def set_graphs(self, enabled: bool) -> None:
+        """Replay full-frame sample batches as captured HIP graphs."""
+        _lib.check(_lib.load().dr_hip_session_set_graphs(self._h, int(enabled)), "graphs")
+
+    def query(self) -> bool:
+        """True when all encoded work has finished (non-blocking)."""
+        d = C.c_int()
+        _lib.check(_lib.load().dr_hip_session_query(self._h, C.byref(d)), "query")
+        return bool(d.value)
+
     def set_stage_timing(self, enabled: bool) -> None:
         _lib.check(_lib.load().dr_hip_session_set_stage_timing(self._h, int(enabled)), "stage timing")
 
@@ -254,3 +283,75 @@ class RenderSession:
         _lib.check(_lib.load().dr_hip_session_timing(self._h, C.byref(total), ms, n, len(STAGES)), "timing")
         return {"total_ms": total.value, "stage_ms": dict(zip(STAGES, list(ms))),
                 "stage_launches": dict(zip(STAGES, [int(x) for x in n]))}
+
+
+# ---------------------------------------------------------------- serving --
+REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+class RenderServer:
+    """Device-resident render server (dr_hip_server_*): requests on scenes
+    already uploaded to HBM are admitted by memory budget while others run
+    (continuous batching), each replaying graph-captured sample batches on
+    its own stream. A request's pixels equal those of a session rendering it
+    alone."""
+
+    def __init__(self, renderer: DenrimRenderer, memory_budget_bytes: int = 8 << 30, max_active: int = 16,
+                 samples_per_batch: int = 0, use_graphs: bool = True):
+        cfg = _lib.ServerSettings(memory_budget_bytes, max_active, samples_per_batch, int(use_graphs))
+        h = C.c_void_p()
+        _lib.check(_lib.load().dr_hip_server_create(renderer._h, C.byref(cfg), C.byref(h)), "server")
+        self.renderer = renderer
+        self._h = h
+        self._sizes = {}
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().dr_hip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, scene: GpuScene, settings: RenderSettings, camera: Camera, samples: int,
+               samples_per_launch: int = 0) -> int:
+        s = settings_struct(settings, samples_per_launch)
+        cam = gpu_camera_struct(camera.gpu_camera(settings.width, settings.height))
+        t = C.c_uint64()
+        _lib.check(_lib.load().dr_hip_server_submit(self._h, scene._h, C.byref(s), C.byref(cam), samples, C.byref(t)),
+                   "submit")
+        self._sizes[t.value] = (settings.width, settings.height)
+        self._keep[t.value] = scene  # the scene must outlive the request
+        return t.value
+
+    def state(self, ticket: int) -> str:
+        st = C.c_int()
+        _lib.check(_lib.load().dr_hip_server_poll(self._h, ticket, C.byref(st)), "poll")
+        return REQUEST_STATES[st.value]
+
+    def wait(self, ticket: int, timeout_ms: int = 600000) -> str:
+        st = C.c_int()
+        _lib.check(_lib.load().dr_hip_server_wait(self._h, ticket, timeout_ms, C.byref(st)), "wait")
+        return REQUEST_STATES[st.value]
+
+    def result(self, ticket: int, output: RenderOutput = RenderOutput.BEAUTY) -> np.ndarray:
+        W, H = self._sizes[ticket]
+        out = np.zeros((H, W, 4), np.float32)
+        _lib.check(_lib.load().dr_hip_server_result(self._h, ticket, int(output), _ptr(out), out.size), "result")
+        return out
+
+    def timing(self, ticket: int) -> dict:
+        q, r = C.c_double(), C.c_double()
+        _lib.check(_lib.load().dr_hip_server_timing(self._h, ticket, C.byref(q), C.byref(r)), "timing")
+        return {"queued_ms": q.value, "run_ms": r.value, "latency_ms": q.value + r.value}
+
+    def release(self, ticket: int) -> None:
+        _lib.check(_lib.load().dr_hip_server_release(self._h, ticket), "release")
+        self._sizes.pop(ticket, None)
+        self._keep.pop(ticket, None)
+
+    def usage(self) -> dict:
+        b, a, q = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        _lib.check(_lib.load().dr_hip_server_usage(self._h, C.byref(b), C.byref(a), C.byref(q)), "usage")
+        return {"used_bytes": b.value, "active": a.value, "queued": q.value}

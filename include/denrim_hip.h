@@ -189,6 +189,17 @@ typedef struct {
 
 typedef struct dr_sdf_volume dr_sdf_volume;
 
+/* Render server (no reference counterpart: the reference renders one
+ * session at a time on the caller's thread). */
+typedef struct dr_server dr_server;
+typedef struct {
+    uint64_t memory_budget_bytes;  /* session memory of requests on the GPU at once */
+    uint32_t max_active;           /* requests on the GPU at once */
+    uint32_t samples_per_batch;    /* samples a request encodes per scheduling step; 0 = all */
+    uint32_t use_graphs;           /* replay batch graphs (dr_hip_session_set_graphs) */
+} dr_server_settings;
+typedef enum { DR_REQ_QUEUED = 0, DR_REQ_RUNNING = 1, DR_REQ_DONE = 2, DR_REQ_FAILED = 3 } dr_request_state;
+
 /* Thread-local message of the last failing call ("" if none). */
 const char *dr_last_error(void);
 
@@ -246,6 +257,50 @@ dr_status dr_hip_session_read_output(dr_session *session, dr_output kind, float 
  * counters accumulated since creation or the last reset_stats. */
 dr_status dr_hip_session_stats(dr_session *session, uint64_t *counters, size_t count);
 dr_status dr_hip_session_reset_stats(dr_session *session);
+
+/* Rebinds an existing session to another scene, settings and camera,
+ * reusing its device memory (no allocation): the session then behaves like
+ * a newly created one. Fails with DR_ERR_OUT_OF_MEMORY when the new
+ * settings need larger buffers than the session holds. No reference
+ * counterpart: it lets a server recycle sessions without hipMalloc/hipFree. */
+dr_status dr_hip_session_retarget(dr_session *session, dr_scene *scene, const dr_settings *settings,
+                                  const dr_gpu_camera *camera, const dr_gpu_camera *previous_camera);
+
+/* Device memory a session with these settings allocates (path state and
+ * output planes): the admission size of a served request. */
+dr_status dr_hip_session_estimate_bytes(const dr_scene *scene, const dr_settings *settings, uint64_t *bytes);
+
+/* Batch graphs (no reference counterpart: the reference encodes Metal
+ * command buffers per sample, RenderSession.swift:891-1007): when enabled,
+ * each full-frame batch size is captured once as a HIP graph (all stage
+ * launches of one sample batch) and replayed; results are identical to the
+ * launch-by-launch path. Invalidated by update_camera. */
+dr_status dr_hip_session_set_graphs(dr_session *session, int enabled);
+
+/* Non-blocking completion check of the work encoded so far
+ * (MTLCommandBuffer status in the reference). */
+dr_status dr_hip_session_query(dr_session *session, int *done);
+
+/* Device-resident render server with continuous batching: requests
+ * (resident scene, settings, camera, sample count) are admitted by memory
+ * budget and active limit while others run, each on its own stream and a
+ * recycled session (graph-replayed batches when a request has several); a
+ * request's pixels equal those of a session rendering it alone. Results
+ * stay on the device until read (dr_hip_server_result) and released; a
+ * released request's session returns to the server's pool. The budget
+ * bounds all session memory the server holds, pooled sessions included.
+ * The serving loop of the material-preview stream; the reference has no
+ * counterpart. The scenes must outlive their requests. */
+dr_status dr_hip_server_create(dr_device *device, const dr_server_settings *settings, dr_server **out);
+dr_status dr_hip_server_destroy(dr_server *server);  /* finishes admitted and queued requests first */
+dr_status dr_hip_server_submit(dr_server *server, dr_scene *scene, const dr_settings *settings,
+                               const dr_gpu_camera *camera, uint32_t samples, uint64_t *ticket);
+dr_status dr_hip_server_poll(dr_server *server, uint64_t ticket, dr_request_state *state);
+dr_status dr_hip_server_wait(dr_server *server, uint64_t ticket, uint32_t timeout_ms, dr_request_state *state);
+dr_status dr_hip_server_result(dr_server *server, uint64_t ticket, dr_output kind, float *rgba, size_t floats);
+dr_status dr_hip_server_timing(dr_server *server, uint64_t ticket, double *queued_ms, double *run_ms);
+dr_status dr_hip_server_release(dr_server *server, uint64_t ticket);
+dr_status dr_hip_server_usage(dr_server *server, uint64_t *used_bytes, uint32_t *active, uint32_t *queued);
 
 /* GPU time of the session's work, measured with HIP events on the session
  * stream: `total_ms` of the last render/encode call; per-stage sums since the

@@ -1,0 +1,125 @@
+"""GPU tests of batch graphs and the render server.
+
+  * Graph replays of sample batches equal the launch-by-launch path
+    bit for bit, over several calls with eager work and synchronizations in
+    between (the device-side sample base advances inside the graph).
+  * Served requests: each result equals the same request rendered alone on
+    a plain session, bit for bit (tolerance: exact), whether it ran alone or
+    in a mix of sizes and sample counts admitted while others were running;
+    the memory budget bounds what runs at once.
+  * A retargeted session (another scene, size, settings and camera in the
+    same device memory) renders exactly what a new session renders, and
+    refuses settings that need more memory than it holds.
+"""
+import numpy as np
+import pytest
+
+from denrimrendererkit_amd import RenderQuality, RenderScene, RenderSettings, compile_scene, synthetic
+from denrimrendererkit_amd.renderer import RenderServer
+from denrimrendererkit_amd.settings import RenderOutput
+
+pytestmark = pytest.mark.gpu
+
+
+def _settings(W, H, bounces=4):
+    return RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=7)
+
+
+def test_graph_replays_match_eager(renderer):
+    scene = RenderScene.cornell_box()
+    gs = renderer.upload(compile_scene(scene))
+    st = _settings(40, 32)
+    eager = renderer.make_session(gs, st, camera=scene.camera, samples_per_launch=3)
+    graph = renderer.make_session(gs, st, camera=scene.camera, samples_per_launch=3)
+    graph.set_graphs(True)
+    for n in (3, 3, 5, 1, 3):  # full batches, a partial batch, a single sample
+        eager.render(n)
+        graph.render(n)
+        eager.pixels()  # eager work and synchronization between replays
+        for out in (RenderOutput.BEAUTY, RenderOutput.DEPTH, RenderOutput.NORMAL, RenderOutput.MATERIAL_ID):
+            assert np.array_equal(graph.pixels(out), eager.pixels(out)), (n, out)
+        assert graph.sample_count == eager.sample_count
+    graph.reset_accumulation()
+    eager.reset_accumulation()
+    graph.render(4)
+    eager.render(4)
+    assert np.array_equal(graph.pixels(), eager.pixels())
+
+
+def test_retarget_equals_new_session(renderer):
+    scenes = [synthetic.material_preview_scene(seed, hdri=(64, 32), segments=20) for seed in (3, 4)]
+    gs = [renderer.upload(compile_scene(s)) for s in scenes]
+    sess = renderer.make_session(gs[0], _settings(64, 48), camera=scenes[0].camera, samples_per_launch=4)
+    sess.render(6)
+    for i, (W, H, spp, k) in ((1, (48, 40, 5, 2)), (0, (32, 32, 3, 3)), (1, (64, 48, 4, 4))):
+        st = _settings(W, H, bounces=3)
+        sess.retarget(gs[i], st, scenes[i].camera, samples_per_launch=k)
+        sess.render(spp)
+        fresh = renderer.make_session(gs[i], st, camera=scenes[i].camera, samples_per_launch=k)
+        fresh.render(spp)
+        assert sess.sample_count == fresh.sample_count == spp
+        for out in (RenderOutput.BEAUTY, RenderOutput.DEPTH, RenderOutput.OBJECT_ID):
+            assert np.array_equal(sess.pixels(out), fresh.pixels(out)), (i, W, H, out)
+        assert np.array_equal(sess.stats(), fresh.stats())
+    from denrimrendererkit_amd.scene import DenrimRendererError
+    with pytest.raises(DenrimRendererError):
+        sess.retarget(gs[0], _settings(128, 128), scenes[0].camera, samples_per_launch=4)
+
+
+def _alone(renderer, gs, camera, st, spp):
+    s = renderer.make_session(gs, st, camera=camera)
+    s.render(spp)
+    return s.pixels(RenderOutput.BEAUTY), s.pixels(RenderOutput.ALBEDO)
+
+
+def test_served_requests_equal_alone_in_mix(renderer):
+    scenes = [synthetic.material_preview_scene(seed, hdri=(64, 32), segments=24) for seed in range(3)]
+    uploaded = [renderer.upload(compile_scene(s)) for s in scenes]
+    reqs = [(0, 48, 48, 8), (1, 64, 32, 4), (2, 24, 40, 16), (0, 32, 32, 2), (1, 48, 48, 8), (2, 64, 64, 3)]
+    server = RenderServer(renderer, memory_budget_bytes=4 << 30, max_active=3)
+    try:
+        tickets = [server.submit(uploaded[i], _settings(W, H), scenes[i].camera, spp) for i, W, H, spp in reqs]
+        for t in tickets:
+            assert server.wait(t) == "done"
+        for t, (i, W, H, spp) in zip(tickets, reqs):
+            beauty, albedo = _alone(renderer, uploaded[i], scenes[i].camera, _settings(W, H), spp)
+            assert np.array_equal(server.result(t), beauty), t
+            assert np.array_equal(server.result(t, RenderOutput.ALBEDO), albedo), t
+            tm = server.timing(t)
+            assert tm["run_ms"] > 0 and tm["queued_ms"] >= 0
+            server.release(t)
+        # Alone: one request on an idle server gives the same pixels again.
+        i, W, H, spp = reqs[4]
+        t = server.submit(uploaded[i], _settings(W, H), scenes[i].camera, spp)
+        assert server.wait(t) == "done"
+        assert np.array_equal(server.result(t), _alone(renderer, uploaded[i], scenes[i].camera, _settings(W, H), spp)[0])
+        server.release(t)
+        use = server.usage()
+        assert use["active"] == 0 and use["queued"] == 0
+        assert 0 < use["used_bytes"] <= 4 << 30  # pooled sessions stay within the budget
+    finally:
+        server.close()
+
+
+def test_server_budget_limits_concurrency(renderer):
+    scene = synthetic.material_preview_scene(5, hdri=(64, 32), segments=16)
+    gs = renderer.upload(compile_scene(scene))
+    st = _settings(32, 32, bounces=2)
+    from denrimrendererkit_amd import _lib
+    import ctypes as C
+    from denrimrendererkit_amd.renderer import settings_struct
+    need = C.c_uint64()
+    ss = settings_struct(st, 4)
+    _lib.check(_lib.load().dr_hip_session_estimate_bytes(gs._h, C.byref(ss), C.byref(need)), "estimate")
+    server = RenderServer(renderer, memory_budget_bytes=int(need.value * 1.5), max_active=8)
+    try:
+        tickets = [server.submit(gs, st, scene.camera, 4, samples_per_launch=4) for _ in range(4)]
+        for t in tickets:
+            assert server.wait(t) == "done"
+        ref = server.result(tickets[0])
+        for t in tickets[1:]:
+            assert np.array_equal(server.result(t), ref)
+        too_big = server.submit(gs, _settings(512, 512), scene.camera, 4, samples_per_launch=64)
+        assert server.wait(too_big) == "failed"
+    finally:
+        server.close()
