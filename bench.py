@@ -286,7 +286,9 @@ def main():
     sec = max(wall, gpu_s)
     mrays = queries_per_step * args.steps / sec / 1e6
     spp_s = W * H * args.steps / sec
-    beauty = sess.pixels()
+    r0 = time.perf_counter()
+    beauty = sess.pixels()  # one beauty plane over PCIe: what a caller reads after the K samples
+    readback_s = time.perf_counter() - r0
     assert np.isfinite(beauty).all(), "non-finite beauty"
 
     # Roofline of the dominant kernel (largest share of stage time).
@@ -332,6 +334,8 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "spp_per_s": round(spp_s, 1), "wall_s": round(wall, 4), "gpu_event_s": round(gpu_s, 4),
+           "readback_ms": round(readback_s * 1e3, 3),
+           "pcie_inclusive_mrays_per_s": round(queries_per_step * args.steps / (sec + readback_s) / 1e6, 2),
            "config": config, "roofline": roofline, "cpu_baseline": cpu}
     if serving is not None:
         out["preview_p50_ms"], out["preview_p99_ms"] = serving["p50_ms"], serving["p99_ms"]

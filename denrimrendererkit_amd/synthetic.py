@@ -286,6 +286,34 @@ def layered_terrain_scene(scale: float = 1.0, hdri: Tuple[int, int] = (2048, 102
     return s
 
 
+def layered_material_gallery(hdri: Tuple[int, int] = (64, 32), segments: int = 16) -> RenderScene:
+    """Parity scene for the layered BSDF: two spheres of each of the six
+    material kinds of the terrain workload (rough dielectric, metal,
+    clearcoat, thin film, sheen, random-walk subsurface) on a ground plane,
+    lit by 8 emissive triangles and a log-normal HDRI with a sun."""
+    rng = content_rng(12)
+    env = Environment(texture=lognormal_hdri(rng, *hdri, sun_peak=500.0), intensity=1.0, max_radiance=0.0)
+    cam = Camera(origin=(0.0, 2.2, 7.5), target=(0.0, 0.6, 0.0), vertical_field_of_view_degrees=45)
+    s = RenderScene(camera=cam, environment=env)
+    ground = s.add_material(Material(base_color=(0.45, 0.42, 0.38), roughness=0.8))
+    s.add(Mesh.quad((-8, 0, -8), (-8, 0, 8), (8, 0, 8), (8, 0, -8)), ground)
+    sphere = uv_sphere(segments, segments + 1, 0.45)
+    for kind in range(6):
+        for row in range(2):
+            mat = s.add_material(_layered_material(rng, kind))
+            x, z = -3.0 + 1.2 * kind, -0.7 + 1.4 * row
+            s.add(sphere, mat, Transform.translation((x, 0.45, z)))
+    lights = s.add_material(Material(base_color=(1, 1, 1), emission=(1.0, 0.9, 0.75), emission_strength=25.0,
+                                     specular=0.0))
+    tris = []
+    for k in range(8):
+        cx, cz, cy, r = -3.5 + k, float(rng.uniform(-1.5, 1.5)), float(rng.uniform(2.0, 3.0)), 0.4
+        p = [(cx + r * math.cos(k + j * 2.0944), cy, cz + r * math.sin(k + j * 2.0944)) for j in range(3)]
+        tris.extend(_facing(p, (cx, cy - 1.0, cz)))
+    s.add(Mesh(np.array(tris, f32), np.arange(len(tris), dtype=np.uint32)), lights)
+    return s
+
+
 def sdf_primitives(rng: np.random.Generator, count: int = 200, metal: int = 1, matte: int = 2):
     from .sdf import SDFPrimitive
     prims = []
