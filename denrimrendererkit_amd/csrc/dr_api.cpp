@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/denrim_hip.h"
@@ -422,11 +423,40 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     return DR_OK;
 }
 
+}  // extern "C"
+
+// Scene lifetime: sessions (and requests queued on a render server) hold a
+// reference; a released scene is freed when the last holder lets go. The
+// server's scheduler thread takes and drops references, hence the lock.
+static std::mutex gSceneRefs;
+
+void drSceneRetain(dr_scene *scene) {
+    std::lock_guard<std::mutex> lk(gSceneRefs);
+    scene->sessions++;
+}
+
+void drSceneDrop(dr_scene *scene) {
+    bool last;
+    {
+        std::lock_guard<std::mutex> lk(gSceneRefs);
+        last = --scene->sessions == 0 && scene->released;
+    }
+    if (last) {
+        freeScene(scene);
+        delete scene;
+    }
+}
+
+extern "C" {
+
 dr_status dr_hip_scene_release(dr_scene *scene) {
     if (!scene) return DR_OK;
-    if (scene->sessions > 0) {
-        scene->released = true;  // freed with its last session
-        return DR_OK;
+    {
+        std::lock_guard<std::mutex> lk(gSceneRefs);
+        if (scene->sessions > 0) {
+            scene->released = true;  // freed with its last holder
+            return DR_OK;
+        }
     }
     freeScene(scene);
     delete scene;
@@ -567,7 +597,7 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
         delete s;
         return rc;
     }
-    scene->sessions++;
+    drSceneRetain(scene);
     *out = s;
     return DR_OK;
 }
@@ -585,11 +615,8 @@ dr_status dr_hip_session_retarget(dr_session *s, dr_scene *scene, const dr_setti
     dr_scene *old = s->scene;
     rc = bindSession(s, scene, st, camera, prev, sh);  // s->scene is `scene` from here on
     if (old != scene) {
-        scene->sessions++;
-        if (--old->sessions == 0 && old->released) {
-            freeScene(old);
-            delete old;
-        }
+        drSceneRetain(scene);
+        drSceneDrop(old);
     }
     return rc;
 }
@@ -608,10 +635,7 @@ dr_status dr_hip_session_destroy(dr_session *s) {
     freeSession(s);
     dr_scene *sc = s->scene;
     delete s;
-    if (sc && --sc->sessions == 0 && sc->released) {
-        freeScene(sc);
-        delete sc;
-    }
+    if (sc) drSceneDrop(sc);
     return DR_OK;
 }
 

@@ -31,6 +31,8 @@
 #include "../../include/denrim_hip.h"
 
 int drDeviceOrdinal(const dr_device *device);  // dr_api.cpp
+void drSceneRetain(dr_scene *scene);            // dr_api.cpp: scene references
+void drSceneDrop(dr_scene *scene);
 
 namespace {
 
@@ -244,8 +246,10 @@ dr_status dr_hip_server_destroy(dr_server *s) {
     }
     s->cv.notify_all();
     s->worker.join();
-    for (auto &kv : s->all)
+    for (auto &kv : s->all) {
         if (kv.second->shell.session) dr_hip_session_destroy(kv.second->shell.session);
+        drSceneDrop(kv.second->scene);
+    }
     for (auto &sh : s->pool) dr_hip_session_destroy(sh.session);
     delete s;
     return DR_OK;
@@ -269,6 +273,7 @@ dr_status dr_hip_server_submit(dr_server *s, dr_scene *scene, const dr_settings 
     dr_status rc = dr_hip_session_estimate_bytes(scene, &r->settings, &r->bytes);
     if (rc != DR_OK) return rc;
     r->tSubmit = Clock::now();
+    drSceneRetain(scene);  // the request holds the scene until released
     std::lock_guard<std::mutex> lk(s->mu);
     r->ticket = s->nextTicket++;
     *ticket = r->ticket;
@@ -347,6 +352,7 @@ dr_status dr_hip_server_release(dr_server *s, uint64_t ticket) {
     if (r->state != DR_REQ_DONE && r->state != DR_REQ_FAILED) return DR_ERR_INVALID_ARGUMENT;
     if (r->readers) return DR_ERR_INVALID_ARGUMENT;  // a result() call is reading it
     if (r->shell.session) s->pool.push_back(r->shell);
+    drSceneDrop(r->scene);
     s->all.erase(it);
     s->cv.notify_all();
     return DR_OK;

@@ -290,7 +290,8 @@ dr_status dr_hip_session_query(dr_session *session, int *done);
  * released request's session returns to the server's pool. The budget
  * bounds all session memory the server holds, pooled sessions included.
  * The serving loop of the material-preview stream; the reference has no
- * counterpart. The scenes must outlive their requests. */
+ * counterpart. A request holds a reference to its scene until released
+ * (dr_hip_scene_release defers the free). */
 dr_status dr_hip_server_create(dr_device *device, const dr_server_settings *settings, dr_server **out);
 dr_status dr_hip_server_destroy(dr_server *server);  /* finishes admitted and queued requests first */
 dr_status dr_hip_server_submit(dr_server *server, dr_scene *scene, const dr_settings *settings,

@@ -123,3 +123,23 @@ def test_server_budget_limits_concurrency(renderer):
         assert server.wait(too_big) == "failed"
     finally:
         server.close()
+
+
+def test_request_holds_released_scene(renderer):
+    # A queued request keeps its scene alive after the caller releases it.
+    scene = synthetic.material_preview_scene(9, hdri=(64, 32), segments=16)
+    st = _settings(40, 40)
+    keep = renderer.upload(compile_scene(scene))
+    alone = _alone(renderer, keep, scene.camera, st, 4)[0]
+    gs = renderer.upload(compile_scene(scene))
+    server = RenderServer(renderer, memory_budget_bytes=1 << 30, max_active=1)
+    try:
+        busy = server.submit(keep, _settings(256, 256), scene.camera, 32)
+        t = server.submit(gs, st, scene.camera, 4)
+        server._keep.pop(t)
+        gs.release()
+        assert server.wait(t) == "done" and server.wait(busy) == "done"
+        assert np.array_equal(server.result(t), alone)
+        server.release(t)
+    finally:
+        server.close()
