@@ -6,7 +6,8 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libdenrim_hip.so")
-SOURCES = ["dr_api.cpp", "dr_bvh.cpp", "dr_sdf_build.cpp", "dr_server.cpp", "dr_kernels.hip"]
+SOURCES = ["dr_api.cpp", "dr_bvh.cpp", "dr_sdf_build.cpp", "dr_server.cpp", "dr_unet.cpp", "dr_kernels.hip",
+           "dr_denoise.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

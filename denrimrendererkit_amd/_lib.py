@@ -27,7 +27,11 @@ EXPORTS = (
     "dr_sdf_build_dense", "dr_hip_session_estimate_bytes", "dr_hip_session_set_graphs", "dr_hip_session_query",
     "dr_hip_server_create", "dr_hip_server_destroy", "dr_hip_server_submit", "dr_hip_server_poll",
     "dr_hip_server_wait", "dr_hip_server_result", "dr_hip_server_timing", "dr_hip_server_release",
-    "dr_hip_server_usage", "dr_hip_session_retarget",
+    "dr_hip_server_usage", "dr_hip_session_retarget", "dr_hip_session_denoise",
+    "dr_hip_session_set_denoiser_network", "dr_hip_unet_create", "dr_hip_unet_destroy", "dr_hip_unet_param_count",
+    "dr_hip_unet_get_params", "dr_hip_unet_set_params", "dr_hip_unet_set_batch", "dr_hip_unet_forward_backward",
+    "dr_hip_unet_get_grads", "dr_hip_unet_get_output", "dr_hip_unet_train_step", "dr_hip_unet_dataset_reserve",
+    "dr_hip_unet_dataset_add", "dr_hip_unet_train_dataset",
 )
 
 
@@ -71,7 +75,9 @@ class Settings(C.Structure):
         ("sample_radiance_clamp", C.c_float), ("transparent_background", C.c_uint32),
         ("shows_environment_background", C.c_uint32), ("background_color", C.c_float * 3),
         ("denoiser_enabled", C.c_uint32), ("collects_stats", C.c_uint32), ("frame_seed", C.c_uint32),
-        ("samples_per_launch", C.c_uint32),
+        ("samples_per_launch", C.c_uint32), ("denoiser", C.c_uint32), ("denoise_radius", C.c_uint32),
+        ("denoise_iterations", C.c_uint32), ("denoise_normal_sigma", C.c_float), ("denoise_depth_sigma", C.c_float),
+        ("denoise_albedo_sigma", C.c_float), ("denoise_color_sigma", C.c_float),
     ]
 
 
@@ -143,6 +149,22 @@ def load():
     L.dr_hip_server_timing.argtypes = [vp, u64, C.POINTER(dbl), C.POINTER(dbl)]
     L.dr_hip_server_release.argtypes = [vp, u64]
     L.dr_hip_server_usage.argtypes = [vp, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32)]
+    f32, fp = C.c_float, C.POINTER(C.c_float)
+    L.dr_hip_session_denoise.argtypes = [vp]
+    L.dr_hip_session_set_denoiser_network.argtypes = [vp, vp]
+    L.dr_hip_unet_create.argtypes = [vp, u32, C.POINTER(vp)]
+    L.dr_hip_unet_destroy.argtypes = [vp]
+    L.dr_hip_unet_param_count.argtypes = [vp, C.POINTER(u64)]
+    L.dr_hip_unet_get_params.argtypes = [vp, vp, C.c_size_t]
+    L.dr_hip_unet_set_params.argtypes = [vp, vp, C.c_size_t]
+    L.dr_hip_unet_set_batch.argtypes = [vp, vp, vp, u32, u32, u32]
+    L.dr_hip_unet_forward_backward.argtypes = [vp, fp]
+    L.dr_hip_unet_get_grads.argtypes = [vp, vp, C.c_size_t]
+    L.dr_hip_unet_get_output.argtypes = [vp, vp, C.c_size_t]
+    L.dr_hip_unet_train_step.argtypes = [vp, f32, fp]
+    L.dr_hip_unet_dataset_reserve.argtypes = [vp, u32, u32, u32]
+    L.dr_hip_unet_dataset_add.argtypes = [vp, vp, vp, C.POINTER(u32)]
+    L.dr_hip_unet_train_dataset.argtypes = [vp, u32, u32, f32, f32, u32, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

@@ -19,6 +19,8 @@
 #include "dr_bvh.h"
 #include "dr_kernels.h"
 #include "dr_scene.h"
+#include "dr_denoise.h"
+#include "dr_internal.h"
 
 namespace {
 
@@ -28,6 +30,12 @@ dr_status fail(dr_status s, const std::string &msg) {
     g_error = msg;
     return s;
 }
+
+}  // namespace
+
+dr_status drFail(dr_status s, const std::string &msg) { return fail(s, msg); }
+
+namespace {
 
 #define DR_HIP(call, what)                                                                                   \
     do {                                                                                                     \
@@ -65,65 +73,6 @@ DrCamera toCam(const dr_gpu_camera &c) {
 }
 
 }  // namespace
-
-struct dr_device {
-    int ordinal = 0;
-    int cuCount = 0;
-    std::string arch;
-};
-
-struct dr_scene {
-    dr_device *device = nullptr;
-    DrSceneDev dev{};
-    DrNode *nodes = nullptr;
-    float4 *triPos = nullptr;
-    DrTriangle *triangles = nullptr;
-    DrMaterial *materials = nullptr;
-    DrLight *lights = nullptr;
-    float2 *env = nullptr;
-    uint32_t *envGuide = nullptr;
-    uint4 *texDesc = nullptr;
-    float4 *texels = nullptr;
-    DrVolume *volumes = nullptr;
-    DrVolumeSample *denseSamples = nullptr;
-    DrBrick *bricks = nullptr;
-    uint16_t *dist16 = nullptr;
-    float *dist32 = nullptr;
-    uint2 *sampleMaterial = nullptr;
-    DrBrickGrid *grids = nullptr;
-    uint32_t *gridIndices = nullptr;
-    dr::BvhBuildStats stats;
-    int sessions = 0;
-    bool released = false;
-};
-
-struct dr_session {
-    dr_scene *scene = nullptr;
-    DrSceneDev sceneDev{};  // the scene's device view with this session's quality level
-    dr_settings settings{};
-    DrCamera camera{}, previousCamera{};
-    uint32_t sampleCount = 0;
-    uint32_t samplesPerLaunch = 1;
-    uint32_t nSlots = 0;
-    uint32_t traceGrid = 0;
-    hipStream_t stream = nullptr;
-    DrPaths paths{};
-    DrOutputs out{};
-    void *pathBlock = nullptr;   // one allocation for all path-state arrays
-    void *outBlock = nullptr;    // one allocation for all output planes
-    hipEvent_t evBegin = nullptr, evEnd = nullptr;
-    std::vector<hipEvent_t> stageEvents;
-    bool stageTiming = false;
-    double stageMs[DR_STAGE_COUNT] = {0};
-    uint64_t stageLaunches[DR_STAGE_COUNT] = {0};
-    double lastMs = 0.0;
-    // Batch graphs: full-frame batches captured once per batch size and
-    // replayed; the batch's first sample index lives in sampleBase.
-    bool useGraphs = false;
-    uint32_t *sampleBase = nullptr;
-    size_t pathCapacity = 0, outCapacity = 0;  // bytes of the two blocks
-    std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
-};
 
 // Path-state block of a session: 8 float4 arrays of N slots, 4 float4
 // arrays of 3N (shadow records), u32 queues (2 x N ping-pong, 3N shadow,
@@ -477,6 +426,7 @@ static void freeSession(dr_session *s) {
     dropGraphs(s);
     devFree(s->pathBlock);
     devFree(s->outBlock);
+    devFree(s->unetWork);
     if (s->evBegin) (void)hipEventDestroy(s->evBegin);
     if (s->evEnd) (void)hipEventDestroy(s->evEnd);
     for (auto ev : s->stageEvents) (void)hipEventDestroy(ev);
@@ -490,6 +440,7 @@ struct SessionShape {
     uint32_t k, nSlots, traceGrid, stride;
     PathLayout lay;
     size_t outBytes;
+    int planes;
 };
 static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, SessionShape &sh) {
     if (st->width == 0 || st->height == 0)  // RenderSession.swift:304-306
@@ -504,7 +455,11 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
     sh.traceGrid = std::max(1u, std::min(full, (sh.nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
     sh.stride = std::max(sh.nSlots, sh.traceGrid * DR_TRACE_BLOCK);
     sh.lay = pathLayout(sh.nSlots, sh.stride);
-    sh.outBytes = size_t(pixels) * 16 * 8;
+    if (st->denoiser == DR_DENOISER_APPLE_SVGF)  // RenderSession.swift:575-576
+        return fail(DR_ERR_INVALID_SCENE, "Apple SVGF denoising is not available on this platform.");
+    if (st->denoiser > DR_DENOISER_LEARNED) return fail(DR_ERR_INVALID_ARGUMENT, "unknown denoiser");
+    sh.planes = st->denoiser != DR_DENOISER_NONE ? 10 : 8;  // + denoised beauty and its ping-pong temporary
+    sh.outBytes = size_t(pixels) * 16 * sh.planes;
     return DR_OK;
 }
 
@@ -548,7 +503,7 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     s->sampleBase = reinterpret_cast<uint32_t *>(b + l.base);
     P.overflow = reinterpret_cast<uint32_t *>(b + l.ovf);
     P.overflowStride = sh.stride;
-    const size_t plane = sh.outBytes / 8;
+    const size_t plane = sh.outBytes / size_t(sh.planes);
     char *o = static_cast<char *>(s->outBlock);
     DrOutputs &O = s->out;
     O.accumulation = reinterpret_cast<float4 *>(o + 0 * plane);
@@ -559,6 +514,9 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     O.objectID = reinterpret_cast<float4 *>(o + 5 * plane);
     O.motion = reinterpret_cast<float4 *>(o + 6 * plane);
     O.triangleID = reinterpret_cast<float4 *>(o + 7 * plane);
+    s->denoised = sh.planes > 8 ? reinterpret_cast<float4 *>(o + 8 * plane) : nullptr;
+    s->denoiseTemp = sh.planes > 8 ? reinterpret_cast<float4 *>(o + 9 * plane) : nullptr;
+    s->denoisedCurrent = false;
     DR_HIP(hipMemsetAsync(s->outBlock, 0, sh.outBytes, s->stream), "clear outputs");
     DR_HIP(hipMemsetAsync(P.stats, 0, DR_CNT_SLOTS * 8, s->stream), "clear stats");
     for (int i = 0; i < DR_STAGE_COUNT; ++i) {
@@ -649,7 +607,7 @@ static DrLaunch makeLaunch(const dr_session *s, uint32_t sampleBegin, uint32_t s
     L.frameSeed = st.frame_seed;
     L.transparentBackground = st.transparent_background ? 1u : 0u;
     L.showsEnvironmentBackground = st.shows_environment_background ? 1u : 0u;
-    L.denoiserEnabled = st.denoiser_enabled ? 1u : 0u;
+    L.denoiserEnabled = st.denoiser_enabled || st.denoiser != DR_DENOISER_NONE ? 1u : 0u;  // RenderSession.swift:941
     L.sampleRadianceClamp = st.sample_radiance_clamp < 0.f ? 64.f : st.sample_radiance_clamp;
     L.backgroundColor[0] = st.background_color[0];
     L.backgroundColor[1] = st.background_color[1];
@@ -753,6 +711,7 @@ static dr_status batchGraph(dr_session *s, uint32_t k, hipGraphExec_t *out) {
 
 static dr_status encodeSamples(dr_session *s, uint32_t samples) {
     DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
+    if (samples > 0) s->denoisedCurrent = false;
     DR_HIP(hipEventRecord(s->evBegin, s->stream), "event");
     const bool graphs = s->useGraphs && !s->stageTiming;
     if (graphs && samples > 0) DR_HIP(dr::launchSetSampleBase(s->sampleBase, s->sampleCount, s->stream), "set base");
@@ -807,6 +766,7 @@ dr_status dr_hip_session_render_tile(dr_session *s, uint32_t x, uint32_t y, uint
     uint32_t tw = std::min(w, W - tx), th = std::min(h, H - ty);
     if (tw == 0 || th == 0) return DR_OK;
     DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
+    s->denoisedCurrent = false;
     DR_HIP(hipEventRecord(s->evBegin, s->stream), "event");
     DrLaunch L = makeLaunch(s, s->sampleCount, 1, tx, ty, tw, th);
     dr_status r = enqueueBatch(s, L);
@@ -820,6 +780,7 @@ dr_status dr_hip_session_render_tile(dr_session *s, uint32_t x, uint32_t y, uint
 dr_status dr_hip_session_reset(dr_session *s) {
     if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
     s->sampleCount = 0;
+    s->denoisedCurrent = false;
     return DR_OK;
 }
 
@@ -829,6 +790,7 @@ dr_status dr_hip_session_update_camera(dr_session *s, const dr_gpu_camera *camer
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : old;
     s->sampleCount = 0;
+    s->denoisedCurrent = false;
     (void)hipStreamSynchronize(s->stream);
     dropGraphs(s);  // the graphs hold the old camera
     return DR_OK;
@@ -840,14 +802,60 @@ dr_status dr_hip_session_sample_count(const dr_session *s, uint32_t *out) {
     return DR_OK;
 }
 
+dr_status dr_hip_session_denoise(dr_session *s) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    if (s->settings.denoiser == DR_DENOISER_NONE || s->denoisedCurrent) return DR_OK;
+    DR_HIP(hipSetDevice(s->scene->device->ordinal), "hipSetDevice");
+    const dr_settings &st = s->settings;
+    if (st.denoiser == DR_DENOISER_SIMPLE_SPATIAL) {
+        // SimpleSpatialDenoiser.encode (Denoising/Denoiser.swift:112-164): 1-5
+        // passes with step 1 << pass, the last one into the denoised plane.
+        const uint32_t n = std::max(1u, std::min(st.denoise_iterations, 5u));
+        const float4 *src = s->out.accumulation;
+        for (uint32_t i = 0; i < n; ++i) {
+            float4 *dst = (n - 1 - i) % 2 == 0 ? s->denoised : s->denoiseTemp;
+            dr::AtrousParams p;
+            p.width = st.width, p.height = st.height;
+            p.radius = std::max(1u, std::min(st.denoise_radius, 2u));
+            p.step = 1u << i;
+            p.normalSigma = std::max(st.denoise_normal_sigma, 0.0001f);
+            p.depthSigma = std::max(st.denoise_depth_sigma, 0.0001f);
+            p.albedoSigma = std::max(st.denoise_albedo_sigma, 0.0001f);
+            p.colorSigma = std::max(st.denoise_color_sigma, 0.0001f);
+            DR_HIP(dr::launchAtrous(src, s->out.depth, s->out.normal, s->out.albedo, dst, p, s->stream), "a-trous pass");
+            src = dst;
+        }
+    } else {
+        if (!s->network) return fail(DR_ERR_INVALID_ARGUMENT, "no denoiser network set on this session");
+        dr_status rc = drUnetDenoiseSession(s->network, s, &s->unetWork, &s->unetWorkBytes);
+        if (rc != DR_OK) return rc;
+    }
+    s->denoisedCurrent = true;
+    return DR_OK;
+}
+
+dr_status dr_hip_session_set_denoiser_network(dr_session *s, dr_unet *net) {
+    if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
+    s->network = net;
+    s->denoisedCurrent = false;
+    return DR_OK;
+}
+
 dr_status dr_hip_session_read_output(dr_session *s, dr_output kind, float *rgba, size_t floats) {
     if (!s || !rgba) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
     const size_t pixels = size_t(s->settings.width) * s->settings.height;
     if (floats < pixels * 4) return fail(DR_ERR_INVALID_ARGUMENT, "output buffer too small");
     if (int(kind) < 0 || int(kind) > 7) return fail(DR_ERR_INVALID_ARGUMENT, "unknown output kind");
+    // The beauty output is the denoised plane when a denoiser is set
+    // (RenderSession.texture(for:), RenderSession.swift:1072-1081).
+    const bool denoised = kind == DR_OUTPUT_BEAUTY && s->settings.denoiser != DR_DENOISER_NONE;
+    if (denoised) {
+        dr_status rc = dr_hip_session_denoise(s);
+        if (rc != DR_OK) return rc;
+    }
     DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
-    const float4 *planes[8] = {s->out.accumulation, s->out.depth, s->out.normal, s->out.albedo,
-                               s->out.materialID, s->out.objectID, s->out.motion, s->out.triangleID};
+    const float4 *planes[8] = {denoised ? s->denoised : s->out.accumulation, s->out.depth, s->out.normal,
+                               s->out.albedo, s->out.materialID, s->out.objectID, s->out.motion, s->out.triangleID};
     DR_HIP(hipMemcpy(rgba, planes[int(kind)], pixels * 16, hipMemcpyDeviceToHost), "readback");
     return DR_OK;
 }

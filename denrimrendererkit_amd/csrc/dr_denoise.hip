@@ -1,0 +1,552 @@
+// dr_denoise.hip -- denoiser kernels (declarations and contracts in dr_denoise.h).
+//
+// The a-trous filter restates simpleSpatialDenoiseKernel
+// (Resources/Shaders/Denoise.metal:39-115). The U-Net kernels are this
+// backend's own: 3x3 convolutions as implicit GEMMs on the gfx950 matrix
+// cores (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation), 64
+// output pixels (an 8x8 tile) per workgroup of four waves, each wave a
+// 16-pixel x cout block; the weight gradient stages an 8x8 pixel tile of the
+// output gradient and of the shifted input in LDS, transposed so that each
+// MFMA operand is one 8-byte LDS read, and reduces over pixels in registers.
+#include <hip/hip_runtime.h>
+
+#include "dr_denoise.h"
+
+namespace dr {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest-even FP32 -> BF16 (NaN stays NaN).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// ---------------------------------------------------------------- a-trous
+
+// B3-spline taps of the 5x5 kernel (Denoise.metal:29-38).
+__device__ __forceinline__ float atrousTap(int o) {
+    o = o < 0 ? -o : o;
+    return o == 0 ? 0.375f : (o == 1 ? 0.25f : 0.0625f);
+}
+
+__device__ __forceinline__ float3 decodeNormal(float4 e) {
+    float3 n = make_float3(e.x * 2.f - 1.f, e.y * 2.f - 1.f, e.z * 2.f - 1.f);
+    float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+    return make_float3(n.x / len, n.y / len, n.z / len);
+}
+
+__device__ __forceinline__ float dist2(float3 a, float3 b) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__global__ __launch_bounds__(256) void k_atrous(const float4 *__restrict__ src, const float4 *__restrict__ depth,
+                                                const float4 *__restrict__ normal, const float4 *__restrict__ albedo,
+                                                float4 *__restrict__ dst, AtrousParams p) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= int(p.width) || y >= int(p.height)) return;
+    const int W = int(p.width), H = int(p.height);
+    const size_t idx = size_t(y) * W + x;
+    const float4 c = src[idx];
+    const float cd = depth[idx].x;
+    if (c.w <= 0.0001f || !isfinite(cd) || cd <= 0.f) {  // Denoise.metal:56-60
+        dst[idx] = c;
+        return;
+    }
+    const float3 cn = decodeNormal(normal[idx]);
+    const float4 ca4 = albedo[idx];
+    const float3 ca = make_float3(ca4.x, ca4.y, ca4.z), cc = make_float3(c.x, c.y, c.z);
+    const int r = int(p.radius), step = int(p.step > 0 ? p.step : 1);
+    const float ns2 = p.normalSigma * p.normalSigma;
+    const float ds = fmaxf(p.depthSigma * fmaxf(cd, 1.f), 0.0001f), ds2 = ds * ds;
+    const float as2 = p.albedoSigma * p.albedoSigma, cs2 = p.colorSigma * p.colorSigma;
+    float sr = 0.f, sg = 0.f, sb = 0.f, tot = 0.f;
+    for (int oy = -r; oy <= r; ++oy) {
+        for (int ox = -r; ox <= r; ++ox) {
+            const int sx = min(max(x + ox * step, 0), W - 1), sy = min(max(y + oy * step, 0), H - 1);
+            const size_t j = size_t(sy) * W + sx;
+            const float4 s = src[j];
+            const float sd = depth[j].x;
+            if (s.w <= 0.0001f || !isfinite(sd) || sd <= 0.f) continue;
+            const float spatial = atrousTap(ox) * atrousTap(oy);
+            const float3 sn = decodeNormal(normal[j]);
+            const float nd = 1.f - fminf(fmaxf(cn.x * sn.x + cn.y * sn.y + cn.z * sn.z, 0.f), 1.f);
+            const float wn = expf(-(nd * nd) / (2.f * ns2));
+            const float dd = sd - cd;
+            const float wd = expf(-(dd * dd) / (2.f * ds2));
+            const float4 sa4 = albedo[j];
+            const float wa = expf(-dist2(ca, make_float3(sa4.x, sa4.y, sa4.z)) / (2.f * as2));
+            const float wc = expf(-dist2(cc, make_float3(s.x, s.y, s.z)) / (2.f * cs2));
+            const float w = spatial * wn * wd * wa * wc;
+            sr += s.x * w;
+            sg += s.y * w;
+            sb += s.z * w;
+            tot += w;
+        }
+    }
+    float dr_ = c.x, dg = c.y, db = c.z;
+    if (tot > 0.f) {
+        dr_ = sr / tot;
+        dg = sg / tot;
+        db = sb / tot;
+    }
+    // mix(centre, denoised, 0.85) with mix(x, y, a) = x + (y - x) * a.
+    dst[idx] = make_float4(c.x + (dr_ - c.x) * 0.85f, c.y + (dg - c.y) * 0.85f, c.z + (db - c.z) * 0.85f, c.w);
+}
+
+// ------------------------------------------------------------ convolution
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs c) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tilesX = (c.W + 7) >> 3, tilesY = (c.H + 7) >> 3;
+    int t = blockIdx.x;
+    const int img = t / (tilesX * tilesY);
+    t -= img * tilesX * tilesY;
+    const int ty = t / tilesX, tx = t - ty * tilesX;
+    // A operand: lane holds row (pixel) lane % 16, k = 4 * (lane / 16) + 0..3.
+    const int pi = lane & 15;
+    const int py = ty * 8 + wave * 2 + (pi >> 3), px = tx * 8 + (pi & 7);
+    const int kq = (lane >> 4) * 4;
+    const int cin = c.ca + c.cb;
+    const int Hb = c.H >> 1, Wb = c.W >> 1;
+    f4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int sy = py + tap / 3 - 1, sx = px + tap % 3 - 1;
+        const bool inb = sy >= 0 && sy < c.H && sx >= 0 && sx < c.W;
+        const uint16_t *pa = c.a, *pb = c.b;
+        if (inb) {
+            pa = c.a + ((size_t(img) * c.H + sy) * c.W + sx) * c.ca + kq;
+            if (c.cb) pb = c.b + ((size_t(img) * Hb + (sy >> 1)) * Wb + (sx >> 1)) * c.cb + kq;
+        }
+        // B operand: lane holds column (output channel) lane % 16, same k.
+        const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin + kq;
+        for (int kc = 0; kc < cin; kc += 16) {
+            s4 av = s4{0, 0, 0, 0};
+            if (inb) av = *reinterpret_cast<const s4 *>(kc < c.ca ? pa + kc : pb + (kc - c.ca));
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const s4 bv = *reinterpret_cast<const s4 *>(wt + size_t(nb) * 16 * cin + kc);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nb], 0, 0, 0);
+            }
+        }
+    }
+    // D: lane holds rows (pixels) 4 * (lane / 16) + j, column lane % 16.
+    const int co0 = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = kq + j;
+        const int oy = ty * 8 + wave * 2 + (i >> 3), ox = tx * 8 + (i & 7);
+        if (oy >= c.H || ox >= c.W) continue;
+        const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int co = nb * 16 + co0;
+            float v = acc[nb][j] + (c.bias ? c.bias[co] : 0.f);
+            if (c.relu) v = fmaxf(v, 0.f);
+            if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
+            else static_cast<uint16_t *>(c.out)[o + co] = f2bf(v);
+        }
+    }
+}
+
+constexpr int kWgRow = 64 + 4;  // LDS row: 64 pixels + 4 elements of padding
+constexpr int kWgMaxQ = 16;     // weight tiles per wave (cout/16 * cin/16 <= 64)
+
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs g) {
+    extern __shared__ uint16_t lds[];
+    const int cin = g.ca + g.cb;
+    uint16_t *sD = lds;                    // [cout][kWgRow]: output gradient, pixel-minor
+    uint16_t *sI = lds + g.cout * kWgRow;  // [cin][kWgRow]: shifted input, pixel-minor
+    const int tap = blockIdx.y, dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kq = (lane >> 4) * 4;
+    const int nbk = cin / 16, tiles = (g.cout / 16) * nbk;
+    const int tilesX = (g.W + 7) >> 3, tilesY = (g.H + 7) >> 3, ntile = g.n * tilesX * tilesY;
+    const int Hb = g.H >> 1, Wb = g.W >> 1;
+    const bool bias = g.db && tap == 4;
+    f4 acc[kWgMaxQ];
+#pragma unroll
+    for (int q = 0; q < kWgMaxQ; ++q) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.f;
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int img = t / (tilesX * tilesY), r = t - img * (tilesX * tilesY);
+        const int ty = r / tilesX, tx = r - ty * tilesX;
+        const int vo = g.cout / 8;
+        for (int e = threadIdx.x; e < 64 * vo; e += 256) {
+            const int p = e / vo, cv = e - p * vo;
+            const int y = ty * 8 + (p >> 3), x = tx * 8 + (p & 7);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y < g.H && x < g.W)
+                v = *reinterpret_cast<const uint4 *>(g.dpre + ((size_t(img) * g.H + y) * g.W + x) * g.cout + cv * 8);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sD[(cv * 8 + k) * kWgRow + p] = uint16_t(w4[k >> 1] >> ((k & 1) * 16));
+        }
+        const int vi = cin / 8;
+        for (int e = threadIdx.x; e < 64 * vi; e += 256) {
+            const int p = e / vi, cv = e - p * vi, c8 = cv * 8;
+            const int y = ty * 8 + (p >> 3) + dy, x = tx * 8 + (p & 7) + dx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < g.H && x >= 0 && x < g.W) {
+                if (c8 < g.ca)
+                    v = *reinterpret_cast<const uint4 *>(g.a + ((size_t(img) * g.H + y) * g.W + x) * g.ca + c8);
+                else
+                    v = *reinterpret_cast<const uint4 *>(g.b + ((size_t(img) * Hb + (y >> 1)) * Wb + (x >> 1)) * g.cb +
+                                                         (c8 - g.ca));
+            }
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sI[(c8 + k) * kWgRow + p] = uint16_t(w4[k >> 1] >> ((k & 1) * 16));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kWgMaxQ; ++q) {
+            const int tt = wave + 4 * q;
+            if (tt < tiles) {
+                const int mb = tt / nbk, nb = tt - mb * nbk;
+                const uint16_t *ra = sD + (mb * 16 + (lane & 15)) * kWgRow + kq;
+                const uint16_t *rb = sI + (nb * 16 + (lane & 15)) * kWgRow + kq;
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    const s4 av = *reinterpret_cast<const s4 *>(ra + ks * 16);
+                    const s4 bv = *reinterpret_cast<const s4 *>(rb + ks * 16);
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+                }
+            }
+        }
+        if (bias && int(threadIdx.x) < g.cout)
+            for (int p = 0; p < 64; ++p) bacc += bf2f(sD[threadIdx.x * kWgRow + p]);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kWgMaxQ; ++q) {
+        const int tt = wave + 4 * q;
+        if (tt < tiles) {
+            const int mb = tt / nbk, nb = tt - mb * nbk;
+            const int ci = nb * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int co = mb * 16 + kq + j;
+                unsafeAtomicAdd(&g.dw[(size_t(tap) * g.cout + co) * cin + ci], acc[q][j]);
+            }
+        }
+    }
+    if (bias && int(threadIdx.x) < g.cout) unsafeAtomicAdd(&g.db[threadIdx.x], bacc);
+}
+
+// ---------------------------------------------------------- elementwise
+
+__global__ void k_pool(const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int n, int H, int W, int C) {
+    const int Ho = H >> 1, Wo = W >> 1, vc = C / 8;
+    const size_t total = size_t(n) * Ho * Wo * vc;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int cv = int(i % vc);
+    size_t rest = i / vc;
+    const int x = int(rest % Wo);
+    rest /= Wo;
+    const int y = int(rest % Ho), img = int(rest / Ho);
+    auto at = [&](int yy, int xx) {
+        return *reinterpret_cast<const uint4 *>(in + ((size_t(img) * H + yy) * W + xx) * C + cv * 8);
+    };
+    const uint4 a = at(2 * y, 2 * x), b = at(2 * y, 2 * x + 1), c = at(2 * y + 1, 2 * x), d = at(2 * y + 1, 2 * x + 1);
+    const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w};
+    const uint32_t wc[4] = {c.x, c.y, c.z, c.w}, wd[4] = {d.x, d.y, d.z, d.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t lohi[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int sh = h * 16;
+            const float v = ((bf2f(uint16_t(wa[k] >> sh)) + bf2f(uint16_t(wb[k] >> sh))) +
+                             (bf2f(uint16_t(wc[k] >> sh)) + bf2f(uint16_t(wd[k] >> sh)))) *
+                            0.25f;
+            lohi[h] = f2bf(v);
+        }
+        o[k] = lohi[0] | (lohi[1] << 16);
+    }
+    *reinterpret_cast<uint4 *>(out + ((size_t(img) * Ho + y) * Wo + x) * C + cv * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ void k_pool_bwd(float *__restrict__ din, const float *__restrict__ dout, int n, int H, int W, int C) {
+    const size_t total = size_t(n) * H * W * C;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = int(i % C);
+    size_t rest = i / C;
+    const int x = int(rest % W);
+    rest /= W;
+    const int y = int(rest % H), img = int(rest / H);
+    din[i] += dout[((size_t(img) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * C + c] * 0.25f;
+}
+
+__global__ void k_grad_slice(float *__restrict__ dst, const float *__restrict__ src, int srcC, int off, int C, int n,
+                             int H, int W, int upsum) {
+    const int Ho = upsum ? H >> 1 : H, Wo = upsum ? W >> 1 : W;
+    const size_t total = size_t(n) * Ho * Wo * C;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = int(i % C);
+    size_t rest = i / C;
+    const int x = int(rest % Wo);
+    rest /= Wo;
+    const int y = int(rest % Ho), img = int(rest / Ho);
+    if (!upsum) {
+        dst[i] += src[((size_t(img) * H + y) * W + x) * srcC + off + c];
+        return;
+    }
+    auto at = [&](int yy, int xx) { return src[((size_t(img) * H + yy) * W + xx) * srcC + off + c]; };
+    dst[i] += (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+}
+
+__global__ void k_relu_bwd(uint16_t *__restrict__ dpre, const float *__restrict__ dout, const uint16_t *__restrict__ out,
+                           size_t count) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    dpre[i] = f2bf(bf2f(out[i]) > 0.f ? dout[i] : 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_loss_l1(const float *__restrict__ pred, const float *__restrict__ target,
+                                                 uint16_t *__restrict__ dpre, float *__restrict__ lossSum,
+                                                 size_t pixels, float scale) {
+    __shared__ float part[4];
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (p < pixels) {
+        uint16_t gv[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) gv[c] = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float d = pred[p * 16 + c] - target[p * 4 + c];
+            l += fabsf(d);
+            gv[c] = f2bf(d > 0.f ? scale : (d < 0.f ? -scale : 0.f));
+        }
+        uint4 *o = reinterpret_cast<uint4 *>(dpre + p * 16);
+        o[0] = make_uint4(gv[0] | (uint32_t(gv[1]) << 16), gv[2] | (uint32_t(gv[3]) << 16), 0u, 0u);
+        o[1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    for (int s = 32; s > 0; s >>= 1) l += __shfl_xor(l, s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(lossSum, ((part[0] + part[1]) + (part[2] + part[3])) * scale);
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void k_weights_bf16(const float *__restrict__ w, uint16_t *__restrict__ wf, uint16_t *__restrict__ wt,
+                               int cout, int cin) {
+    const size_t total = size_t(9) * cout * cin;
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int tap = int(e / (size_t(cout) * cin));
+    const int r = int(e - size_t(tap) * cout * cin), co = r / cin, ci = r - co * cin;
+    const uint16_t h = f2bf(w[e]);
+    wf[e] = h;
+    wt[(size_t(8 - tap) * cin + ci) * cout + co] = h;
+}
+
+__global__ void k_fill(float *p, size_t n, float v) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_to_bf16(const float *__restrict__ s, uint16_t *__restrict__ d, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = f2bf(s[i]);
+}
+
+__global__ void k_pack_features(const float4 *__restrict__ beauty, const float4 *__restrict__ albedo,
+                                const float4 *__restrict__ normal, const float4 *__restrict__ depth, int imgW,
+                                int imgH, float invSpp, uint16_t *__restrict__ xout, int H, int W) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    const size_t i = size_t(min(y, imgH - 1)) * imgW + min(x, imgW - 1);
+    const float4 b = beauty[i], a = albedo[i], nr = normal[i];
+    const float d = depth[i].x;
+    float f[16];
+    f[0] = log1pf(fmaxf(b.x, 0.f) / (a.x + 0.01f));
+    f[1] = log1pf(fmaxf(b.y, 0.f) / (a.y + 0.01f));
+    f[2] = log1pf(fmaxf(b.z, 0.f) / (a.z + 0.01f));
+    f[3] = a.x, f[4] = a.y, f[5] = a.z;
+    f[6] = nr.x, f[7] = nr.y, f[8] = nr.z;
+    f[9] = log1pf(fmaxf(d, 0.f));
+    f[10] = invSpp;
+    f[11] = b.w;
+    f[12] = f[13] = f[14] = f[15] = 0.f;
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = uint32_t(f2bf(f[2 * k])) | (uint32_t(f2bf(f[2 * k + 1])) << 16);
+    uint4 *o = reinterpret_cast<uint4 *>(xout + (size_t(y) * W + x) * 16);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+__global__ void k_pack_target(const float4 *__restrict__ beauty, const float4 *__restrict__ albedo, int imgW, int imgH,
+                              float *__restrict__ t, int H, int W) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    const size_t i = size_t(min(y, imgH - 1)) * imgW + min(x, imgW - 1);
+    const float4 b = beauty[i], a = albedo[i];
+    float4 v;
+    v.x = log1pf(fmaxf(b.x, 0.f) / (a.x + 0.01f));
+    v.y = log1pf(fmaxf(b.y, 0.f) / (a.y + 0.01f));
+    v.z = log1pf(fmaxf(b.z, 0.f) / (a.z + 0.01f));
+    v.w = 0.f;
+    reinterpret_cast<float4 *>(t)[size_t(y) * W + x] = v;
+}
+
+__global__ void k_unpack(const float *__restrict__ yv, const float4 *__restrict__ albedo,
+                         const float4 *__restrict__ beauty, int imgW, int imgH, int W, float4 *__restrict__ out) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= imgW || y >= imgH) return;
+    const size_t i = size_t(y) * imgW + x;
+    const float *p = yv + (size_t(y) * W + x) * 16;
+    const float4 a = albedo[i];
+    // Clamped to e^30 - 1: far beyond any radiance the network is trained on.
+    out[i] = make_float4(expm1f(fminf(fmaxf(p[0], 0.f), 30.f)) * (a.x + 0.01f),
+                         expm1f(fminf(fmaxf(p[1], 0.f), 30.f)) * (a.y + 0.01f),
+                         expm1f(fminf(fmaxf(p[2], 0.f), 30.f)) * (a.z + 0.01f), beauty[i].w);
+}
+
+inline unsigned blocks(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launchAtrous(const float4 *src, const float4 *depth, const float4 *normal, const float4 *albedo, float4 *dst,
+                        const AtrousParams &p, hipStream_t stream) {
+    dim3 grid((p.width + 15) / 16, (p.height + 15) / 16);
+    hipLaunchKernelGGL(k_atrous, grid, dim3(256), 0, stream, src, depth, normal, albedo, dst, p);
+    return hipGetLastError();
+}
+
+hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
+    if (c.ca % 16 || c.cb % 16 || c.cout % 16 || (c.cb && (c.H % 2 || c.W % 2))) return hipErrorInvalidValue;
+    const unsigned grid = unsigned(c.n * ((c.W + 7) / 8) * ((c.H + 7) / 8));
+    if (grid == 0) return hipSuccess;
+    switch (c.cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv3x3<1>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 3: hipLaunchKernelGGL(k_conv3x3<3>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 5: hipLaunchKernelGGL(k_conv3x3<5>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 7: hipLaunchKernelGGL(k_conv3x3<7>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 8: hipLaunchKernelGGL(k_conv3x3<8>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 12: hipLaunchKernelGGL(k_conv3x3<12>, dim3(grid), dim3(256), 0, stream, c); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchWgrad(const WgradArgs &g, hipStream_t stream) {
+    const int cin = g.ca + g.cb;
+    if (g.ca % 16 || g.cb % 16 || g.cout % 16 || (g.cout / 16) * (cin / 16) > 4 * kWgMaxQ ||
+        (g.cb && (g.H % 2 || g.W % 2)))
+        return hipErrorInvalidValue;
+    const int ntile = g.n * ((g.W + 7) / 8) * ((g.H + 7) / 8);
+    if (ntile == 0) return hipSuccess;
+    const unsigned gx = unsigned(ntile < 96 ? ntile : 96);
+    const size_t lds = size_t(g.cout + cin) * kWgRow * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_wgrad, dim3(gx, 9), dim3(256), lds, stream, g);
+    return hipGetLastError();
+}
+
+hipError_t launchPool(const uint16_t *in, uint16_t *out, int n, int H, int W, int C, hipStream_t stream) {
+    const size_t total = size_t(n) * (H / 2) * (W / 2) * (C / 8);
+    if (total) hipLaunchKernelGGL(k_pool, dim3(blocks(total)), dim3(256), 0, stream, in, out, n, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t launchPoolBwd(float *din, const float *dout, int n, int H, int W, int C, hipStream_t stream) {
+    const size_t total = size_t(n) * H * W * C;
+    if (total) hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(total)), dim3(256), 0, stream, din, dout, n, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t launchGradSlice(float *dst, const float *src, int srcC, int off, int C, int n, int H, int W, bool upsum,
+                           hipStream_t stream) {
+    const size_t total = size_t(n) * (upsum ? H / 2 : H) * (upsum ? W / 2 : W) * C;
+    if (total)
+        hipLaunchKernelGGL(k_grad_slice, dim3(blocks(total)), dim3(256), 0, stream, dst, src, srcC, off, C, n, H, W,
+                           upsum ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launchReluBwd(uint16_t *dpre, const float *dout, const uint16_t *out, size_t count, hipStream_t stream) {
+    if (count) hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(count)), dim3(256), 0, stream, dpre, dout, out, count);
+    return hipGetLastError();
+}
+
+hipError_t launchLossL1(const float *pred, const float *target, uint16_t *dpre, float *lossSum, size_t pixels,
+                        float scale, hipStream_t stream) {
+    if (pixels)
+        hipLaunchKernelGGL(k_loss_l1, dim3(blocks(pixels)), dim3(256), 0, stream, pred, target, dpre, lossSum, pixels,
+                           scale);
+    return hipGetLastError();
+}
+
+hipError_t launchAdam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                      float c1, float c2, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, stream, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launchWeightsBf16(const float *w, uint16_t *wf, uint16_t *wt, int cout, int cin, hipStream_t stream) {
+    const size_t total = size_t(9) * cout * cin;
+    hipLaunchKernelGGL(k_weights_bf16, dim3(blocks(total)), dim3(256), 0, stream, w, wf, wt, cout, cin);
+    return hipGetLastError();
+}
+
+hipError_t launchFill(float *p, size_t n, float v, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launchToBf16(const float *src, uint16_t *dst, size_t n, hipStream_t stream) {
+    if (n) hipLaunchKernelGGL(k_to_bf16, dim3(blocks(n)), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launchPackFeatures(const float4 *beauty, const float4 *albedo, const float4 *normal, const float4 *depth,
+                              int imgW, int imgH, float invSpp, uint16_t *x, int H, int W, hipStream_t stream) {
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    hipLaunchKernelGGL(k_pack_features, grid, dim3(256), 0, stream, beauty, albedo, normal, depth, imgW, imgH, invSpp,
+                       x, H, W);
+    return hipGetLastError();
+}
+
+hipError_t launchPackTarget(const float4 *beauty, const float4 *albedo, int imgW, int imgH, float *t, int H, int W,
+                            hipStream_t stream) {
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    hipLaunchKernelGGL(k_pack_target, grid, dim3(256), 0, stream, beauty, albedo, imgW, imgH, t, H, W);
+    return hipGetLastError();
+}
+
+hipError_t launchUnpack(const float *y, const float4 *albedo, const float4 *beauty, int imgW, int imgH, int W,
+                        float4 *out, hipStream_t stream) {
+    dim3 grid((imgW + 15) / 16, (imgH + 15) / 16);
+    hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, stream, y, albedo, beauty, imgW, imgH, W, out);
+    return hipGetLastError();
+}
+
+}  // namespace dr

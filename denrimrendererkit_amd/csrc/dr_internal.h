@@ -1,0 +1,85 @@
+// dr_internal.h -- the C ABI's opaque handles as the library sees them
+// (shared by the session, denoiser and server sources; not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/denrim_hip.h"
+#include "dr_bvh.h"
+#include "dr_kernels.h"
+#include "dr_scene.h"
+
+struct dr_device {
+    int ordinal = 0;
+    int cuCount = 0;
+    std::string arch;
+};
+
+struct dr_scene {
+    dr_device *device = nullptr;
+    DrSceneDev dev{};
+    DrNode *nodes = nullptr;
+    float4 *triPos = nullptr;
+    DrTriangle *triangles = nullptr;
+    DrMaterial *materials = nullptr;
+    DrLight *lights = nullptr;
+    float2 *env = nullptr;
+    uint32_t *envGuide = nullptr;
+    uint4 *texDesc = nullptr;
+    float4 *texels = nullptr;
+    DrVolume *volumes = nullptr;
+    DrVolumeSample *denseSamples = nullptr;
+    DrBrick *bricks = nullptr;
+    uint16_t *dist16 = nullptr;
+    float *dist32 = nullptr;
+    uint2 *sampleMaterial = nullptr;
+    DrBrickGrid *grids = nullptr;
+    uint32_t *gridIndices = nullptr;
+    dr::BvhBuildStats stats;
+    int sessions = 0;
+    bool released = false;
+};
+
+struct dr_session {
+    dr_scene *scene = nullptr;
+    DrSceneDev sceneDev{};  // the scene's device view with this session's quality level
+    dr_settings settings{};
+    DrCamera camera{}, previousCamera{};
+    uint32_t sampleCount = 0;
+    uint32_t samplesPerLaunch = 1;
+    uint32_t nSlots = 0;
+    uint32_t traceGrid = 0;
+    hipStream_t stream = nullptr;
+    DrPaths paths{};
+    DrOutputs out{};
+    void *pathBlock = nullptr;   // one allocation for all path-state arrays
+    void *outBlock = nullptr;    // one allocation for all output planes
+    hipEvent_t evBegin = nullptr, evEnd = nullptr;
+    std::vector<hipEvent_t> stageEvents;
+    bool stageTiming = false;
+    double stageMs[DR_STAGE_COUNT] = {0};
+    uint64_t stageLaunches[DR_STAGE_COUNT] = {0};
+    double lastMs = 0.0;
+    // Batch graphs: full-frame batches captured once per batch size and
+    // replayed; the batch's first sample index lives in sampleBase.
+    bool useGraphs = false;
+    uint32_t *sampleBase = nullptr;
+    size_t pathCapacity = 0, outCapacity = 0;  // bytes of the two blocks
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
+    // Denoised beauty (RenderSession.swift:1115-1168): a plane and a
+    // ping-pong temporary, valid until the next sample or reset.
+    float4 *denoised = nullptr, *denoiseTemp = nullptr;
+    bool denoisedCurrent = false;
+    dr_unet *network = nullptr;  // learned denoiser, when settings.denoiser is DR_DENOISER_LEARNED
+    void *unetWork = nullptr;    // the network's activations for this session's image
+    size_t unetWorkBytes = 0;
+};
+
+// Sets the thread's dr_last_error() message and returns s.
+dr_status drFail(dr_status s, const std::string &msg);
+// Enqueues the U-Net on the session's beauty into session->denoised.
+dr_status drUnetDenoiseSession(dr_unet *net, dr_session *session, void **work, size_t *workBytes);
+

@@ -153,7 +153,12 @@ def settings_struct(settings: RenderSettings, samples_per_launch: int = 0) -> _l
     s.shows_environment_background = int(settings.shows_environment_background)
     for k in range(3):
         s.background_color[k] = float(settings.background_color[k])
-    s.denoiser_enabled = int(bool(settings.denoise))
+    d = settings.denoise
+    s.denoiser_enabled = int(d.denoiser != 0)
+    s.denoiser = int(d.denoiser)
+    s.denoise_radius, s.denoise_iterations = int(d.radius), int(d.iterations)
+    s.denoise_normal_sigma, s.denoise_depth_sigma = float(d.normal_sigma), float(d.depth_sigma)
+    s.denoise_albedo_sigma, s.denoise_color_sigma = float(d.albedo_sigma), float(d.color_sigma)
     s.collects_stats = int(settings.collects_sdf_traversal_stats)
     s.frame_seed = settings.frame_seed & 0xFFFFFFFF
     s.samples_per_launch = samples_per_launch
@@ -225,6 +230,17 @@ class RenderSession:
                    "retarget")
         self.scene, self.settings, self.camera, self.previous_camera = scene, settings, camera, prev_cam
         self._settings = s
+
+    def set_denoiser_network(self, network) -> None:
+        """The UNetDenoiser used when settings.denoise selects LEARNED."""
+        _lib.check(_lib.load().dr_hip_session_set_denoiser_network(self._h, network._h if network else None),
+                   "denoiser network")
+        self._network = network  # kept alive with the session
+
+    def denoise(self) -> None:
+        """Enqueues the configured denoiser on the current beauty (no readback);
+        pixels(BEAUTY) runs it when needed (RenderSession.swift:1115-1168)."""
+        _lib.check(_lib.load().dr_hip_session_denoise(self._h), "denoise")
 
     @property
     def sample_count(self) -> int:

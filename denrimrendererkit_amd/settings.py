@@ -3,7 +3,7 @@ Renderer/RenderOutput.swift:4-25, RenderSession.swift:8-17)."""
 from __future__ import annotations
 
 import enum
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 from .scene import Camera
@@ -42,6 +42,43 @@ class RenderAccelerationMode(enum.Enum):
     HIP = "hip"
 
 
+class RenderDenoiser(enum.IntEnum):
+    """RenderDenoiser (Denoising/Denoiser.swift:8-18) plus this backend's
+    learned U-Net; APPLE_SVGF is refused as on platforms without MPS."""
+    NONE = 0
+    SIMPLE_SPATIAL = 1
+    APPLE_SVGF = 2
+    LEARNED = 3
+
+
+@dataclass
+class DenoiseSettings:
+    """DenoiseSettings (Denoising/Denoiser.swift:20-75), same defaults."""
+    denoiser: RenderDenoiser = RenderDenoiser.NONE
+    radius: int = 2
+    normal_sigma: float = 0.18
+    depth_sigma: float = 0.03
+    albedo_sigma: float = 0.25
+    color_sigma: float = 1.0
+    iterations: int = 3
+
+    @staticmethod
+    def none() -> "DenoiseSettings":
+        return DenoiseSettings()
+
+    @staticmethod
+    def simple_spatial() -> "DenoiseSettings":
+        return DenoiseSettings(denoiser=RenderDenoiser.SIMPLE_SPATIAL)
+
+    @staticmethod
+    def apple_svgf() -> "DenoiseSettings":  # Denoiser.swift:68-76
+        return DenoiseSettings(RenderDenoiser.APPLE_SVGF, 2, 0.18, 0.08, 0.25, 4.0, 5)
+
+    @staticmethod
+    def learned() -> "DenoiseSettings":
+        return DenoiseSettings(denoiser=RenderDenoiser.LEARNED)
+
+
 @dataclass
 class RenderSettings:
     """RenderSettings.swift:29-101 (defaults at :75-87)."""
@@ -53,7 +90,7 @@ class RenderSettings:
     transparent_background: bool = False
     shows_environment_background: bool = True
     background_color: Sequence[float] = (0.0, 0.0, 0.0)
-    denoise: bool = False
+    denoise: DenoiseSettings = field(default_factory=DenoiseSettings)
     sample_radiance_clamp: Optional[float] = None
     collects_sdf_traversal_stats: bool = False
     frame_seed: int = 0x1234ABCD  # RenderSession.swift:917

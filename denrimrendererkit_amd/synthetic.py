@@ -205,6 +205,54 @@ def cornell_room_scene() -> RenderScene:
     return s
 
 
+def room_variant(seed: int) -> RenderScene:
+    """Config-1-style scene family for the learned denoiser: the closed 4x3x4
+    room with a level-4 icosphere and a height-perturbed floor panel, with
+    seeded wall colours, sphere material (Lambertian, metal or one of the
+    six layered kinds), sphere size and place, ceiling-light size and
+    strength, floor-panel roughness and camera pose inside the room."""
+    rng = np.random.Generator(np.random.PCG64(0x5EED0005 * 7919 + seed))
+    u = lambda a, b: float(rng.uniform(a, b))  # noqa: E731
+    origin = (u(-1.4, 1.4), u(0.6, 2.4), u(1.2, 1.9))
+    target = (u(-1.5, 1.5), u(0.3, 1.8), u(-1.9, -0.6))
+    cam = Camera(origin=origin, target=target, vertical_field_of_view_degrees=u(35, 65))
+    s = RenderScene(camera=cam, environment=Environment())
+    grey = u(0.45, 0.8)
+    walls = [s.add_material(Material(base_color=(grey, grey, grey), specular=0.0, roughness=0.5))]
+    for _ in range(2):
+        walls.append(s.add_material(Material(base_color=tuple(float(x) for x in rng.uniform(0.1, 0.85, 3)),
+                                             specular=0.0, roughness=0.5)))
+    quads = [
+        ((-2, 0, -2), (2, 0, -2), (2, 0, 2), (-2, 0, 2)),
+        ((-2, 3, -2), (-2, 3, 2), (2, 3, 2), (2, 3, -2)),
+        ((-2, 0, -2), (-2, 3, -2), (2, 3, -2), (2, 0, -2)),
+        ((-2, 0, 2), (2, 0, 2), (2, 3, 2), (-2, 3, 2)),
+        ((-2, 0, -2), (-2, 0, 2), (-2, 3, 2), (-2, 3, -2)),
+        ((2, 0, -2), (2, 3, -2), (2, 3, 2), (2, 0, 2)),
+    ]
+    for k, q in enumerate(quads):
+        s.add(Mesh.quad(*_facing(q, (0.0, 1.5, 0.0))), walls[1] if k == 4 else (walls[2] if k == 5 else walls[0]))
+    h, lx, lz = u(0.25, 0.7), u(-1.0, 1.0), u(-1.0, 1.0)
+    s.add_quad_light(QuadLight(*_facing(((lx - h, 2.999, lz - h), (lx + h, 2.999, lz - h), (lx + h, 2.999, lz + h),
+                                         (lx - h, 2.999, lz + h)), (lx, 0.0, lz)), color=(1, 1, 1),
+                               intensity=u(8.0, 30.0)))
+    kind = int(rng.integers(0, 8))
+    base = tuple(float(x) for x in rng.uniform(0.15, 0.9, 3))
+    if kind == 6:
+        ball_mat = Material(base_color=base, specular=0.0, roughness=0.5)
+    elif kind == 7:
+        ball_mat = Material(base_color=base, metallic=1.0, roughness=u(0.05, 0.6))
+    else:
+        ball_mat = _layered_material(rng, kind)
+    r = u(0.4, 0.8)
+    s.add(icosphere(4, r), s.add_material(ball_mat), Transform.translation((u(-1.1, 1.1), r, u(-1.4, 0.0))))
+    metal = s.add_material(Material(base_color=(0.9, 0.9, 0.9), metallic=1.0, roughness=u(0.1, 0.6)))
+    heights = 0.02 + rng.normal(0.0, 0.02, size=(41, 41))
+    x0, z0 = u(-1.9, -0.1), u(-1.9, -0.1)
+    s.add(grid_mesh(heights, x0, x0 + 2.0, z0, z0 + 2.0), metal)
+    return s
+
+
 def _layered_material(rng: np.random.Generator, kind: int) -> Material:
     base = tuple(float(x) for x in rng.uniform(0.2, 0.9, 3))
     rough = float(rng.uniform(0.05, 0.7))

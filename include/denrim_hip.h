@@ -116,7 +116,24 @@ typedef struct {
     uint32_t collects_stats;          /* collectsSDFTraversalStats: query / node / triangle counters */
     uint32_t frame_seed;              /* 0x1234ABCD in the reference (RenderSession.swift:917) */
     uint32_t samples_per_launch;      /* samples of one pixel in flight together; 0 = auto */
+    /* DenoiseSettings (Denoising/Denoiser.swift:20-75): the beauty output is
+     * denoised on read when `denoiser` is not DR_DENOISER_NONE (which also
+     * switches the AOVs to the pixel-centre ray, RenderSession.swift:941). */
+    uint32_t denoiser;                /* dr_denoiser */
+    uint32_t denoise_radius;          /* clamped to 1...2 (Denoiser.swift:144) */
+    uint32_t denoise_iterations;      /* clamped to 1...5 (Denoiser.swift:122) */
+    float denoise_normal_sigma, denoise_depth_sigma, denoise_albedo_sigma, denoise_color_sigma;
 } dr_settings;
+
+/* RenderDenoiser (Denoising/Denoiser.swift:8-18) plus this backend's
+ * learned U-Net. Apple SVGF (a closed Apple library) is refused with
+ * DR_ERR_INVALID_SCENE as on platforms without MPS (RenderSession.swift:575-576). */
+typedef enum {
+    DR_DENOISER_NONE = 0,
+    DR_DENOISER_SIMPLE_SPATIAL = 1,   /* simpleSpatialDenoiseKernel (Resources/Shaders/Denoise.metal:39-115) */
+    DR_DENOISER_APPLE_SVGF = 2,
+    DR_DENOISER_LEARNED = 3,          /* the U-Net of a dr_unet set with dr_hip_session_set_denoiser_network */
+} dr_denoiser;
 
 /* Counters read by dr_hip_session_stats (the reference's primary, bounce
  * and shadow scene-query counters, PathTrace.metal:201-203, plus traversal
@@ -276,6 +293,46 @@ dr_status dr_hip_session_estimate_bytes(const dr_scene *scene, const dr_settings
  * launches of one sample batch) and replayed; results are identical to the
  * launch-by-launch path. Invalidated by update_camera. */
 dr_status dr_hip_session_set_graphs(dr_session *session, int enabled);
+
+/* Enqueues the configured denoiser on the session's current beauty
+ * (RenderSession.ensureDenoisedBeautyIsCurrent, RenderSession.swift:1115-1168)
+ * without reading it back; dr_hip_session_read_output(DR_OUTPUT_BEAUTY)
+ * calls it when the denoised plane is not current. */
+dr_status dr_hip_session_denoise(dr_session *session);
+
+/* Learned denoiser: a 5-level U-Net (3x3 convolutions, channels 32-128,
+ * 633,600 weights + biases) on BF16 MFMA with FP32 accumulation, for inference
+ * and training. Input per pixel: demodulated log beauty, albedo, normal,
+ * log depth and 1/spp; output: demodulated log radiance. No reference
+ * counterpart (the reference's denoisers are the a-trous filter and Apple
+ * SVGF). Parameters are FP32 masters in layer order, each layer W[9][cout][cin]
+ * then b[cout]. */
+typedef struct dr_unet dr_unet;
+dr_status dr_hip_unet_create(dr_device *device, uint32_t seed, dr_unet **out);
+dr_status dr_hip_unet_destroy(dr_unet *net);
+dr_status dr_hip_unet_param_count(const dr_unet *net, uint64_t *count);
+dr_status dr_hip_unet_get_params(dr_unet *net, float *params, size_t count);
+dr_status dr_hip_unet_set_params(dr_unet *net, const float *params, size_t count);
+/* A training batch from host arrays: features n*h*w*16 (rounded to BF16),
+ * targets n*h*w*4 (demodulated log radiance in channels 0-2); h, w multiples of 16. */
+dr_status dr_hip_unet_set_batch(dr_unet *net, const float *features, const float *targets, uint32_t n, uint32_t h,
+                                uint32_t w);
+/* Forward + backward on the current batch (mean L1 loss over 3 channels);
+ * gradients are kept for dr_hip_unet_get_grads, parameters unchanged. */
+dr_status dr_hip_unet_forward_backward(dr_unet *net, float *loss);
+dr_status dr_hip_unet_get_grads(dr_unet *net, float *grads, size_t count);
+dr_status dr_hip_unet_get_output(dr_unet *net, float *out, size_t floats);  /* n*h*w*16 of the last forward */
+/* Forward + backward + Adam (beta 0.9 / 0.999, eps 1e-8) on the current batch. */
+dr_status dr_hip_unet_train_step(dr_unet *net, float learning_rate, float *loss);
+/* Device-resident training set built from sessions: each example is a noisy
+ * session's beauty/albedo/normal/depth and a target session's beauty (same
+ * size h x w, camera and scene); training draws batches from it on the device. */
+dr_status dr_hip_unet_dataset_reserve(dr_unet *net, uint32_t capacity, uint32_t h, uint32_t w);
+dr_status dr_hip_unet_dataset_add(dr_unet *net, dr_session *noisy, dr_session *target, uint32_t *index);
+dr_status dr_hip_unet_train_dataset(dr_unet *net, uint32_t steps, uint32_t batch, float learning_rate,
+                                    float final_learning_rate, uint32_t seed, float *losses);
+/* Network used by a session whose settings select DR_DENOISER_LEARNED. */
+dr_status dr_hip_session_set_denoiser_network(dr_session *session, dr_unet *net);
 
 /* Non-blocking completion check of the work encoded so far
  * (MTLCommandBuffer status in the reference). */
