@@ -184,6 +184,46 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
             "mix": "res {96^2:.5, 256^2:.25, 512^2:.15, 1280x720:.1}, spp {16,64,256}, max depth %d" % max_bounces}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def denoiser_bench(renderer, frames, spp=4):
+    """Learned-denoiser inference on a 1920x1080 config-1-style room at
+    `spp` spp: the U-Net forward pass (19 BF16 MFMA convolutions + pooling,
+    feature packing and output) enqueued `frames` times on the session
+    stream, timed by the host clock around a synchronize; FLOPs from the
+    layer table (2 * H * W * 9 * Cin * Cout per convolution)."""
+    import os as _os
+    from denrimrendererkit_amd import RenderQuality, RenderSettings, compile_scene, synthetic
+    from denrimrendererkit_amd.denoise import DEFAULT_WEIGHTS, UNetDenoiser, inference_flops
+    from denrimrendererkit_amd.settings import DenoiseSettings
+    W, H = 1920, 1080
+    scene = synthetic.room_variant(2001)
+    gs = renderer.upload(compile_scene(scene))
+    net = UNetDenoiser.pretrained(renderer) if _os.path.exists(DEFAULT_WEIGHTS) else UNetDenoiser(renderer, seed=0)
+    st = RenderSettings(width=W, height=H, max_bounces=4, quality=RenderQuality.FINAL, frame_seed=0,
+                        denoise=DenoiseSettings.learned())
+    sess = renderer.make_session(gs, st, camera=scene.camera)
+    sess.render(spp)
+    sess.set_denoiser_network(net)
+    sess.denoise()  # warm-up: workspace allocation
+    sess.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sess.set_denoiser_network(net)  # marks the denoised plane stale
+        sess.denoise()
+    sess.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / frames
+    flops = inference_flops(W, H)
+    tflops = flops / (ms * 1e-3) / 1e12
+    sess.close()
+    return {"workload": f"U-Net inference {W}x{H} ({W}x1088 padded), {spp}-spp room input",
+            "ms_per_frame": round(ms, 3), "gflop_per_frame": round(flops / 1e9, 2),
+            "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)},
+            "trained_weights": _os.path.exists(DEFAULT_WEIGHTS)}
+
+
 def cpu_baseline(build, camera, W, H, bounces, target_seconds):
     """The oracle (C++ restatement, FP32, OpenMP over rows) on this host's
     cores over a band of rows of the same frame at 1 spp."""
@@ -222,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the material-preview serving run (0: skip)")
+    ap.add_argument("--denoiser-frames", type=int, default=10,
+                    help="1080p learned-denoiser inferences to time (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -318,6 +360,12 @@ def main():
         serving = serving_bench(renderer, args.serving_requests)
         log(f"serving: {serving}")
 
+    denoiser = None
+    if args.denoiser_frames > 0:
+        sess.close()
+        denoiser = denoiser_bench(renderer, args.denoiser_frames)
+        log(f"denoiser: {denoiser}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(build, scene.camera, W, H, bounces, args.cpu_seconds)
@@ -337,6 +385,8 @@ def main():
            "readback_ms": round(readback_s * 1e3, 3),
            "pcie_inclusive_mrays_per_s": round(queries_per_step * args.steps / (sec + readback_s) / 1e6, 2),
            "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    if denoiser is not None:
+        out["denoiser"] = denoiser
     if serving is not None:
         out["preview_p50_ms"], out["preview_p99_ms"] = serving["p50_ms"], serving["p99_ms"]
         out["serving"] = serving

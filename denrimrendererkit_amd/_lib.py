@@ -31,7 +31,7 @@ EXPORTS = (
     "dr_hip_session_set_denoiser_network", "dr_hip_unet_create", "dr_hip_unet_destroy", "dr_hip_unet_param_count",
     "dr_hip_unet_get_params", "dr_hip_unet_set_params", "dr_hip_unet_set_batch", "dr_hip_unet_forward_backward",
     "dr_hip_unet_get_grads", "dr_hip_unet_get_output", "dr_hip_unet_train_step", "dr_hip_unet_dataset_reserve",
-    "dr_hip_unet_dataset_add", "dr_hip_unet_train_dataset",
+    "dr_hip_unet_dataset_add", "dr_hip_unet_train_dataset", "dr_hip_server_set_denoiser_network",
 )
 
 
@@ -165,6 +165,7 @@ def load():
     L.dr_hip_unet_dataset_reserve.argtypes = [vp, u32, u32, u32]
     L.dr_hip_unet_dataset_add.argtypes = [vp, vp, vp, C.POINTER(u32)]
     L.dr_hip_unet_train_dataset.argtypes = [vp, u32, u32, f32, f32, u32, vp]
+    L.dr_hip_server_set_denoiser_network.argtypes = [vp, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

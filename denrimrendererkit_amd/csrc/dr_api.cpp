@@ -585,6 +585,7 @@ dr_status dr_hip_session_estimate_bytes(const dr_scene *scene, const dr_settings
     dr_status rc = sessionShape(scene, st, sh);
     if (rc != DR_OK) return rc;
     *bytes = sh.lay.bytes + sh.outBytes;
+    if (st->denoiser == DR_DENOISER_LEARNED) *bytes += drUnetWorkBytes(st->width, st->height);
     return DR_OK;
 }
 

@@ -80,6 +80,8 @@ struct dr_session {
 
 // Sets the thread's dr_last_error() message and returns s.
 dr_status drFail(dr_status s, const std::string &msg);
+// Device bytes of the U-Net's activations for one width x height image.
+size_t drUnetWorkBytes(uint32_t width, uint32_t height);
 // Enqueues the U-Net on the session's beauty into session->denoised.
 dr_status drUnetDenoiseSession(dr_unet *net, dr_session *session, void **work, size_t *workBytes);
 

@@ -56,6 +56,7 @@ struct Request {
     std::vector<float> host;  // output planes copied off the device (8 x pixels x 4)
     int readers = 0;          // result() calls reading the session's planes
     uint32_t done = 0, inflight = 0;
+    bool finishing = false;   // samples done, denoiser enqueued
     Clock::time_point tSubmit, tStart, tDone;
 };
 
@@ -75,6 +76,7 @@ struct dr_server {
     std::map<uint64_t, std::unique_ptr<Request>> all;
     uint64_t nextTicket = 1;
     uint64_t allocated = 0;          // bytes of every session the server holds
+    dr_unet *network = nullptr;      // learned denoiser for requests that select it
     bool stop = false;
     std::thread worker;
     int ordinal = 0;
@@ -161,6 +163,7 @@ void dr_server::admit() {
         const uint32_t k = std::max(1u, r->settings.samples_per_launch);
         const bool graphs = cfg.use_graphs && (r->samples + k - 1) / k >= 4u;
         dr_hip_session_set_graphs(r->shell.session, graphs ? 1 : 0);
+        dr_hip_session_set_denoiser_network(r->shell.session, network);
         r->state = DR_REQ_RUNNING;
         r->tStart = Clock::now();
         active.push_back(r);
@@ -195,25 +198,34 @@ void dr_server::run() {
             }
         }
         for (Request *r : act) {
-            if (r->inflight == 0 || r->state != DR_REQ_RUNNING) continue;
+            if ((r->inflight == 0 && !r->finishing) || r->state != DR_REQ_RUNNING) continue;
             int done = 0;
             dr_status q = dr_hip_session_query(r->shell.session, &done);
             if (q == DR_OK && !done) continue;
             progressed = true;
+            if (q == DR_OK && !r->finishing) {
+                r->done += r->inflight;
+                r->inflight = 0;
+                if (r->done < r->samples) continue;
+                if (r->settings.denoiser != DR_DENOISER_NONE) {
+                    // Denoise on the device before the request is done, so a
+                    // result is a plain copy (RenderSession.swift:1115-1168).
+                    q = dr_hip_session_denoise(r->shell.session);
+                    if (q == DR_OK) {
+                        r->finishing = true;
+                        continue;
+                    }
+                }
+            }
             std::lock_guard<std::mutex> lk(mu);
+            active.erase(std::find(active.begin(), active.end(), r));
             if (q != DR_OK) {
-                active.erase(std::find(active.begin(), active.end(), r));
                 fail(r, dr_last_error());
                 continue;
             }
-            r->done += r->inflight;
-            r->inflight = 0;
-            if (r->done >= r->samples) {
-                r->state = DR_REQ_DONE;
-                r->tDone = Clock::now();
-                active.erase(std::find(active.begin(), active.end(), r));
-                doneCv.notify_all();
-            }
+            r->state = DR_REQ_DONE;
+            r->tDone = Clock::now();
+            doneCv.notify_all();
         }
         if (!progressed) {
             std::unique_lock<std::mutex> lk(mu);
@@ -355,6 +367,13 @@ dr_status dr_hip_server_release(dr_server *s, uint64_t ticket) {
     drSceneDrop(r->scene);
     s->all.erase(it);
     s->cv.notify_all();
+    return DR_OK;
+}
+
+dr_status dr_hip_server_set_denoiser_network(dr_server *s, dr_unet *net) {
+    if (!s) return DR_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->network = net;
     return DR_OK;
 }
 

@@ -293,18 +293,29 @@ dr_status ensureLoss(dr_unet *u, uint32_t n) {
 
 // Learned denoising of a session's beauty into session->denoised (enqueued
 // on the session stream; the workspace is the session's own).
+namespace {
+// Inference activations of one imgW x imgH image (padded to multiples of 16).
+size_t inferLayout(int imgW, int imgH, size_t *off) {
+    const int W = (imgW + 15) & ~15, H = (imgH + 15) & ~15;
+    size_t bytes = 0;
+    for (int t = 0; t < NT; ++t) {
+        if (off) off[t] = bytes;
+        bytes += align256(size_t(H >> kTensors[t].level) * (W >> kTensors[t].level) * kTensors[t].c *
+                          (t == Y ? 4 : 2));
+    }
+    return bytes;
+}
+}  // namespace
+
+size_t drUnetWorkBytes(uint32_t width, uint32_t height) { return inferLayout(int(width), int(height), nullptr); }
+
 dr_status drUnetDenoiseSession(dr_unet *u, dr_session *s, void **work, size_t *workBytes) {
     const int imgW = int(s->settings.width), imgH = int(s->settings.height);
     const int W = (imgW + 15) & ~15, H = (imgH + 15) & ~15;
     // The session's inference workspace: activations of one image.
     Workspace w;
-    size_t bytes = 0;
     std::vector<size_t> off(NT);
-    for (int t = 0; t < NT; ++t) {
-        off[t] = bytes;
-        bytes += align256(size_t(H >> kTensors[t].level) * (W >> kTensors[t].level) * kTensors[t].c *
-                          (t == Y ? 4 : 2));
-    }
+    const size_t bytes = inferLayout(imgW, imgH, off.data());
     if (*workBytes < bytes) {
         (void)hipStreamSynchronize(s->stream);
         if (*work) (void)hipFree(*work);

@@ -27,6 +27,21 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data
 
 
+# (input channels, upsampled input channels, output channels, level) of the
+# 19 convolutions, in order (the table of csrc/dr_unet.cpp).
+LAYERS = ((16, 0, 32, 0), (32, 0, 32, 0), (32, 0, 32, 1), (32, 0, 32, 1), (32, 0, 48, 2), (48, 0, 48, 2),
+          (48, 0, 64, 3), (64, 0, 64, 3), (64, 0, 128, 4), (128, 0, 128, 4), (64, 128, 64, 3), (64, 0, 64, 3),
+          (48, 64, 48, 2), (48, 0, 48, 2), (32, 48, 32, 1), (32, 0, 32, 1), (32, 32, 32, 0), (32, 0, 32, 0),
+          (32, 0, 16, 0))
+
+
+def inference_flops(width: int, height: int) -> int:
+    """Multiply-add FLOPs of one forward pass: sum over convolutions of
+    2 * H_l * W_l * 9 * C_in * C_out on the padded (multiple-of-16) image."""
+    W, H = (width + 15) // 16 * 16, (height + 15) // 16 * 16
+    return sum(2 * (H >> lv) * (W >> lv) * 9 * (ca + cb) * co for ca, cb, co, lv in LAYERS)
+
+
 class UNetDenoiser:
     def __init__(self, renderer, seed: int = 0, params: Optional[np.ndarray] = None):
         h = C.c_void_p()

@@ -367,6 +367,12 @@ class RenderServer:
         self._sizes.pop(ticket, None)
         self._keep.pop(ticket, None)
 
+    def set_denoiser_network(self, network) -> None:
+        """Network for requests whose settings select RenderDenoiser.LEARNED."""
+        _lib.check(_lib.load().dr_hip_server_set_denoiser_network(self._h, network._h if network else None),
+                   "denoiser network")
+        self._network = network
+
     def usage(self) -> dict:
         b, a, q = C.c_uint64(), C.c_uint32(), C.c_uint32()
         _lib.check(_lib.load().dr_hip_server_usage(self._h, C.byref(b), C.byref(a), C.byref(q)), "usage")

@@ -359,6 +359,9 @@ dr_status dr_hip_server_result(dr_server *server, uint64_t ticket, dr_output kin
 dr_status dr_hip_server_timing(dr_server *server, uint64_t ticket, double *queued_ms, double *run_ms);
 dr_status dr_hip_server_release(dr_server *server, uint64_t ticket);
 dr_status dr_hip_server_usage(dr_server *server, uint64_t *used_bytes, uint32_t *active, uint32_t *queued);
+/* Network for served requests whose settings select DR_DENOISER_LEARNED;
+ * denoising runs on the device when a request's samples are done. */
+dr_status dr_hip_server_set_denoiser_network(dr_server *server, dr_unet *net);
 
 /* GPU time of the session's work, measured with HIP events on the session
  * stream: `total_ms` of the last render/encode call; per-stage sums since the

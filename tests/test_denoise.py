@@ -26,6 +26,14 @@ def test_unet_layer_table():
         assert ca % 16 == 0 and cb % 16 == 0 and co % 16 == 0 and (co // 16) * ((ca + cb) // 16) <= 64
 
 
+def test_package_layer_table_matches_oracle():
+    from denrimrendererkit_amd.denoise import LAYERS, inference_flops
+    ls = D.layers()
+    assert [(ca, cb, co) for _, ca, cb, co, _, _ in ls] == [(ca, cb, co) for ca, cb, co, _ in LAYERS]
+    assert [D.TENSORS[D.OPS[i][3]][1] for i, *_ in ls] == [lv for *_, lv in LAYERS]
+    assert inference_flops(32, 32) == sum(2 * (32 >> lv) ** 2 * 9 * (ca + cb) * co for ca, cb, co, lv in LAYERS)
+
+
 def test_unet_oracle_backward_equals_finite_differences():
     rng = np.random.default_rng(1)
     p = D.init_params(3)
