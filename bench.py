@@ -80,35 +80,58 @@ def workload(name):
 SERVE_RES = ((96, 96), (256, 256), (512, 512), (1280, 720))
 SERVE_RES_P = (0.5, 0.25, 0.15, 0.1)
 SERVE_SPP = (16, 64, 256)
+# Denoised stream: (resolution, requested spp) -> spp rendered before the
+# learned denoiser, chosen so the denoised relMSE is at or below plain
+# accumulation's at the requested spp on held-out preview scenes
+# (profiles/denoiser_relmse_calibration.md); 0 = no denoiser (plain at the
+# requested spp: at 96^2 the network does not reach the plain relMSE, nor at
+# 256^2 for 256 spp).
+SERVE_DENOISED_SPP = {
+    (96, 96): {16: 0, 64: 0, 256: 0},
+    (256, 256): {16: 1, 64: 2, 256: 0},
+    (512, 512): {16: 1, 64: 2, 256: 8},
+    (1280, 720): {16: 1, 64: 1, 256: 2},
+}
 
 
-def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load=1.25):
+def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load=4.0, verify=8):
     """The material-preview request stream: sphere (20k tris) on a plane
     with seeded layered materials, 1024x512 HDRI; resolution from {96^2,
     256^2, 512^2, 1280x720} w.p. {.5, .25, .15, .1}; spp from {16, 64, 256};
-    Poisson arrivals at `load` x the one-at-a-time service rate (from a
-    pilot). Scenes are resident in HBM before timing. The same arrival
-    schedule is served (a) one request at a time on a plain session and (b)
-    by the render server (continuous batching, graph replays). Both render
-    the same samples, so their images are identical (equal relMSE)."""
+    Poisson arrivals at `load` x the one-at-a-time plain service rate (from a
+    pilot), so every stream runs saturated and images/s is its throughput.
+    Scenes are resident in HBM before timing. The same arrival schedule is
+    served (a) one request at a time on a plain session, (b) by the render
+    server (continuous batching, graph replays) at the same spp (identical
+    images), (c) by the server with the learned denoiser at the spp of
+    SERVE_DENOISED_SPP. relMSE against 4096-spp renders is measured for up to
+    `verify` requests of at most 256^2 in (a) and (c)."""
+    import threading
+
     from denrimrendererkit_amd import RenderQuality, RenderSettings, compile_scene, synthetic
+    from denrimrendererkit_amd.denoise import DEFAULT_WEIGHTS, UNetDenoiser, relmse
     from denrimrendererkit_amd.renderer import RenderServer
+    from denrimrendererkit_amd.settings import DenoiseSettings
     rng = np.random.default_rng(0x5EED0004 + seed)
     scenes = [synthetic.material_preview_scene(k) for k in range(n_scenes)]
     gpu = [renderer.upload(compile_scene(sc)) for sc in scenes]
+    net = UNetDenoiser.pretrained(renderer, DEFAULT_WEIGHTS)
     reqs = []
     for _ in range(n_requests):
         W, H = SERVE_RES[rng.choice(4, p=SERVE_RES_P)]
         reqs.append((int(rng.integers(n_scenes)), W, H, int(SERVE_SPP[rng.integers(3)])))
+    check = [k for k, (_, W, H, _) in enumerate(reqs) if W * H <= 256 * 256][:verify]
 
-    def settings(W, H):
-        return RenderSettings(width=W, height=H, max_bounces=max_bounces, quality=RenderQuality.FINAL, frame_seed=0)
+    def settings(W, H, denoise=False):
+        return RenderSettings(width=W, height=H, max_bounces=max_bounces, quality=RenderQuality.FINAL, frame_seed=0,
+                              denoise=DenoiseSettings.learned() if denoise else DenoiseSettings())
 
     def plain(i, W, H, spp):
         sess = renderer.make_session(gpu[i], settings(W, H), camera=scenes[i].camera)
         sess.render(spp)
-        sess.pixels()
+        px = sess.pixels()
         sess.close()
+        return px
 
     pilot = [reqs[k] for k in range(min(6, n_requests))]
     t0 = time.perf_counter()
@@ -119,69 +142,94 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
     gaps = rng.exponential(1.0 / rate, n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
 
-    # (a) one at a time, in arrival order
-    lat_plain = []
-    t0 = time.perf_counter()
-    for r, a in zip(reqs, arrivals):
-        now = time.perf_counter() - t0
-        if now < a:
-            time.sleep(a - now)
-        plain(*r)
-        lat_plain.append(time.perf_counter() - t0 - a)
-    span_plain = time.perf_counter() - t0
-
-    # (b) served: submit at the arrival times; a collector thread reads each
-    # finished request's beauty plane and releases it (as plain does).
-    import threading
-    server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=16)
-    outstanding, lat_served, errors = {}, [], []
-    lock = threading.Lock()
-    submitted = threading.Event()
-
-    def collect():
-        while True:
-            with lock:
-                items = list(outstanding.items())
-            if not items and submitted.is_set():
-                return
-            for t, a in items:
-                st = server.state(t)
-                if st == "failed":
-                    errors.append(t)
-                if st in ("done", "failed"):
-                    if st == "done":
-                        server.result(t)
-                    server.release(t)
-                    lat_served.append(time.perf_counter() - t0 - a)
-                    with lock:
-                        del outstanding[t]
-            time.sleep(2e-4)
-
-    t0 = time.perf_counter()
-    collector = threading.Thread(target=collect)
-    collector.start()
-    for (i, W, H, spp), a in zip(reqs, arrivals):
-        now = time.perf_counter() - t0
-        if now < a:
-            time.sleep(a - now)
-        with lock:
-            outstanding[server.submit(gpu[i], settings(W, H), scenes[i].camera, spp)] = a
-    submitted.set()
-    collector.join()
-    span_served = time.perf_counter() - t0
-    server.close()
-    if errors:
-        raise RuntimeError("served requests failed: %s" % errors)
-
     def pct(x, q):
         return round(float(np.percentile(np.asarray(x) * 1e3, q)), 2)
-    return {"requests": n_requests, "arrival_rate_per_s": round(rate, 3), "load_vs_plain": load,
-            "images_per_s": round(n_requests / span_served, 3), "p50_ms": pct(lat_served, 50),
-            "p99_ms": pct(lat_served, 99),
-            "plain": {"images_per_s": round(n_requests / span_plain, 3), "p50_ms": pct(lat_plain, 50),
-                      "p99_ms": pct(lat_plain, 99)},
-            "speedup_images_per_s": round(span_plain / span_served, 3),
-            "mix": "res {96^2:.5, 256^2:.25, 512^2:.15, 1280x720:.1}, spp {16,64,256}, max depth %d" % max_bounces}
+
+    # (a) one at a time, in arrival order
+    lat, images = [], {}
+    t0 = time.perf_counter()
+    for k, (r, a) in enumerate(zip(reqs, arrivals)):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        px = plain(*r)
+        if k in check:
+            images[k] = px
+        lat.append(time.perf_counter() - t0 - a)
+    span = time.perf_counter() - t0
+    out = {"plain": {"images_per_s": round(n_requests / span, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}}
+
+    def served(denoise):
+        server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=16)
+        server.set_denoiser_network(net)
+        outstanding, lat, errors, got = {}, [], [], {}
+        lock = threading.Lock()
+        submitted = threading.Event()
+
+        def collect():  # reads each finished request's beauty plane and releases it
+            while True:
+                with lock:
+                    items = list(outstanding.items())
+                if not items and submitted.is_set():
+                    return
+                for t, (k, a) in items:
+                    st = server.state(t)
+                    if st in ("done", "failed"):
+                        if st == "done":
+                            px = server.result(t)
+                            if k in check:
+                                got[k] = px
+                        else:
+                            errors.append(t)
+                        server.release(t)
+                        lat.append(time.perf_counter() - t0 - a)
+                        with lock:
+                            del outstanding[t]
+                time.sleep(2e-4)
+
+        t0 = time.perf_counter()
+        collector = threading.Thread(target=collect)
+        collector.start()
+        for k, ((i, W, H, spp), a) in enumerate(zip(reqs, arrivals)):
+            now = time.perf_counter() - t0
+            if now < a:
+                time.sleep(a - now)
+            d_spp = SERVE_DENOISED_SPP[(W, H)][spp] if denoise else 0
+            st = settings(W, H, denoise=d_spp > 0)
+            t = server.submit(gpu[i], st, scenes[i].camera, d_spp or spp)
+            with lock:
+                outstanding[t] = (k, a)
+        submitted.set()
+        collector.join()
+        span = time.perf_counter() - t0
+        server.close()
+        if errors:
+            raise RuntimeError("served requests failed: %s" % errors)
+        return {"images_per_s": round(n_requests / span, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}, got
+
+    out["served"], _ = served(False)
+    out["served_denoised"], den_images = served(True)
+    # relMSE of the verified requests against 4096-spp renders (another frame seed).
+    rel_plain, rel_den = [], []
+    for k in check:
+        i, W, H, spp = reqs[k]
+        ref = renderer.make_session(gpu[i], RenderSettings(width=W, height=H, max_bounces=max_bounces,
+                                                           quality=RenderQuality.FINAL, frame_seed=99),
+                                    camera=scenes[i].camera)
+        ref.render(4096)
+        target = ref.pixels()
+        ref.close()
+        rel_plain.append(relmse(images[k], target))
+        rel_den.append(relmse(den_images[k], target))
+    out["relmse_verified"] = {"requests": len(check), "plain_mean": round(float(np.mean(rel_plain)), 5),
+                              "served_denoised_mean": round(float(np.mean(rel_den)), 5)}
+    out.update({"requests": n_requests, "arrival_rate_per_s": round(rate, 3), "load_vs_plain": load,
+                "speedup_served": round(out["served"]["images_per_s"] / out["plain"]["images_per_s"], 3),
+                "speedup_served_denoised": round(out["served_denoised"]["images_per_s"] /
+                                                 out["plain"]["images_per_s"], 3),
+                "mix": "res {96^2:.5, 256^2:.25, 512^2:.15, 1280x720:.1}, spp {16,64,256}, max depth %d" %
+                       max_bounces})
+    return out
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -388,7 +436,8 @@ def main():
     if denoiser is not None:
         out["denoiser"] = denoiser
     if serving is not None:
-        out["preview_p50_ms"], out["preview_p99_ms"] = serving["p50_ms"], serving["p99_ms"]
+        out["preview_p50_ms"] = serving["served_denoised"]["p50_ms"]
+        out["preview_p99_ms"] = serving["served_denoised"]["p99_ms"]
         out["serving"] = serving
     print(json.dumps(out), flush=True)
 

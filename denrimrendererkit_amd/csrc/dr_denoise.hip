@@ -3,11 +3,11 @@
 // The a-trous filter restates simpleSpatialDenoiseKernel
 // (Resources/Shaders/Denoise.metal:39-115). The U-Net kernels are this
 // backend's own: 3x3 convolutions as implicit GEMMs on the gfx950 matrix
-// cores (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation), 64
-// output pixels (an 8x8 tile) per workgroup of four waves, each wave a
-// 16-pixel x cout block; the weight gradient stages an 8x8 pixel tile of the
-// output gradient and of the shifted input in LDS, transposed so that each
-// MFMA operand is one 8-byte LDS read, and reduces over pixels in registers.
+// cores (BF16 operands, FP32 accumulation). The forward and data-gradient
+// convolution runs v_mfma_f32_16x16x32_bf16 on 16-pixel row blocks; the
+// weight gradient runs v_mfma_f32_16x16x16_bf16 over 8x8 pixel tiles of the
+// output gradient and the shifted input staged in LDS, transposed so that
+// each MFMA operand is one 8-byte LDS read, reduced over pixels in registers.
 #include <hip/hip_runtime.h>
 
 #include "dr_denoise.h"
@@ -102,58 +102,81 @@ __global__ __launch_bounds__(256) void k_atrous(const float4 *__restrict__ src, 
 
 // ------------------------------------------------------------ convolution
 
-template <int NB>
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// One workgroup = a 16-pixel-wide tile of 4 * MB rows; wave w computes rows
+// w * MB ... w * MB + MB - 1 (one 16-pixel MFMA row block each) for all cout
+// channels (NB blocks of 16). K runs over (tap, 32-channel chunk): the
+// lane's A operand is 8 channels of one pixel (16 bytes), its B operand 8
+// weights of one output channel, and each weight fragment is reused by the
+// MB row blocks. Channel chunks past cin (cin % 32 == 16) read zeros.
+template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs c) {
+    constexpr int TH = 4 * MB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tilesX = (c.W + 7) >> 3, tilesY = (c.H + 7) >> 3;
+    const int tilesX = (c.W + 15) >> 4, tilesY = (c.H + TH - 1) / TH;
     int t = blockIdx.x;
     const int img = t / (tilesX * tilesY);
     t -= img * tilesX * tilesY;
     const int ty = t / tilesX, tx = t - ty * tilesX;
-    // A operand: lane holds row (pixel) lane % 16, k = 4 * (lane / 16) + 0..3.
-    const int pi = lane & 15;
-    const int py = ty * 8 + wave * 2 + (pi >> 3), px = tx * 8 + (pi & 7);
-    const int kq = (lane >> 4) * 4;
+    const int px = tx * 16 + (lane & 15);
+    const int row0 = ty * TH + wave * MB;
+    const int kg = (lane >> 4) * 8;
     const int cin = c.ca + c.cb;
     const int Hb = c.H >> 1, Wb = c.W >> 1;
-    f4 acc[NB];
+    f4 acc[MB][NB];
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
+    const bf8 zero8 = {};
     for (int tap = 0; tap < 9; ++tap) {
-        const int sy = py + tap / 3 - 1, sx = px + tap % 3 - 1;
-        const bool inb = sy >= 0 && sy < c.H && sx >= 0 && sx < c.W;
-        const uint16_t *pa = c.a, *pb = c.b;
-        if (inb) {
-            pa = c.a + ((size_t(img) * c.H + sy) * c.W + sx) * c.ca + kq;
-            if (c.cb) pb = c.b + ((size_t(img) * Hb + (sy >> 1)) * Wb + (sx >> 1)) * c.cb + kq;
-        }
-        // B operand: lane holds column (output channel) lane % 16, same k.
-        const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin + kq;
-        for (int kc = 0; kc < cin; kc += 16) {
-            s4 av = s4{0, 0, 0, 0};
-            if (inb) av = *reinterpret_cast<const s4 *>(kc < c.ca ? pa + kc : pb + (kc - c.ca));
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int sx = px + dx;
+        const bool xin = sx >= 0 && sx < c.W;
+        const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin;
+        for (int kc = 0; kc < cin; kc += 32) {
+            const int k0 = kc + kg;
+            const bool kin = k0 < cin;
+            bf8 bv[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const s4 bv = *reinterpret_cast<const s4 *>(wt + size_t(nb) * 16 * cin + kc);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nb], 0, 0, 0);
+            for (int nb = 0; nb < NB; ++nb)
+                bv[nb] = kin ? *reinterpret_cast<const bf8 *>(wt + size_t(nb) * 16 * cin + k0) : zero8;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const int sy = row0 + mb + dy;
+                bf8 av = zero8;
+                if (kin && xin && sy >= 0 && sy < c.H) {
+                    const uint16_t *src =
+                        k0 < c.ca ? c.a + ((size_t(img) * c.H + sy) * c.W + sx) * c.ca + k0
+                                  : c.b + ((size_t(img) * Hb + (sy >> 1)) * Wb + (sx >> 1)) * c.cb + (k0 - c.ca);
+                    av = *reinterpret_cast<const bf8 *>(src);
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nb], acc[mb][nb], 0, 0, 0);
             }
         }
     }
-    // D: lane holds rows (pixels) 4 * (lane / 16) + j, column lane % 16.
+    // D: lane holds rows (pixels) 4 * (lane / 16) + j, column (channel) lane % 16.
     const int co0 = lane & 15;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int i = kq + j;
-        const int oy = ty * 8 + wave * 2 + (i >> 3), ox = tx * 8 + (i & 7);
-        if (oy >= c.H || ox >= c.W) continue;
-        const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+    for (int mb = 0; mb < MB; ++mb) {
+        const int oy = row0 + mb;
+        if (oy >= c.H) continue;
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int co = nb * 16 + co0;
-            float v = acc[nb][j] + (c.bias ? c.bias[co] : 0.f);
-            if (c.relu) v = fmaxf(v, 0.f);
-            if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
-            else static_cast<uint16_t *>(c.out)[o + co] = f2bf(v);
+        for (int j = 0; j < 4; ++j) {
+            const int ox = tx * 16 + (lane >> 4) * 4 + j;
+            if (ox >= c.W) continue;
+            const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + co0;
+                float v = acc[mb][nb][j] + (c.bias ? c.bias[co] : 0.f);
+                if (c.relu) v = fmaxf(v, 0.f);
+                if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
+                else static_cast<uint16_t *>(c.out)[o + co] = f2bf(v);
+            }
         }
     }
 }
@@ -440,22 +463,27 @@ hipError_t launchAtrous(const float4 *src, const float4 *depth, const float4 *no
     return hipGetLastError();
 }
 
+template <int NB, int MB>
+hipError_t convLaunch(const ConvArgs &c, hipStream_t stream) {
+    const unsigned grid = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 4 * MB - 1) / (4 * MB)));
+    if (grid) hipLaunchKernelGGL((k_conv3x3<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
+    return hipGetLastError();
+}
+
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
     if (c.ca % 16 || c.cb % 16 || c.cout % 16 || (c.cb && (c.H % 2 || c.W % 2))) return hipErrorInvalidValue;
-    const unsigned grid = unsigned(c.n * ((c.W + 7) / 8) * ((c.H + 7) / 8));
-    if (grid == 0) return hipSuccess;
+    // Row blocks per wave: 4 for narrow outputs, fewer as the accumulators grow.
     switch (c.cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv3x3<1>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 2: hipLaunchKernelGGL(k_conv3x3<2>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 3: hipLaunchKernelGGL(k_conv3x3<3>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 4: hipLaunchKernelGGL(k_conv3x3<4>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 5: hipLaunchKernelGGL(k_conv3x3<5>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 7: hipLaunchKernelGGL(k_conv3x3<7>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 8: hipLaunchKernelGGL(k_conv3x3<8>, dim3(grid), dim3(256), 0, stream, c); break;
-    case 12: hipLaunchKernelGGL(k_conv3x3<12>, dim3(grid), dim3(256), 0, stream, c); break;
+    case 1: return convLaunch<1, 4>(c, stream);
+    case 2: return convLaunch<2, 4>(c, stream);
+    case 3: return convLaunch<3, 4>(c, stream);
+    case 4: return convLaunch<4, 4>(c, stream);
+    case 5: return convLaunch<5, 2>(c, stream);
+    case 7: return convLaunch<7, 2>(c, stream);
+    case 8: return convLaunch<8, 2>(c, stream);
+    case 12: return convLaunch<12, 1>(c, stream);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launchWgrad(const WgradArgs &g, hipStream_t stream) {

@@ -143,3 +143,35 @@ def test_request_holds_released_scene(renderer):
         server.release(t)
     finally:
         server.close()
+
+
+def test_denoised_requests_equal_alone_in_mix(renderer):
+    # Served requests with the learned denoiser: denoised on the device when
+    # their samples are done; pixels equal a session with the same network.
+    from denrimrendererkit_amd.denoise import UNetDenoiser
+    from denrimrendererkit_amd.settings import DenoiseSettings
+    net = UNetDenoiser(renderer, seed=11)
+    scenes = [synthetic.material_preview_scene(seed, hdri=(64, 32), segments=20) for seed in (6, 7)]
+    uploaded = [renderer.upload(compile_scene(s)) for s in scenes]
+
+    def st(W, H, den):
+        s = _settings(W, H)
+        if den:
+            s.denoise = DenoiseSettings.learned()
+        return s
+
+    reqs = [(0, 48, 32, 2, True), (1, 64, 64, 4, False), (1, 40, 24, 1, True), (0, 32, 48, 3, True)]
+    server = RenderServer(renderer, memory_budget_bytes=2 << 30, max_active=3)
+    server.set_denoiser_network(net)
+    try:
+        tickets = [server.submit(uploaded[i], st(W, H, d), scenes[i].camera, spp) for i, W, H, spp, d in reqs]
+        for t, (i, W, H, spp, d) in zip(tickets, reqs):
+            assert server.wait(t) == "done"
+            alone = renderer.make_session(uploaded[i], st(W, H, d), camera=scenes[i].camera)
+            if d:
+                alone.set_denoiser_network(net)
+            alone.render(spp)
+            assert np.array_equal(server.result(t), alone.pixels()), (t, d)
+            server.release(t)
+    finally:
+        server.close()
