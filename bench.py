@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--workload", default="terrain_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=24,
+    ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the material-preview serving run (0: skip)")
     ap.add_argument("--denoiser-frames", type=int, default=10,
                     help="1080p learned-denoiser inferences to time (0: skip)")

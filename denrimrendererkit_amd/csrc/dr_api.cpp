@@ -698,6 +698,7 @@ static dr_status batchGraph(dr_session *s, uint32_t k, hipGraphExec_t *out) {
     hipError_t e = r == DR_OK ? dr::launchAdvanceSampleBase(s->sampleBase, k, s->stream) : hipSuccess;
     hipGraph_t graph = nullptr;
     hipError_t ec = hipStreamEndCapture(s->stream, &graph);
+    if ((r != DR_OK || e != hipSuccess || ec != hipSuccess) && graph) (void)hipGraphDestroy(graph);
     if (r != DR_OK) return r;
     DR_HIP(e, "advance launch");
     DR_HIP(ec, "end capture");
