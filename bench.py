@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="terrain_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--intersection", choices=("moller-trumbore", "watertight"), default="moller-trumbore",
+                    help="triangle test of the timed render (the reference's is Moller-Trumbore)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the material-preview serving run (0: skip)")
@@ -344,11 +346,14 @@ def main():
     gscene = renderer.upload(build)
     info = gscene.info()
     log(f"uploaded + BVH: {info} in {time.perf_counter() - t1:.1f} s")
-    settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0)
+    from denrimrendererkit_amd.settings import TriangleIntersection
+    mode = TriangleIntersection.WATERTIGHT if args.intersection == "watertight" else TriangleIntersection.MOLLER_TRUMBORE
+    settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
+                              intersection=mode)
 
     # Counting pass: ray queries, node fetches and triangle tests per step.
     csettings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
-                               collects_sdf_traversal_stats=True)
+                               collects_sdf_traversal_stats=True, intersection=mode)
     cs = renderer.make_session(gscene, csettings, camera=scene.camera, samples_per_launch=1)
     cs.render(1)
     c = cs.stats().astype(np.float64)
@@ -418,7 +423,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(build, scene.camera, W, H, bounces, args.cpu_seconds)
 
-    config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"],
+    config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"], "intersection": args.intersection,
                    "ray_queries_per_step": int(queries_per_step), "rays_per_path": round(queries_per_step / (W * H), 3)})
     if len(build.volume_bricks):
         tot = sdf_ext + sdf_sh

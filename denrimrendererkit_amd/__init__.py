@@ -6,7 +6,8 @@ in ``include/denrim_hip.h``.
 """
 from .scene import (AccelerationBuild, Camera, CameraLens, DenrimRendererError, Environment, Material, Mesh,
                     QuadLight, RenderScene, Texture2D, TextureSamplingMode, Transform, compile_scene)
-from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
+from .settings import (DenoiseSettings, RenderAccelerationMode, RenderDenoiser, RenderOutput, RenderQuality, RenderSettings,
+                       TriangleIntersection)
 
 
 def __getattr__(name):
@@ -21,6 +22,7 @@ def __getattr__(name):
 __all__ = [
     "AccelerationBuild", "Camera", "CameraLens", "DenrimRendererError", "Environment", "Material", "Mesh",
     "QuadLight", "RenderScene", "Texture2D", "TextureSamplingMode", "Transform", "compile_scene",
-    "RenderAccelerationMode", "RenderOutput", "RenderQuality", "RenderSettings",
+    "RenderAccelerationMode", "RenderOutput", "RenderQuality", "RenderSettings", "DenoiseSettings", "RenderDenoiser",
+    "TriangleIntersection",
     "DenrimRenderer", "RenderSession", "GpuScene", "RenderServer",
 ]

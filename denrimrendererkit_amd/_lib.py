@@ -32,6 +32,7 @@ EXPORTS = (
     "dr_hip_unet_get_params", "dr_hip_unet_set_params", "dr_hip_unet_set_batch", "dr_hip_unet_forward_backward",
     "dr_hip_unet_get_grads", "dr_hip_unet_get_output", "dr_hip_unet_train_step", "dr_hip_unet_dataset_reserve",
     "dr_hip_unet_dataset_add", "dr_hip_unet_train_dataset", "dr_hip_server_set_denoiser_network",
+    "dr_hip_scene_intersect",
 )
 
 
@@ -77,7 +78,7 @@ class Settings(C.Structure):
         ("denoiser_enabled", C.c_uint32), ("collects_stats", C.c_uint32), ("frame_seed", C.c_uint32),
         ("samples_per_launch", C.c_uint32), ("denoiser", C.c_uint32), ("denoise_radius", C.c_uint32),
         ("denoise_iterations", C.c_uint32), ("denoise_normal_sigma", C.c_float), ("denoise_depth_sigma", C.c_float),
-        ("denoise_albedo_sigma", C.c_float), ("denoise_color_sigma", C.c_float),
+        ("denoise_albedo_sigma", C.c_float), ("denoise_color_sigma", C.c_float), ("intersection", C.c_uint32),
     ]
 
 
@@ -166,6 +167,7 @@ def load():
     L.dr_hip_unet_dataset_add.argtypes = [vp, vp, vp, C.POINTER(u32)]
     L.dr_hip_unet_train_dataset.argtypes = [vp, u32, u32, f32, f32, u32, vp]
     L.dr_hip_server_set_denoiser_network.argtypes = [vp, vp]
+    L.dr_hip_scene_intersect.argtypes = [vp, vp, u32, u32, vp, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

@@ -412,6 +412,31 @@ dr_status dr_hip_scene_release(dr_scene *scene) {
     return DR_OK;
 }
 
+dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t count, uint32_t intersection, float *t,
+                                 uint32_t *ids) {
+    if (!scene || (count && (!rays || !t || !ids))) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (intersection > DR_INTERSECTION_WATERTIGHT) return fail(DR_ERR_INVALID_ARGUMENT, "unknown intersection test");
+    if (count == 0) return DR_OK;
+    DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
+    DrSceneDev S = scene->dev;
+    S.watertight = intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
+    const size_t lanes = (size_t(count) + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK * DR_TRACE_BLOCK;
+    const size_t bytes = size_t(count) * 32 + size_t(count) * 8 + lanes * (DR_STACK_MAX - DR_LDS_STACK) * 4;
+    char *buf = nullptr;
+    DR_HIP(hipMalloc(&buf, bytes), "intersect buffers");
+    float4 *dRays = reinterpret_cast<float4 *>(buf);
+    float *dT = reinterpret_cast<float *>(buf + size_t(count) * 32);
+    uint32_t *dIds = reinterpret_cast<uint32_t *>(buf + size_t(count) * 36);
+    uint32_t *dOvf = reinterpret_cast<uint32_t *>(buf + size_t(count) * 40);
+    hipError_t e = hipMemcpy(dRays, rays, size_t(count) * 32, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dr::launchIntersect(S, dRays, count, dT, dIds, dOvf, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(t, dT, size_t(count) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ids, dIds, size_t(count) * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    DR_HIP(e, "intersect");
+    return DR_OK;
+}
+
 dr_status dr_hip_scene_info(const dr_scene *s, uint32_t *nodes, uint32_t *leaves, uint32_t *depth, double *secs) {
     if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "scene is NULL");
     if (nodes) *nodes = s->stats.nodeCount;
@@ -458,6 +483,7 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
     if (st->denoiser == DR_DENOISER_APPLE_SVGF)  // RenderSession.swift:575-576
         return fail(DR_ERR_INVALID_SCENE, "Apple SVGF denoising is not available on this platform.");
     if (st->denoiser > DR_DENOISER_LEARNED) return fail(DR_ERR_INVALID_ARGUMENT, "unknown denoiser");
+    if (st->intersection > DR_INTERSECTION_WATERTIGHT) return fail(DR_ERR_INVALID_ARGUMENT, "unknown intersection test");
     sh.planes = st->denoiser != DR_DENOISER_NONE ? 10 : 8;  // + denoised beauty and its ping-pong temporary
     sh.outBytes = size_t(pixels) * 16 * sh.planes;
     return DR_OK;
@@ -471,6 +497,7 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     s->settings = *st;
     s->sceneDev = scene->dev;
     s->sceneDev.sdf.renderQuality = st->quality;  // constants.renderQuality (RenderSettings.swift:106-115)
+    s->sceneDev.watertight = st->intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : s->camera;
     s->samplesPerLaunch = sh.k;

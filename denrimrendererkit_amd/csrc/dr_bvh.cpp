@@ -219,8 +219,8 @@ struct Flattener {
             float fb;
             std::memcpy(&fb, &bits, 4);
             pos.push_back(make_float4(v0.x, v0.y, v0.z, fb));
-            pos.push_back(make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.f));
-            pos.push_back(make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.f));
+            pos.push_back(make_float4(v1.x, v1.y, v1.z, 0.f));
+            pos.push_back(make_float4(v2.x, v2.y, v2.z, 0.f));
         }
         leaves++;
         sah += double(n.box.area()) / rootArea * kTriCost * n.count;

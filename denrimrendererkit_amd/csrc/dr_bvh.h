@@ -20,7 +20,9 @@ struct BvhBuildStats {
 // (reference GPUTriangle records) and writes the traversal layout:
 //   nodes  -- DrNode per inner node, depth-first pre-order, root at 0;
 //   triPos -- 3 float4 per triangle in leaf order: {v0, bits(orig index)},
-//             {v1 - v0, 0}, {v2 - v0, 0}.
+//             {v1, 0}, {v2, 0} (exact vertices: the watertight test needs
+//             them; the Moller-Trumbore test forms v1 - v0, v2 - v0 itself,
+//             the same float subtractions as the reference).
 // Child boxes are widened outward by a few ulps of their magnitude so the
 // fused-multiply-add slab test can never cull a box that the exact test
 // would keep; that keeps the closest hit identical to a brute-force search.

@@ -78,5 +78,10 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
 // Device word of the first sample index of the next batch: set it, or add
 // the batch's sample count after a batch (the last node of a batch graph).
 hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st);
+// Closest triangle hit of `count` rays (2 float4 each: origin, direction):
+// t (+inf on a miss) and the original triangle index (~0 on a miss);
+// overflow holds the traversal-stack spill of ceil(count / 128) * 128 lanes.
+hipError_t launchIntersect(const DrSceneDev &S, const float4 *rays, uint32_t count, float *t, uint32_t *ids,
+                           uint32_t *overflow, hipStream_t st);
 hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t st);
 }

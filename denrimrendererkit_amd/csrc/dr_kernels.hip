@@ -1202,6 +1202,35 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLau
 // --------------------------------------------------------------- launches --
 namespace dr {
 
+// One ray per lane through the same while-while traversal and triangle
+// tests as the path tracer's extension queries (triangle geometry only).
+__global__ __launch_bounds__(DR_TRACE_BLOCK) void k_intersect(DrSceneDev S, const float4 *__restrict__ rays,
+                                                              uint32_t count, float *__restrict__ tOut,
+                                                              uint32_t *__restrict__ idOut, uint32_t *overflow) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * DR_TRACE_BLOCK + threadIdx.x;
+    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, overflow + i, gridDim.x * DR_TRACE_BLOCK};
+    const bool live = i < count;
+    // Lanes past the end trace a ray that misses every box.
+    const float4 o = live ? rays[2 * i] : make_float4(3e38f, 3e38f, 3e38f, 0.f);
+    const float4 d = live ? rays[2 * i + 1] : make_float4(0.f, 0.f, 1.f, 0.f);
+    DrRayPre r = dr_prep(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
+    DrHitRec best{live ? __builtin_inff() : 0.f, 0.f, 0.f, 0xffffffffu};
+    uint32_t nv = 0, nt = 0;
+    dr_traverse<false>(S, r, best, stk, nv, nt);
+    if (live) {
+        tOut[i] = best.id != 0xffffffffu ? best.t : __builtin_inff();
+        idOut[i] = best.id;
+    }
+}
+
+hipError_t launchIntersect(const DrSceneDev &S, const float4 *rays, uint32_t count, float *t, uint32_t *ids,
+                           uint32_t *overflow, hipStream_t st) {
+    const uint32_t blocks = (count + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK;
+    if (blocks) hipLaunchKernelGGL(k_intersect, dim3(blocks), dim3(DR_TRACE_BLOCK), 0, st, S, rays, count, t, ids, overflow);
+    return hipGetLastError();
+}
+
 hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st) {
     hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, st, base, value);
     return hipGetLastError();

@@ -114,6 +114,7 @@ struct DrSceneDev {
     uint32_t envTexIndexPlusOne;
     float envIntensity, envRotationY, envMaxRadiance;
     uint32_t opaqueOnly;         // 1: no material can let a shadow ray through
+    uint32_t watertight;         // 1: watertight triangle test, 0: Moller-Trumbore (the reference's)
     DrSdfDev sdf;                // sparse distance volumes (volumeCount 0: none)
 };
 

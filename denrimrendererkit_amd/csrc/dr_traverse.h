@@ -81,7 +81,7 @@ __device__ __forceinline__ float dr_box(const DrRayPre &r, float x0, float y0, f
 // through to the reference arithmetic, so the accepted set and the values of
 // u, v, t are unchanged.
 __device__ __forceinline__ bool dr_tri(const DrRayPre &r, float4 A, float4 B, float4 C, float &t, float &u, float &v) {
-    vec3 v0 = v3(A.x, A.y, A.z), e1 = v3(B.x, B.y, B.z), e2 = v3(C.x, C.y, C.z);
+    vec3 v0 = v3(A.x, A.y, A.z), e1 = v3(B.x - A.x, B.y - A.y, B.z - A.z), e2 = v3(C.x - A.x, C.y - A.y, C.z - A.z);
     vec3 p = cross(r.d, e2);
     float det = dot(e1, p);
     if (fabsf(det) < 1e-7f) return false;
@@ -102,6 +102,55 @@ __device__ __forceinline__ bool dr_tri(const DrRayPre &r, float4 A, float4 B, fl
     t = tt;
     u = uu;
     v = vv;
+    return true;
+}
+
+__device__ __forceinline__ float dr_comp(vec3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013) with backface culling off:
+// shear and scale so the ray is the +z axis (kz = the dominant direction
+// axis, kx/ky swapped for a negative one to keep the winding), evaluate the
+// three 2-D edge functions, re-evaluate them in double when one is exactly
+// zero, and accept when they share a sign. No ray through a shared edge or
+// vertex slips between the triangles. The reference's t > 0.0005 acceptance
+// and u, v = the weights of v1, v2 are kept so hits shade as with the
+// Moller-Trumbore test.
+__device__ __forceinline__ bool dr_tri_wt(const DrRayPre &r, float4 A4, float4 B4, float4 C4, float &t, float &u,
+                                          float &v) {
+    const float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    const int kz = ay > ax ? (az > ay ? 2 : 1) : (az > ax ? 2 : 0);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    const float dz = dr_comp(r.d, kz);
+    if (dz < 0.0f) {
+        const int tmp = kx;
+        kx = ky;
+        ky = tmp;
+    }
+    const float Sx = dr_comp(r.d, kx) / dz, Sy = dr_comp(r.d, ky) / dz, Sz = 1.0f / dz;
+    const vec3 A = v3(A4.x - r.o.x, A4.y - r.o.y, A4.z - r.o.z);
+    const vec3 B = v3(B4.x - r.o.x, B4.y - r.o.y, B4.z - r.o.z);
+    const vec3 C = v3(C4.x - r.o.x, C4.y - r.o.y, C4.z - r.o.z);
+    const float Az = dr_comp(A, kz), Bz = dr_comp(B, kz), Cz = dr_comp(C, kz);
+    const float Ax = dr_comp(A, kx) - Sx * Az, Ay = dr_comp(A, ky) - Sy * Az;
+    const float Bx = dr_comp(B, kx) - Sx * Bz, By = dr_comp(B, ky) - Sy * Bz;
+    const float Cx = dr_comp(C, kx) - Sx * Cz, Cy = dr_comp(C, ky) - Sy * Cz;
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (Sz * Az) + V * (Sz * Bz) + W * (Sz * Cz);
+    const float rcp = 1.0f / det;
+    const float tt = T * rcp;
+    if (!(tt > 0.0005f)) return false;
+    t = tt;
+    u = V * rcp;
+    v = W * rcp;
     return true;
 }
 
@@ -202,7 +251,7 @@ __device__ __forceinline__ bool dr_leaf_step(const DrSceneDev &S, const DrRayPre
     ++s.leafFirst;
     --s.leafCount;
     float t, u, v;
-    if (dr_tri(r, A, B, C, t, u, v)) {
+    if (S.watertight ? dr_tri_wt(r, A, B, C, t, u, v) : dr_tri(r, A, B, C, t, u, v)) {
         uint32_t id = __float_as_uint(A.w);
         if (t < best.t || (t == best.t && id < best.id)) {
             best.t = t;

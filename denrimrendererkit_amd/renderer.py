@@ -133,6 +133,18 @@ class GpuScene:
         _lib.check(_lib.load().dr_hip_scene_info(self._h, C.byref(n), C.byref(l), C.byref(d), C.byref(s)), "info")
         return {"nodes": n.value, "leaves": l.value, "max_depth": d.value, "build_seconds": s.value}
 
+    def intersect(self, rays: np.ndarray, intersection: int = 0):
+        """Closest triangle hits of rays (N, 8) = (origin, _, direction, _):
+        (t, triangle index), +inf / 0xffffffff on a miss
+        (dr_hip_scene_intersect; CPUIntersector in the reference)."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        t = np.zeros(len(r), np.float32)
+        ids = np.zeros(len(r), np.uint32)
+        if len(r):
+            _lib.check(_lib.load().dr_hip_scene_intersect(self._h, _ptr(r), len(r), int(intersection), _ptr(t),
+                                                          _ptr(ids)), "intersect")
+        return t, ids
+
     def release(self):
         if getattr(self, "_h", None):
             _lib.load().dr_hip_scene_release(self._h)
@@ -159,6 +171,7 @@ def settings_struct(settings: RenderSettings, samples_per_launch: int = 0) -> _l
     s.denoise_radius, s.denoise_iterations = int(d.radius), int(d.iterations)
     s.denoise_normal_sigma, s.denoise_depth_sigma = float(d.normal_sigma), float(d.depth_sigma)
     s.denoise_albedo_sigma, s.denoise_color_sigma = float(d.albedo_sigma), float(d.color_sigma)
+    s.intersection = int(settings.intersection)
     s.collects_stats = int(settings.collects_sdf_traversal_stats)
     s.frame_seed = settings.frame_seed & 0xFFFFFFFF
     s.samples_per_launch = samples_per_launch

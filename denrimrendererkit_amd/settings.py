@@ -42,6 +42,14 @@ class RenderAccelerationMode(enum.Enum):
     HIP = "hip"
 
 
+class TriangleIntersection(enum.IntEnum):
+    """Ray/triangle test: the reference's Moller-Trumbore
+    (PathTrace.metal:1801-1852) or the watertight test of Woop, Benthin and
+    Wald (JCGT 2013), which no ray slips through at a shared edge."""
+    MOLLER_TRUMBORE = 0
+    WATERTIGHT = 1
+
+
 class RenderDenoiser(enum.IntEnum):
     """RenderDenoiser (Denoising/Denoiser.swift:8-18) plus this backend's
     learned U-Net; APPLE_SVGF is refused as on platforms without MPS."""
@@ -94,6 +102,7 @@ class RenderSettings:
     sample_radiance_clamp: Optional[float] = None
     collects_sdf_traversal_stats: bool = False
     frame_seed: int = 0x1234ABCD  # RenderSession.swift:917
+    intersection: TriangleIntersection = TriangleIntersection.MOLLER_TRUMBORE
 
     @property
     def resolved_sample_radiance_clamp(self) -> float:

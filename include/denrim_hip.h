@@ -123,7 +123,18 @@ typedef struct {
     uint32_t denoise_radius;          /* clamped to 1...2 (Denoiser.swift:144) */
     uint32_t denoise_iterations;      /* clamped to 1...5 (Denoiser.swift:122) */
     float denoise_normal_sigma, denoise_depth_sigma, denoise_albedo_sigma, denoise_color_sigma;
+    uint32_t intersection;            /* dr_intersection */
 } dr_settings;
+
+/* Ray/triangle test. The reference's is Moller-Trumbore
+ * (Resources/Shaders/PathTrace.metal:1801-1852), which can let a ray slip
+ * through an edge shared by two triangles; the watertight test (Woop,
+ * Benthin and Wald, JCGT 2013) cannot. Both accept t > 0.0005 and give the
+ * same u, v convention, so shading is unchanged. */
+typedef enum {
+    DR_INTERSECTION_MOLLER_TRUMBORE = 0,
+    DR_INTERSECTION_WATERTIGHT = 1,
+} dr_intersection;
 
 /* RenderDenoiser (Denoising/Denoiser.swift:8-18) plus this backend's
  * learned U-Net. Apple SVGF (a closed Apple library) is refused with
@@ -274,6 +285,15 @@ dr_status dr_hip_session_read_output(dr_session *session, dr_output kind, float 
  * counters accumulated since creation or the last reset_stats. */
 dr_status dr_hip_session_stats(dr_session *session, uint64_t *counters, size_t count);
 dr_status dr_hip_session_reset_stats(dr_session *session);
+
+/* Closest triangle hit of `count` rays (8 floats each: origin xyz, unused,
+ * direction xyz, unused) through the scene's traversal BVH with the chosen
+ * dr_intersection test: t (+inf on a miss) and the triangle index in the
+ * scene's triangle array (~0 on a miss); the lowest index wins ties.
+ * Blocking. The reference's ray query outside a render is CPUIntersector
+ * (Acceleration/CPUIntersector.swift:5-43, brute force on the CPU). */
+dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t count, uint32_t intersection, float *t,
+                                 uint32_t *ids);
 
 /* Rebinds an existing session to another scene, settings and camera,
  * reusing its device memory (no allocation): the session then behaves like

@@ -633,6 +633,45 @@ bool intersectTriangle(const Ray<R> &ray, const or_triangle &tri, R &tOut, R &uO
     vOut = v;
     return true;
 }
+// The watertight test of this build's DR_INTERSECTION_WATERTIGHT mode (no
+// reference counterpart): Woop, Benthin and Wald, "Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013, backface culling off, with the edge
+// functions re-evaluated in double when one is exactly zero; the reference's
+// t > 0.0005 acceptance and u, v = weights of v1, v2.
+template <class R> R comp3(const V3<R> &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+template <class R>
+bool intersectTriangleWatertight(const Ray<R> &ray, const or_triangle &tri, R &tOut, R &uOut, R &vOut) {
+    const V3<R> &d = ray.direction;
+    const R ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    const int kz = ay > ax ? (az > ay ? 2 : 1) : (az > ax ? 2 : 0);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    const R dz = comp3(d, kz);
+    if (dz < R(0)) std::swap(kx, ky);
+    const R Sx = comp3(d, kx) / dz, Sy = comp3(d, ky) / dz, Sz = R(1) / dz;
+    const V3<R> A = v3<R>(tri.v0) - ray.origin, B = v3<R>(tri.v1) - ray.origin, C = v3<R>(tri.v2) - ray.origin;
+    const R Az = comp3(A, kz), Bz = comp3(B, kz), Cz = comp3(C, kz);
+    const R Ax = comp3(A, kx) - Sx * Az, Ay = comp3(A, ky) - Sy * Az;
+    const R Bx = comp3(B, kx) - Sx * Bz, By = comp3(B, ky) - Sy * Bz;
+    const R Cx = comp3(C, kx) - Sx * Cz, Cy = comp3(C, ky) - Sy * Cz;
+    R U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == R(0) || V == R(0) || W == R(0)) {
+        U = R(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = R(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = R(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < R(0) || V < R(0) || W < R(0)) && (U > R(0) || V > R(0) || W > R(0))) return false;
+    const R det = U + V + W;
+    if (det == R(0)) return false;
+    const R T = U * (Sz * Az) + V * (Sz * Bz) + W * (Sz * Cz);
+    const R rcp = R(1) / det;
+    const R t = T * rcp;
+    if (!(t > R(0.0005))) return false;
+    tOut = t;
+    uOut = V * rcp;
+    vOut = W * rcp;
+    return true;
+}
+
 // Attribute part of intersectTriangle (:1842-1850), evaluated for the winner.
 template <class R> void fillTriangleHit(const Ray<R> &ray, const or_triangle &tri, uint32_t index, R t, R u, R v, Hit<R> &h) {
     R w = R(1) - u - v;
@@ -795,7 +834,9 @@ template <class R> Hit<R> intersectScene(Ctx<R> &c, const Ray<R> &ray) {
     auto consider = [&](uint32_t ti) {
         R t, u, v;
         c.cnt[OR_CNT_TRIANGLE_TESTS]++;
-        if (intersectTriangle(ray, s.triangles[ti], t, u, v) && (t < bestT || (t == bestT && ti < best))) {
+        const bool hit = c.st->intersection == 1u ? intersectTriangleWatertight(ray, s.triangles[ti], t, u, v)
+                                                  : intersectTriangle(ray, s.triangles[ti], t, u, v);
+        if (hit && (t < bestT || (t == bestT && ti < best))) {
             bestT = t; bestU = u; bestV = v; best = ti;
         }
     };
@@ -1616,11 +1657,12 @@ int or_render(const or_scene *s, const or_settings *st, const or_camera *cam, co
                           static_cast<float *>(aovs), counters, threads);
 }
 
-int or_intersect(const or_scene *s, const float *rays, uint32_t count, int traversal, float *tOut, uint32_t *idOut,
-                 float *hitOut) {
+int or_intersect(const or_scene *s, const float *rays, uint32_t count, int traversal, int intersection, float *tOut,
+                 uint32_t *idOut, float *hitOut) {
     if (!s) return 1;
     or_settings dummy{};
     dummy.renderQuality = 2u;
+    dummy.intersection = uint32_t(intersection);
     or_camera cam{};
 #pragma omp parallel
     {

@@ -97,6 +97,7 @@ typedef struct {
     float backgroundColor[4];
     uint32_t tileX, tileY, tileWidth, tileHeight;
     uint32_t renderQuality;  // RenderSettings.shaderQualityLevel: 0 preview, 1 interactive, 2 final
+    uint32_t intersection;   // 0 Moller-Trumbore (the reference's), 1 watertight (this build's option)
 } or_settings;
 
 typedef struct {
@@ -156,7 +157,7 @@ int or_render(const or_scene *scene, const or_settings *settings,
  * and, if hitOut is not NULL, 8 floats per ray: shading normal xyz,
  * materialID, materialID2, materialBlend, objectID, frontFacing. */
 int or_intersect(const or_scene *scene, const float *rays, uint32_t count,
-                 int traversal, float *tOut, uint32_t *idOut, float *hitOut);
+                 int traversal, int intersection, float *tOut, uint32_t *idOut, float *hitOut);
 
 /* RNG restatement (PathTrace.metal:264-276). */
 uint32_t or_hash(uint32_t value);

@@ -53,7 +53,7 @@ class _Settings(C.Structure):
         ("environmentRotationY", C.c_float), ("environmentMaxRadiance", C.c_float),
         ("sampleRadianceClamp", C.c_float), ("backgroundColor", C.c_float * 4),
         ("tileX", C.c_uint32), ("tileY", C.c_uint32), ("tileWidth", C.c_uint32), ("tileHeight", C.c_uint32),
-        ("renderQuality", C.c_uint32),
+        ("renderQuality", C.c_uint32), ("intersection", C.c_uint32),
     ]
 
 
@@ -72,7 +72,8 @@ def lib():
         L.or_scene_export_bvh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_render.argtypes = [C.c_void_p, C.POINTER(_Settings), C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                 C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
-        L.or_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
         L.or_hash.argtypes = [C.c_uint32]
         L.or_hash.restype = C.c_uint32
         L.or_random_float.argtypes = [C.POINTER(C.c_uint32)]
@@ -122,7 +123,7 @@ class OracleScene:
         lib().or_scene_export_bvh(self.handle, _ptr(nodes), _ptr(prim))
         return nodes, prim
 
-    def intersect(self, rays: np.ndarray, brute_force: bool = False, with_hit: bool = False):
+    def intersect(self, rays: np.ndarray, brute_force: bool = False, with_hit: bool = False, intersection: int = 0):
         """Closest hits of rays (N, 8) = (origin, _, direction, _): t, primitive
         id (~0 = miss) and, with_hit, (N, 8) = normal xyz, materialID,
         materialID2, materialBlend, objectID, frontFacing."""
@@ -130,7 +131,8 @@ class OracleScene:
         t = np.zeros(len(rays), np.float32)
         ids = np.zeros(len(rays), np.uint32)
         hit = np.zeros((len(rays), 8), np.float32) if with_hit else None
-        lib().or_intersect(self.handle, _ptr(rays), len(rays), 1 if brute_force else 0, _ptr(t), _ptr(ids), _ptr(hit))
+        lib().or_intersect(self.handle, _ptr(rays), len(rays), 1 if brute_force else 0, int(intersection), _ptr(t),
+                           _ptr(ids), _ptr(hit))
         return (t, ids, hit) if with_hit else (t, ids)
 
 
@@ -143,7 +145,7 @@ class OracleResult:
 
 def make_settings(build, width, height, max_bounces=4, frame_seed=0x1234ABCD, clamp=64.0,
                   transparent_background=False, shows_environment_background=True, background=(0, 0, 0),
-                  denoiser=False, tile=None, quality=2) -> _Settings:
+                  denoiser=False, tile=None, quality=2, intersection=0) -> _Settings:
     s = _Settings()
     s.width, s.height, s.maxBounces, s.frameSeed = width, height, max(1, max_bounces), frame_seed
     s.transparentBackground = int(transparent_background)
@@ -160,6 +162,7 @@ def make_settings(build, width, height, max_bounces=4, frame_seed=0x1234ABCD, cl
     x, y, w, h = tile if tile is not None else (0, 0, width, height)
     s.tileX, s.tileY, s.tileWidth, s.tileHeight = x, y, w, h
     s.renderQuality = quality
+    s.intersection = intersection
     return s
 
 

@@ -129,3 +129,59 @@ def test_converged_relmse_vs_fp64_oracle(renderer, name):
     print(f"{name} {W}x{H} {spp} spp: relMSE vs FP64 oracle {same:.3e} (same streams), "
           f"{noise:.3e} (other seed: noise level)")
     assert same < 1e-5 and same < 1e-3 * noise, (same, noise)
+
+
+def _edge_rays_icosphere():
+    sphere = synthetic.icosphere(3, 1.0)
+    v = np.asarray(sphere.vertices, np.float32)
+    tri = np.asarray(sphere.indices, np.int64).reshape(-1, 3)
+    edges = {tuple(sorted((int(a), int(c)))) for t in tri for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))}
+    dirs = np.concatenate([v, np.array([(v[a] + v[c]) * np.float32(0.5) for a, c in sorted(edges)], np.float32)])
+    rays = np.zeros((len(dirs), 8), np.float32)
+    rays[:, 4:7] = dirs
+    return sphere, rays
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ray_queries_match_oracle(renderer, mode):
+    # dr_hip_scene_intersect (GPU BVH) against the oracle's brute force, both
+    # triangle tests: indices and t bit-exact on random rays in the room and
+    # on rays through every vertex and edge midpoint of a closed icosphere.
+    from denrimrendererkit_amd import Material, RenderScene
+    room = compile_scene(synthetic.cornell_room_scene())
+    rng = np.random.default_rng(21)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, 0:3] = rng.uniform([-1.9, 0.1, -1.9], [1.9, 2.9, 1.9], (20000, 3))
+    rays[:, 4:7] = rng.standard_normal((20000, 3))
+    sphere, edge_rays = _edge_rays_icosphere()
+    s = RenderScene()
+    s.add(sphere, s.add_material(Material()))
+    for build, rr in ((room, rays), (compile_scene(s), edge_rays)):
+        t, ids = renderer.upload(build).intersect(rr, intersection=mode)
+        rt, rids = O.OracleScene(build).intersect(rr, brute_force=True, intersection=mode)
+        assert np.array_equal(ids, rids), int((ids != rids).sum())
+        assert np.array_equal(t, rt)
+    misses = int((ids == 0xFFFFFFFF).sum())
+    print(f"{'watertight' if mode else 'Moller-Trumbore'}: {misses} of {len(edge_rays)} icosphere edge/vertex rays miss")
+    if mode == 1:
+        assert misses == 0
+
+
+def test_watertight_render_parity(renderer):
+    from denrimrendererkit_amd.settings import TriangleIntersection
+    scene = synthetic.cornell_room_scene()
+    build = compile_scene(scene)
+    W, H = 160, 90
+    st = RenderSettings(width=W, height=H, max_bounces=4, quality=RenderQuality.FINAL, frame_seed=0,
+                        intersection=TriangleIntersection.WATERTIGHT)
+    sess = renderer.make_session(renderer.upload(build), st, camera=scene.camera)
+    sess.render(1)
+    osc = O.OracleScene(build)
+    ref = O.render(osc, O.make_settings(build, W, H, max_bounces=4, frame_seed=0, clamp=64.0, intersection=1),
+                   scene.camera.gpu_camera(W, H), sample_count=1, brute_force=True)
+    _check_aovs(sess, ref)
+    _check_beauty(sess, ref)
+    mt = O.render(osc, O.make_settings(build, W, H, max_bounces=4, frame_seed=0, clamp=64.0, intersection=0),
+                  scene.camera.gpu_camera(W, H), sample_count=1, brute_force=True)
+    diff = (ref.aovs[6][..., 0].view(np.uint32) != mt.aovs[6][..., 0].view(np.uint32)).mean()
+    print(f"room 160x90 primary triangle IDs, watertight vs Moller-Trumbore: {diff:.4%} differ")
