@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="terrain_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--environment-sampling", choices=("cdf", "alias"), default="cdf",
+                    help="HDRI texel selection of the timed render (the reference's is a CDF search)")
     ap.add_argument("--intersection", choices=("moller-trumbore", "watertight"), default="moller-trumbore",
                     help="triangle test of the timed render (the reference's is Moller-Trumbore)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -348,12 +350,14 @@ def main():
     log(f"uploaded + BVH: {info} in {time.perf_counter() - t1:.1f} s")
     from denrimrendererkit_amd.settings import TriangleIntersection
     mode = TriangleIntersection.WATERTIGHT if args.intersection == "watertight" else TriangleIntersection.MOLLER_TRUMBORE
+    from denrimrendererkit_amd.settings import EnvironmentSampling
+    env_mode = EnvironmentSampling.ALIAS if args.environment_sampling == "alias" else EnvironmentSampling.CDF
     settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
-                              intersection=mode)
+                              intersection=mode, environment_sampling=env_mode)
 
     # Counting pass: ray queries, node fetches and triangle tests per step.
     csettings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
-                               collects_sdf_traversal_stats=True, intersection=mode)
+                               collects_sdf_traversal_stats=True, intersection=mode, environment_sampling=env_mode)
     cs = renderer.make_session(gscene, csettings, camera=scene.camera, samples_per_launch=1)
     cs.render(1)
     c = cs.stats().astype(np.float64)
@@ -424,6 +428,7 @@ def main():
         cpu = cpu_baseline(build, scene.camera, W, H, bounces, args.cpu_seconds)
 
     config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"], "intersection": args.intersection,
+                   "environment_sampling": args.environment_sampling,
                    "ray_queries_per_step": int(queries_per_step), "rays_per_path": round(queries_per_step / (W * H), 3)})
     if len(build.volume_bricks):
         tot = sdf_ext + sdf_sh

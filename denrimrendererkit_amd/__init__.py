@@ -6,7 +6,8 @@ in ``include/denrim_hip.h``.
 """
 from .scene import (AccelerationBuild, Camera, CameraLens, DenrimRendererError, Environment, Material, Mesh,
                     QuadLight, RenderScene, Texture2D, TextureSamplingMode, Transform, compile_scene)
-from .settings import (DenoiseSettings, RenderAccelerationMode, RenderDenoiser, RenderOutput, RenderQuality, RenderSettings,
+from .settings import (DenoiseSettings, EnvironmentSampling, RenderAccelerationMode, RenderDenoiser, RenderOutput,
+                       RenderQuality, RenderSettings,
                        TriangleIntersection)
 
 
@@ -23,6 +24,6 @@ __all__ = [
     "AccelerationBuild", "Camera", "CameraLens", "DenrimRendererError", "Environment", "Material", "Mesh",
     "QuadLight", "RenderScene", "Texture2D", "TextureSamplingMode", "Transform", "compile_scene",
     "RenderAccelerationMode", "RenderOutput", "RenderQuality", "RenderSettings", "DenoiseSettings", "RenderDenoiser",
-    "TriangleIntersection",
+    "TriangleIntersection", "EnvironmentSampling",
     "DenrimRenderer", "RenderSession", "GpuScene", "RenderServer",
 ]

@@ -79,6 +79,7 @@ class Settings(C.Structure):
         ("samples_per_launch", C.c_uint32), ("denoiser", C.c_uint32), ("denoise_radius", C.c_uint32),
         ("denoise_iterations", C.c_uint32), ("denoise_normal_sigma", C.c_float), ("denoise_depth_sigma", C.c_float),
         ("denoise_albedo_sigma", C.c_float), ("denoise_color_sigma", C.c_float), ("intersection", C.c_uint32),
+        ("environment_sampling", C.c_uint32),
     ]
 
 

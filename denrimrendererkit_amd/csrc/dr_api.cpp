@@ -150,6 +150,7 @@ static void freeScene(dr_scene *s) {
     devFree(s->lights);
     devFree(s->env);
     devFree(s->envGuide);
+    devFree(s->envAlias);
     devFree(s->texDesc);
     devFree(s->texels);
     devFree(s->volumes);
@@ -304,11 +305,43 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
         guide[guideSize] = n - 1;
     }
 
+    // Alias table of the same texel distribution (Walker's method, Vose's
+    // construction in double): worklists of under- and over-full texels in
+    // index order, processed from the back; leftovers keep probability 1.
+    std::vector<uint2> alias;
+    if (d->environment_sample_count > 1) {
+        const uint32_t n = d->environment_sample_count;
+        const auto *env = static_cast<const float2 *>(d->environment_samples);
+        alias.resize(n);
+        std::vector<double> scaled(n);
+        std::vector<float> prob(n, 1.0f);
+        std::vector<uint32_t> small, large;
+        double prev = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const double c = double(env[i].x);
+            scaled[i] = std::max(c - prev, 0.0) * double(n);
+            prev = std::max(prev, c);
+            alias[i].y = i;
+            (scaled[i] < 1.0 ? small : large).push_back(i);
+        }
+        while (!small.empty() && !large.empty()) {
+            const uint32_t l = small.back(), g = large.back();
+            small.pop_back();
+            large.pop_back();
+            prob[l] = float(scaled[l]);
+            alias[l].y = g;
+            scaled[g] = (scaled[g] + scaled[l]) - 1.0;
+            (scaled[g] < 1.0 ? small : large).push_back(g);
+        }
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(&alias[i].x, &prob[i], 4);
+    }
+
     hipError_t e = hipSuccess;
     auto chain = [&](hipError_t x) {
         if (e == hipSuccess) e = x;
     };
     chain(upload(&s->nodes, nodes.data(), nodes.size()));
+    chain(upload(&s->envAlias, alias.data(), alias.size()));
     chain(upload(&s->triPos, triPos.data(), triPos.size()));
     chain(upload(&s->triangles, d->triangles, d->triangle_count));
     chain(upload(&s->materials, d->materials, d->material_count));
@@ -339,6 +372,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     g.lights = s->lights;
     g.env = s->env;
     g.envGuide = s->envGuide;
+    g.envAlias = s->envAlias;
     g.envGuideSize = guideSize;
     g.texDesc = s->texDesc;
     g.texels = s->texels;
@@ -484,6 +518,8 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
         return fail(DR_ERR_INVALID_SCENE, "Apple SVGF denoising is not available on this platform.");
     if (st->denoiser > DR_DENOISER_LEARNED) return fail(DR_ERR_INVALID_ARGUMENT, "unknown denoiser");
     if (st->intersection > DR_INTERSECTION_WATERTIGHT) return fail(DR_ERR_INVALID_ARGUMENT, "unknown intersection test");
+    if (st->environment_sampling > DR_ENV_SAMPLING_ALIAS)
+        return fail(DR_ERR_INVALID_ARGUMENT, "unknown environment sampling");
     sh.planes = st->denoiser != DR_DENOISER_NONE ? 10 : 8;  // + denoised beauty and its ping-pong temporary
     sh.outBytes = size_t(pixels) * 16 * sh.planes;
     return DR_OK;
@@ -498,6 +534,7 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     s->sceneDev = scene->dev;
     s->sceneDev.sdf.renderQuality = st->quality;  // constants.renderQuality (RenderSettings.swift:106-115)
     s->sceneDev.watertight = st->intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
+    s->sceneDev.envAliasSampling = st->environment_sampling == DR_ENV_SAMPLING_ALIAS ? 1u : 0u;
     s->camera = toCam(*camera);
     s->previousCamera = prev ? toCam(*prev) : s->camera;
     s->samplesPerLaunch = sh.k;

@@ -28,6 +28,7 @@ struct dr_scene {
     DrLight *lights = nullptr;
     float2 *env = nullptr;
     uint32_t *envGuide = nullptr;
+    uint2 *envAlias = nullptr;
     uint4 *texDesc = nullptr;
     float4 *texels = nullptr;
     DrVolume *volumes = nullptr;

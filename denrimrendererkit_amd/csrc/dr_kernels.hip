@@ -665,8 +665,9 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
                     float px = randomFloat(state);
                     float py = randomFloat(state);
                     float cdf = randomFloat(state);
+                    const float coin = S.envAliasSampling ? randomFloat(state) : 0.0f;  // alias mode draws one more
                     float envPDF;
-                    vec3 dir = sampleEnvironmentDirection(S, px, py, cdf, envPDF);
+                    vec3 dir = sampleEnvironmentDirection(S, px, py, cdf, coin, envPDF);
                     float cosS = fmaxf(0.0f, dot(h.normal, dir));
                     if (envPDF > 0.0f && cosS > 0.0f) {
                         float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, dir);

@@ -108,6 +108,7 @@ struct DrSceneDev {
     const DrLight *lights;
     const float2 *env;           // (cdf, pdf) per environment texel
     const uint32_t *envGuide;    // envGuideSize + 1 entries: first texel with cdf >= k / envGuideSize
+    const uint2 *envAlias;       // alias table per texel: (bits(probability), alias texel)
     const uint4 *texDesc;        // (offset, width, height, mode)
     const float4 *texels;
     uint32_t triangleCount, materialCount, lightCount, envCount, texCount, envGuideSize;
@@ -115,6 +116,7 @@ struct DrSceneDev {
     float envIntensity, envRotationY, envMaxRadiance;
     uint32_t opaqueOnly;         // 1: no material can let a shadow ray through
     uint32_t watertight;         // 1: watertight triangle test, 0: Moller-Trumbore (the reference's)
+    uint32_t envAliasSampling;   // 1: HDRI texels drawn from the alias table, 0: CDF search (the reference's)
     DrSdfDev sdf;                // sparse distance volumes (volumeCount 0: none)
 };
 

@@ -500,7 +500,7 @@ __device__ __forceinline__ float environmentPDF(const DrSceneDev &S, vec3 d) {  
     return fmaxf(S.env[index].y, 0.0f);
 }
 __device__ __forceinline__ vec3 sampleEnvironmentDirection(const DrSceneDev &S, float px, float py, float cdfSample,
-                                                           float &pdf) {  // :3952-4003
+                                                           float coin, float &pdf) {  // :3952-4003
     pdf = 0.0f;
     if (S.envTexIndexPlusOne == 0u || S.envCount == 0u || S.envTexIndexPlusOne > S.texCount) return v3(0, 1, 0);
     uint4 td = S.texDesc[S.envTexIndexPlusOne - 1u];
@@ -513,7 +513,14 @@ __device__ __forceinline__ vec3 sampleEnvironmentDirection(const DrSceneDev &S, 
     // only inside that bracket. G is a power of two, so target * G and k / G
     // are exact and the index is the one the full search returns.
     uint32_t index = 0u;
-    if (S.envCount > 1u) {
+    if (S.envCount > 1u && S.envAliasSampling) {
+        // Alias table (this backend's option): the CDF sample picks a
+        // column, the coin keeps it or takes its alias -- the same texel
+        // probabilities in O(1).
+        index = min((uint32_t)(cdfSample * (float)S.envCount), S.envCount - 1u);
+        const uint2 a = S.envAlias[index];
+        if (!(coin < __uint_as_float(a.x))) index = a.y;
+    } else if (S.envCount > 1u) {
         float target = clampf(cdfSample, 0.0f, 0.99999994f);
         uint32_t lo = 0u, hi = S.envCount - 1u;
         if (S.envGuideSize > 0u) {

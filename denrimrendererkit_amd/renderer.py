@@ -172,6 +172,7 @@ def settings_struct(settings: RenderSettings, samples_per_launch: int = 0) -> _l
     s.denoise_normal_sigma, s.denoise_depth_sigma = float(d.normal_sigma), float(d.depth_sigma)
     s.denoise_albedo_sigma, s.denoise_color_sigma = float(d.albedo_sigma), float(d.color_sigma)
     s.intersection = int(settings.intersection)
+    s.environment_sampling = int(settings.environment_sampling)
     s.collects_stats = int(settings.collects_sdf_traversal_stats)
     s.frame_seed = settings.frame_seed & 0xFFFFFFFF
     s.samples_per_launch = samples_per_launch

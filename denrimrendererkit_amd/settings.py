@@ -50,6 +50,14 @@ class TriangleIntersection(enum.IntEnum):
     WATERTIGHT = 1
 
 
+class EnvironmentSampling(enum.IntEnum):
+    """How NEE picks an HDRI texel: the reference's CDF search
+    (PathTrace.metal:3929-3950) or an alias table of the same distribution
+    (one more random number per sample; equal in the limit)."""
+    CDF = 0
+    ALIAS = 1
+
+
 class RenderDenoiser(enum.IntEnum):
     """RenderDenoiser (Denoising/Denoiser.swift:8-18) plus this backend's
     learned U-Net; APPLE_SVGF is refused as on platforms without MPS."""
@@ -103,6 +111,7 @@ class RenderSettings:
     collects_sdf_traversal_stats: bool = False
     frame_seed: int = 0x1234ABCD  # RenderSession.swift:917
     intersection: TriangleIntersection = TriangleIntersection.MOLLER_TRUMBORE
+    environment_sampling: EnvironmentSampling = EnvironmentSampling.CDF
 
     @property
     def resolved_sample_radiance_clamp(self) -> float:

@@ -124,7 +124,18 @@ typedef struct {
     uint32_t denoise_iterations;      /* clamped to 1...5 (Denoiser.swift:122) */
     float denoise_normal_sigma, denoise_depth_sigma, denoise_albedo_sigma, denoise_color_sigma;
     uint32_t intersection;            /* dr_intersection */
+    uint32_t environment_sampling;    /* dr_environment_sampling */
 } dr_settings;
+
+/* How NEE picks an HDRI texel. The reference searches the CDF
+ * (Resources/Shaders/PathTrace.metal:3929-3950); the alias table (Walker,
+ * Vose) draws the same texel distribution in O(1) with one more random
+ * number per sample, so its images differ sample by sample and agree in
+ * the limit (judged on converged relMSE). */
+typedef enum {
+    DR_ENV_SAMPLING_CDF = 0,
+    DR_ENV_SAMPLING_ALIAS = 1,
+} dr_environment_sampling;
 
 /* Ray/triangle test. The reference's is Moller-Trumbore
  * (Resources/Shaders/PathTrace.metal:1801-1852), which can let a ray slip

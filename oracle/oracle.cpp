@@ -721,6 +721,8 @@ struct or_scene {
     std::vector<or_material> materials;
     std::vector<or_light> lights;
     std::vector<or_env_sample> env;
+    std::vector<float> envAliasProb;      // alias table of the environment CDF (this build's sampling option)
+    std::vector<uint32_t> envAliasIndex;
     std::vector<or_texture_desc> textures;
     std::vector<or_float4> texels;
     std::vector<Node> nodes;
@@ -1021,7 +1023,38 @@ template <class R> uint32_t selectEnvironmentSampleIndex(const or_env_sample *e,
     return low;
 }
 
-template <class R> V3<R> sampleEnvironmentDirection(const Ctx<R> &c, R px, R py, R cdfSample, R &pdf) {  // :3952-4003
+// Walker's alias method with Vose's construction (M. D. Vose, "A linear
+// algorithm for generating random numbers with a given distribution", IEEE
+// TSE 17(9), 1991) over the texel probabilities cdf[i] - cdf[i-1]: worklists
+// of under- and over-full texels in index order, processed from the back,
+// in double; leftovers get probability 1. The same construction as the
+// library's, so the tables agree bit for bit.
+void buildEnvironmentAlias(const std::vector<or_env_sample> &env, std::vector<float> &prob, std::vector<uint32_t> &alias) {
+    const size_t n = env.size();
+    prob.assign(n, 1.0f);
+    alias.resize(n);
+    std::vector<double> scaled(n);
+    std::vector<uint32_t> small, large;
+    double prev = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double c = double(env[i].cdf);
+        scaled[i] = std::max(c - prev, 0.0) * double(n);
+        prev = std::max(prev, c);
+        alias[i] = uint32_t(i);
+        (scaled[i] < 1.0 ? small : large).push_back(uint32_t(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t l = small.back(), g = large.back();
+        small.pop_back();
+        large.pop_back();
+        prob[l] = float(scaled[l]);
+        alias[l] = g;
+        scaled[g] = (scaled[g] + scaled[l]) - 1.0;
+        (scaled[g] < 1.0 ? small : large).push_back(g);
+    }
+}
+
+template <class R> V3<R> sampleEnvironmentDirection(const Ctx<R> &c, R px, R py, R cdfSample, R coin, R &pdf) {  // :3952-4003
     pdf = R(0);
     const or_settings &st = *c.st;
     uint32_t count = uint32_t(c.s->env.size());
@@ -1030,7 +1063,14 @@ template <class R> V3<R> sampleEnvironmentDirection(const Ctx<R> &c, R px, R py,
     const or_texture_desc &td = c.s->textures[st.environmentTextureIndexPlusOne - 1u];
     uint32_t width = td.width, height = td.height;
     if (width == 0u || height == 0u) return V3<R>(0, 1, 0);
-    uint32_t index = selectEnvironmentSampleIndex(c.s->env.data(), count, cdfSample);
+    uint32_t index;
+    if (st.environmentSampling == 1u && !c.s->envAliasProb.empty()) {
+        // Alias table: a column from the CDF sample, the coin picks it or its alias.
+        index = std::min(uint32_t(float(cdfSample) * float(count)), count - 1u);
+        if (!(float(coin) < c.s->envAliasProb[index])) index = c.s->envAliasIndex[index];
+    } else {
+        index = selectEnvironmentSampleIndex(c.s->env.data(), count, cdfSample);
+    }
     index = std::min(index, count - 1u);
     uint32_t x = index % width, y = index / width;
     R u = (R(x) + px) / R(width);
@@ -1204,8 +1244,9 @@ template <class R> V3<R> sampleDirectLighting(Ctx<R> &c, const Hit<R> &h, const 
         R px = randomFloat<R>(state);
         R py = randomFloat<R>(state);
         R cdf = randomFloat<R>(state);
+        R coin = c.st->environmentSampling == 1u ? randomFloat<R>(state) : R(0);  // alias mode draws one more
         R envPDF;
-        V3<R> dir = sampleEnvironmentDirection(c, px, py, cdf, envPDF);
+        V3<R> dir = sampleEnvironmentDirection(c, px, py, cdf, coin, envPDF);
         R cosS = fmx(R(0), dot(h.normal, dir));
         if (envPDF > R(0) && cosS > R(0)) {
             R bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, dir);
@@ -1603,6 +1644,7 @@ int or_scene_create(const or_scene_desc *d, uint32_t maxLeaf, or_scene **out) {
     s->materials.assign(d->materials, d->materials + d->materialCount);
     if (d->lightCount) s->lights.assign(d->lights, d->lights + d->lightCount);
     if (d->envSampleCount) s->env.assign(d->envSamples, d->envSamples + d->envSampleCount);
+    if (s->env.size() > 1) buildEnvironmentAlias(s->env, s->envAliasProb, s->envAliasIndex);
     if (d->textureCount) s->textures.assign(d->textureDescs, d->textureDescs + d->textureCount);
     if (d->texturePixelCount) s->texels.assign(d->texturePixels, d->texturePixels + d->texturePixelCount);
     if (d->volumeCount) s->volumes.assign(d->volumes, d->volumes + d->volumeCount);
@@ -1655,6 +1697,13 @@ int or_render(const or_scene *s, const or_settings *st, const or_camera *cam, co
                                static_cast<double *>(aovs), counters, threads);
     return renderT<float>(s, st, cam, pc, sampleBegin, sampleCount, traversal, static_cast<float *>(acc),
                           static_cast<float *>(aovs), counters, threads);
+}
+
+int or_environment_alias(const or_scene *s, float *prob, uint32_t *alias, uint32_t count) {
+    if (!s || count != s->envAliasProb.size()) return 1;
+    std::copy(s->envAliasProb.begin(), s->envAliasProb.end(), prob);
+    std::copy(s->envAliasIndex.begin(), s->envAliasIndex.end(), alias);
+    return 0;
 }
 
 int or_intersect(const or_scene *s, const float *rays, uint32_t count, int traversal, int intersection, float *tOut,

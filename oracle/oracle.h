@@ -98,6 +98,7 @@ typedef struct {
     uint32_t tileX, tileY, tileWidth, tileHeight;
     uint32_t renderQuality;  // RenderSettings.shaderQualityLevel: 0 preview, 1 interactive, 2 final
     uint32_t intersection;   // 0 Moller-Trumbore (the reference's), 1 watertight (this build's option)
+    uint32_t environmentSampling;  // 0 CDF search (the reference's), 1 alias table (this build's option)
 } or_settings;
 
 typedef struct {
@@ -158,6 +159,10 @@ int or_render(const or_scene *scene, const or_settings *settings,
  * materialID, materialID2, materialBlend, objectID, frontFacing. */
 int or_intersect(const or_scene *scene, const float *rays, uint32_t count,
                  int traversal, int intersection, float *tOut, uint32_t *idOut, float *hitOut);
+
+/* The alias table built for the scene's environment (count = environment
+ * sample count, > 1), for tests. */
+int or_environment_alias(const or_scene *scene, float *prob, uint32_t *alias, uint32_t count);
 
 /* RNG restatement (PathTrace.metal:264-276). */
 uint32_t or_hash(uint32_t value);

@@ -53,7 +53,7 @@ class _Settings(C.Structure):
         ("environmentRotationY", C.c_float), ("environmentMaxRadiance", C.c_float),
         ("sampleRadianceClamp", C.c_float), ("backgroundColor", C.c_float * 4),
         ("tileX", C.c_uint32), ("tileY", C.c_uint32), ("tileWidth", C.c_uint32), ("tileHeight", C.c_uint32),
-        ("renderQuality", C.c_uint32), ("intersection", C.c_uint32),
+        ("renderQuality", C.c_uint32), ("intersection", C.c_uint32), ("environmentSampling", C.c_uint32),
     ]
 
 
@@ -74,6 +74,7 @@ def lib():
                                 C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.or_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+        L.or_environment_alias.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         L.or_hash.argtypes = [C.c_uint32]
         L.or_hash.restype = C.c_uint32
         L.or_random_float.argtypes = [C.POINTER(C.c_uint32)]
@@ -123,6 +124,14 @@ class OracleScene:
         lib().or_scene_export_bvh(self.handle, _ptr(nodes), _ptr(prim))
         return nodes, prim
 
+    def environment_alias(self):
+        """(prob float32, alias uint32) of the environment alias table."""
+        n = len(self.build.environment_samples)
+        prob = np.zeros(n, np.float32)
+        alias = np.zeros(n, np.uint32)
+        assert lib().or_environment_alias(self.handle, _ptr(prob), _ptr(alias), n) == 0
+        return prob, alias
+
     def intersect(self, rays: np.ndarray, brute_force: bool = False, with_hit: bool = False, intersection: int = 0):
         """Closest hits of rays (N, 8) = (origin, _, direction, _): t, primitive
         id (~0 = miss) and, with_hit, (N, 8) = normal xyz, materialID,
@@ -145,7 +154,7 @@ class OracleResult:
 
 def make_settings(build, width, height, max_bounces=4, frame_seed=0x1234ABCD, clamp=64.0,
                   transparent_background=False, shows_environment_background=True, background=(0, 0, 0),
-                  denoiser=False, tile=None, quality=2, intersection=0) -> _Settings:
+                  denoiser=False, tile=None, quality=2, intersection=0, environment_sampling=0) -> _Settings:
     s = _Settings()
     s.width, s.height, s.maxBounces, s.frameSeed = width, height, max(1, max_bounces), frame_seed
     s.transparentBackground = int(transparent_background)
@@ -163,6 +172,7 @@ def make_settings(build, width, height, max_bounces=4, frame_seed=0x1234ABCD, cl
     s.tileX, s.tileY, s.tileWidth, s.tileHeight = x, y, w, h
     s.renderQuality = quality
     s.intersection = intersection
+    s.environmentSampling = environment_sampling
     return s
 
 
