@@ -269,6 +269,12 @@ class RenderSession:
         _lib.check(_lib.load().dr_hip_session_read_output(self._h, int(output), _ptr(out), out.size), "read output")
         return out
 
+    def write_png(self, path: str, output: RenderOutput = RenderOutput.BEAUTY) -> None:
+        """writePNG(output:to:) (RenderSession.swift:1026-1034): the output's
+        display encoding (PNGWriter.swift:38-56) as an RGBA8 PNG."""
+        from .image import visualized_rgba8, write_png
+        write_png(path, visualized_rgba8(self.pixels(output), output))
+
     def triangle_ids(self) -> np.ndarray:
         """Internal parity plane: global triangle index + 1 (0 = no hit)."""
         return self.pixels(RenderOutput.TRIANGLE_ID)[..., 0].view(np.uint32).copy()

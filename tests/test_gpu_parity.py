@@ -230,3 +230,28 @@ def test_alias_sampling_converges_to_cdf_image(renderer):
           f"4096 spp cdf {conv_cdf:.6f} alias {conv_alias:.6f}")
     assert 0.7 < r_alias / r_cdf < 1.4
     assert conv_alias < 1.5 * conv_cdf
+
+
+@pytest.mark.parametrize("name", ["CornellBox", "MaterialReference"])
+def test_hip_render_png_matches_reference_metrics(renderer, name, tmp_path):
+    # The output side end to end: HIP render -> PNG (the reference's display
+    # encoding) -> ImageMetrics, against the reference's stored metrics and
+    # tolerances (ReferenceMetrics/*.json, copied to tests/golden).
+    import json
+    import os
+
+    from PIL import Image
+
+    from denrimrendererkit_amd import RenderScene
+    from test_oracle_pins import GOLDEN, _assert_metrics, _image_metrics
+    base = json.load(open(os.path.join(GOLDEN, "reference_metrics", f"{name}.json")))
+    W, H = base["metrics"]["width"], base["metrics"]["height"]
+    scene = {"CornellBox": RenderScene.cornell_box, "MaterialReference": RenderScene.material_reference}[name]()
+    st = RenderSettings(width=W, height=H, max_bounces=base["maxBounces"], quality=RenderQuality.FINAL,
+                        frame_seed=0x1234ABCD)
+    sess = renderer.make_session(scene, st)
+    sess.render(base["samples"])
+    path = os.path.join(tmp_path, f"{name}.png")
+    sess.write_png(path)
+    rgba8 = np.asarray(Image.open(path).convert("RGBA"))
+    _assert_metrics(_image_metrics(rgba8), base)
