@@ -79,8 +79,7 @@ DrCamera toCam(const dr_gpu_camera &c) {
 // N walk, N resume), counters, stats, the sample base word and the
 // per-lane traversal stack spill.
 struct PathLayout {
-    size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, stats, base, ovf,
-        ovfSide;
+    size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, stats, base, ovf;
     size_t bytes;
 };
 static PathLayout pathLayout(size_t N, size_t stride) {
@@ -97,7 +96,6 @@ static PathLayout pathLayout(size_t N, size_t stride) {
     l.q0 = take(N * 4), l.q1 = take(N * 4), l.sq = take(3 * N * 4), l.sss = take(N * 4), l.res = take(N * 4);
     l.ctr = take(DR_CTR_WORDS * 4), l.stats = take(DR_CNT_SLOTS * 8), l.base = take(4);
     l.ovf = take(stride * (DR_STACK_MAX - DR_LDS_STACK) * 4);
-    l.ovfSide = take(stride * (DR_STACK_MAX - DR_LDS_STACK) * 4);  // kernels on the side stream
     l.bytes = bytes;
     return l;
 }
@@ -491,9 +489,6 @@ static void freeSession(dr_session *s) {
     if (s->evBegin) (void)hipEventDestroy(s->evBegin);
     if (s->evEnd) (void)hipEventDestroy(s->evEnd);
     for (auto ev : s->stageEvents) (void)hipEventDestroy(ev);
-    if (s->evFork) (void)hipEventDestroy(s->evFork);
-    if (s->evJoin) (void)hipEventDestroy(s->evJoin);
-    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -572,7 +567,6 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     s->sampleBase = reinterpret_cast<uint32_t *>(b + l.base);
     P.overflow = reinterpret_cast<uint32_t *>(b + l.ovf);
     P.overflowStride = sh.stride;
-    s->overflowSide = reinterpret_cast<uint32_t *>(b + l.ovfSide);
     const size_t plane = sh.outBytes / size_t(sh.planes);
     char *o = static_cast<char *>(s->outBlock);
     DrOutputs &O = s->out;
@@ -606,9 +600,6 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
     auto *s = new dr_session;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->evFork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->evJoin, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
     if (e == hipSuccess) e = hipEventCreate(&s->evEnd);
     if (e == hipSuccess) e = hipMalloc(&s->pathBlock, sh.lay.bytes);
@@ -699,44 +690,31 @@ static DrLaunch makeLaunch(const dr_session *s, uint32_t sampleBegin, uint32_t s
 
 static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
     const uint32_t nSlots = L.tileWidth * L.tileHeight * L.samplesInBatch;
-    // Stage timing brackets each launch with events on the stream it runs on.
-    // Kernels on the side stream spill their traversal stacks to their own
-    // region: they run concurrently with the first shadow part.
-    DrPaths sideP = s->paths;
-    sideP.overflow = s->overflowSide;
-    auto run = [&](DrStage stage, uint32_t b, hipStream_t st, uint32_t part = 0) -> hipError_t {
+    auto run = [&](DrStage stage, uint32_t b) -> hipError_t {
         size_t ev = s->stageEvents.size();
         if (s->stageTiming) {
             hipEvent_t a, z;
             if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&z) != hipSuccess) return hipErrorOutOfMemory;
             s->stageEvents.push_back(a);
             s->stageEvents.push_back(z);
-            (void)hipEventRecord(a, st);
+            (void)hipEventRecord(a, s->stream);
         }
-        hipError_t e = dr::launchStage(stage, b, s->sceneDev, L, s->camera, s->previousCamera,
-                                       st == s->side ? sideP : s->paths, s->out, nSlots, s->traceGrid, st, part);
-        if (s->stageTiming) (void)hipEventRecord(s->stageEvents[ev + 1], st);
+        hipError_t e = dr::launchStage(stage, b, s->sceneDev, L, s->camera, s->previousCamera, s->paths, s->out,
+                                       nSlots, s->traceGrid, s->stream);
+        if (s->stageTiming) (void)hipEventRecord(s->stageEvents[ev + 1], s->stream);
         s->stageLaunches[stage]++;
         return e;
     };
-    DR_HIP(run(DR_STAGE_RAYGEN, 0, s->stream), "raygen launch");
+    DR_HIP(run(DR_STAGE_RAYGEN, 0), "raygen launch");
     for (uint32_t b = 0; b < L.maxBounces; ++b) {
-        DR_HIP(run(DR_STAGE_EXTEND, b, s->stream), "extend launch");
-        DR_HIP(run(DR_STAGE_SHADE, b, s->stream), "shade launch");
-        // Fork: subsurface walks and the resumed shading on the side stream,
-        // concurrently with the shadow rays k_shade queued; join before the
-        // resumed paths' shadow rays and the resolve.
-        DR_HIP(hipEventRecord(s->evFork, s->stream), "fork");
-        DR_HIP(hipStreamWaitEvent(s->side, s->evFork, 0), "fork wait");
-        DR_HIP(run(DR_STAGE_SSS, b, s->side), "subsurface launch");
-        DR_HIP(run(DR_STAGE_RESUME, b, s->side), "shade resume launch");
-        DR_HIP(run(DR_STAGE_SHADOW, b, s->stream, 0), "shadow launch");
-        DR_HIP(hipEventRecord(s->evJoin, s->side), "join");
-        DR_HIP(hipStreamWaitEvent(s->stream, s->evJoin, 0), "join wait");
-        DR_HIP(run(DR_STAGE_SHADOW, b, s->stream, 1), "shadow launch");
-        DR_HIP(run(DR_STAGE_RESOLVE, b, s->stream), "resolve launch");
+        DR_HIP(run(DR_STAGE_EXTEND, b), "extend launch");
+        DR_HIP(run(DR_STAGE_SHADE, b), "shade launch");
+        DR_HIP(run(DR_STAGE_SSS, b), "subsurface launch");
+        DR_HIP(run(DR_STAGE_RESUME, b), "shade resume launch");
+        DR_HIP(run(DR_STAGE_SHADOW, b), "shadow launch");
+        DR_HIP(run(DR_STAGE_RESOLVE, b), "resolve launch");
     }
-    DR_HIP(run(DR_STAGE_FINALIZE, 0, s->stream), "finalize launch");
+    DR_HIP(run(DR_STAGE_FINALIZE, 0), "finalize launch");
     return DR_OK;
 }
 
@@ -760,8 +738,7 @@ static void collectStageTimes(dr_session *s) {
             add(DR_STAGE_SHADE);
             add(DR_STAGE_SSS);
             add(DR_STAGE_RESUME);
-            add(DR_STAGE_SHADOW);  // part 0
-            add(DR_STAGE_SHADOW);  // part 1
+            add(DR_STAGE_SHADOW);
             add(DR_STAGE_RESOLVE);
         }
         add(DR_STAGE_FINALIZE);

@@ -54,11 +54,6 @@ struct dr_session {
     uint32_t nSlots = 0;
     uint32_t traceGrid = 0;
     hipStream_t stream = nullptr;
-    // Subsurface walks and the resumed shading run on `side`, overlapped with
-    // the first shadow part on `stream` (fork/join events per bounce).
-    hipStream_t side = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr;
-    uint32_t *overflowSide = nullptr;  // traversal-stack spill of the side stream's kernels
     DrPaths paths{};
     DrOutputs out{};
     void *pathBlock = nullptr;   // one allocation for all path-state arrays

@@ -26,11 +26,7 @@
 #define DR_CTR_SSSQ (DR_CTR_SHADOW + DR_MAX_BOUNCES)
 #define DR_CTR_SSS (DR_CTR_SSSQ + DR_MAX_BOUNCES)
 #define DR_CTR_RESUMEQ (DR_CTR_SSS + DR_MAX_BOUNCES)
-// Shadow rays of resumed paths (after subsurface walks) follow those of
-// k_shade in the shadow queue: count and claim counters of that second part.
-#define DR_CTR_SHADOWQ2 (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
-#define DR_CTR_SHADOW2 (DR_CTR_SHADOWQ2 + DR_MAX_BOUNCES)
-#define DR_CTR_WORDS (DR_CTR_SHADOW2 + DR_MAX_BOUNCES)
+#define DR_CTR_WORDS (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
 
 struct DrPaths {
     float4 *rayO;   // origin.xyz, previousBSDFPDF
@@ -76,11 +72,9 @@ namespace dr {
 // Enqueues one stage of a sample batch on `st`. One batch is RAYGEN, then
 // for b in [0, maxBounces): EXTEND, SHADE, SSS, RESUME, SHADOW, RESOLVE;
 // then FINALIZE.
-// DR_STAGE_SHADOW takes `part`: 0 = the shadow rays k_shade queued, 1 = those
-// of resumed paths (k_shade<RESUME>); the two parts may run on different streams.
 hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
-                       uint32_t traceGrid, hipStream_t st, uint32_t part = 0);
+                       uint32_t traceGrid, hipStream_t st);
 // Device word of the first sample index of the next batch: set it, or add
 // the batch's sample count after a batch (the last node of a batch graph).
 hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st);

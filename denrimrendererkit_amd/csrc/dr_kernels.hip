@@ -754,9 +754,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
 #pragma unroll
     for (uint32_t j = 0; j < 3u; ++j) {
         bool v = (neeMask >> j) & 1u;
-        // Resumed paths queue after k_shade's rays (a second shadow part).
-        uint32_t sp = RESUME ? P.counters[DR_CTR_SHADOWQ + bounce] + waveAppend(v, &P.counters[DR_CTR_SHADOWQ2 + bounce])
-                             : waveAppend(v, &P.counters[DR_CTR_SHADOWQ + bounce]);
+        uint32_t sp = waveAppend(v, &P.counters[DR_CTR_SHADOWQ + bounce]);
         if (v) P.shadowQueue[sp] = slot * 3u + j;
         else P.shC[slot * 3u + j].w = 0.0f;
     }
@@ -926,15 +924,10 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L
 // shadowOccluded (PathTrace.metal:4196-4274) for one NEE ray; writes the
 // transmittance (0 if occluded) into shC[k].w's companion record shR[k].
 template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn,
-                                                           uint32_t part) {
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
-    // Part 0: the rays k_shade queued; part 1: those of resumed paths,
-    // queued after them.
-    const uint32_t qbase = part ? P.counters[DR_CTR_SHADOWQ + bounce] : 0u;
-    const uint32_t count = P.counters[(part ? DR_CTR_SHADOWQ2 : DR_CTR_SHADOWQ) + bounce];
-    uint32_t *steal = &P.counters[(part ? DR_CTR_SHADOW2 : DR_CTR_SHADOW) + bounce];
-    const uint32_t *queue = P.shadowQueue + qbase;
+    const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce];
+    uint32_t *steal = &P.counters[DR_CTR_SHADOW + bounce];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
@@ -946,7 +939,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         dr_trace_queue<true>(
             S, count, steal, stk, nv, nt,
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                k = queue[qi];
+                k = P.shadowQueue[qi];
                 float4 o4 = P.shO[k], d4 = P.shD[k];
                 r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
                 best = DrHitRec{o4.w, 0.0f, 0.0f, 0u};
@@ -973,7 +966,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         dr_trace_queue<false>(
             S, count, steal, stk, nv, nt,
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                k = queue[qi];
+                k = P.shadowQueue[qi];
                 float4 o4 = P.shO[k], d4 = P.shD[k];
                 o = v3(o4.x, o4.y, o4.z);
                 d = v3(d4.x, d4.y, d4.z);
@@ -1030,7 +1023,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         if (base >= count) break;
         uint32_t qi = base + laneId();
         if (qi >= count) continue;
-        uint32_t k = queue[qi];
+        uint32_t k = P.shadowQueue[qi];
         float4 o4 = P.shO[k], d4 = P.shD[k];
         vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
         float maxDistance = o4.w;
@@ -1250,7 +1243,7 @@ hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t
 
 hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, const DrLaunch &L, const DrCamera &cam,
                        const DrCamera &prevCam, const DrPaths &P, const DrOutputs &O, uint32_t nSlots,
-                       uint32_t traceGrid, hipStream_t st, uint32_t part) {
+                       uint32_t traceGrid, hipStream_t st) {
     const uint32_t bThreads = 256;
     const bool vol = S.sdf.volumeCount != 0u;
     switch (stage) {
@@ -1285,11 +1278,9 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
         break;
     case DR_STAGE_SHADOW:
         if (vol)
-            hipLaunchKernelGGL(k_shadow<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled,
-                               part);
+            hipLaunchKernelGGL(k_shadow<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         else
-            hipLaunchKernelGGL(k_shadow<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce,
-                               L.statsEnabled, part);
+            hipLaunchKernelGGL(k_shadow<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         break;
     case DR_STAGE_RESOLVE:
         hipLaunchKernelGGL(k_resolve, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, P, bounce);
