@@ -705,6 +705,12 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
         s->stageLaunches[stage]++;
         return e;
     };
+    if (s->settings.quality < 2u) {
+        // Preview and interactive qualities: one kernel writes the sample
+        // (DenrimRenderer.primaryPipeline, DenrimRenderer.swift:197-215).
+        DR_HIP(run(DR_STAGE_PREVIEW, 0), "preview launch");
+        return DR_OK;
+    }
     DR_HIP(run(DR_STAGE_RAYGEN, 0), "raygen launch");
     for (uint32_t b = 0; b < L.maxBounces; ++b) {
         DR_HIP(run(DR_STAGE_EXTEND, b), "extend launch");
@@ -725,6 +731,7 @@ static void collectStageTimes(dr_session *s) {
     // Events come in (begin, end) pairs in launch order; replay the order.
     size_t i = 0;
     const uint32_t mb = std::max(1u, s->settings.max_bounces);
+    const bool preview = s->settings.quality < 2u;
     while (i + 1 < s->stageEvents.size()) {
         auto add = [&](DrStage st) {
             float ms = 0.f;
@@ -732,6 +739,10 @@ static void collectStageTimes(dr_session *s) {
                 s->stageMs[st] += ms;
             i += 2;
         };
+        if (preview) {
+            add(DR_STAGE_PREVIEW);
+            continue;
+        }
         add(DR_STAGE_RAYGEN);
         for (uint32_t b = 0; b < mb; ++b) {
             add(DR_STAGE_EXTEND);

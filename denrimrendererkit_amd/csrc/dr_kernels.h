@@ -65,7 +65,8 @@ enum DrStage {
     DR_STAGE_FINALIZE = 5,
     DR_STAGE_SSS = 6,     // subsurface random walks of the bounce
     DR_STAGE_RESUME = 7,  // surface shading after a failed walk
-    DR_STAGE_COUNT = 8
+    DR_STAGE_PREVIEW = 8, // flat preview / interactive material kernels (quality 0 / 1)
+    DR_STAGE_COUNT = 9
 };
 
 namespace dr {

@@ -920,6 +920,62 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L
     }
 }
 
+// shadowOccluded (PathTrace.metal:4196-4274) for one ray: up to 8 closest
+// hits, stepping through nearly transparent surfaces and attenuating
+// through transmissive ones; tr returns the transmittance. The triangle-only
+// form limits each query to the remaining distance.
+template <bool VOL>
+__device__ __forceinline__ bool shadowChain(const DrSceneDev &S, vec3 o, vec3 d, float maxDistance, vec3 &tr,
+                                            DrStack &stk, uint32_t &nq, uint32_t &nv, uint32_t &nt,
+                                            DrSdfCounters &sc) {
+    tr = v3s(1.0f);
+    bool occluded;
+    occluded = true;
+    float traveled = 0.0f;
+    for (uint32_t step = 0; step < 8u; ++step) {
+        ++nq;
+        float4 h4, hn4;
+        bool hit;
+        if constexpr (!VOL) {
+            DrRayPre r = dr_prep(o, d);
+            // Hits at or beyond maxDistance - traveled end the chain,
+            // so the search is limited there: exactly (exclusive
+            // limit) on the first step, with a relative margin
+            // afterwards where the reference test
+            // traveled + t >= maxDistance decides.
+            float limit = step == 0u ? maxDistance : (maxDistance - traveled) * 1.000001f + 1e-30f;
+            DrHitRec h{limit, 0.0f, 0.0f, step == 0u ? 0u : DR_MISS};
+            hit = dr_traverse<false>(S, r, h, stk, nv, nt);
+            h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        } else {
+            // The volume search is bounded by the closest triangle,
+            // wherever it lies, so the triangle query is unlimited.
+            hit = sceneClosest<true>(S, o, d, stk, nv, nt, sc, h4, hn4);
+        }
+        if (!hit || traveled + h4.x >= maxDistance) {
+            occluded = false;
+            break;
+        }
+        Surf sh = surfaceAt<VOL>(S, o, d, h4, hn4);
+        Mat m = materialFor(S, sh);
+        if (m.base.w <= 0.001f) {
+            traveled += sh.t;
+            o = sh.position + d * 0.001f;
+            continue;
+        }
+        float transmission = materialTransmission(m);
+        if (transmission <= 0.001f) break;
+        bool inside = !sh.frontFacing && !materialThinWalled(m);
+        vec3 medium = materialHasVolumeScattering(m) && inside ? materialVolumeTransmittance(m, sh.t)
+                                                               : materialTransmissionAbsorption(m, sh.t, inside);
+        tr = tr * materialTransmissionTint(m, 0.35f) * medium * transmission;
+        if (maxComponent(tr) < 0.02f) break;
+        traveled += sh.t;
+        o = sh.position + d * 0.001f;
+    }
+    return occluded;
+}
+
 // ---------------------------------------------------------------- shadow --
 // shadowOccluded (PathTrace.metal:4196-4274) for one NEE ray; writes the
 // transmittance (0 if occluded) into shC[k].w's companion record shR[k].
@@ -1027,53 +1083,8 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         float4 o4 = P.shO[k], d4 = P.shD[k];
         vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
         float maxDistance = o4.w;
-        vec3 tr = v3s(1.0f);
-        bool occluded;
-        {
-            occluded = true;
-            float traveled = 0.0f;
-            for (uint32_t step = 0; step < 8u; ++step) {
-                ++nq;
-                float4 h4, hn4;
-                bool hit;
-                if constexpr (!VOL) {
-                    DrRayPre r = dr_prep(o, d);
-                    // Hits at or beyond maxDistance - traveled end the chain,
-                    // so the search is limited there: exactly (exclusive
-                    // limit) on the first step, with a relative margin
-                    // afterwards where the reference test
-                    // traveled + t >= maxDistance decides.
-                    float limit = step == 0u ? maxDistance : (maxDistance - traveled) * 1.000001f + 1e-30f;
-                    DrHitRec h{limit, 0.0f, 0.0f, step == 0u ? 0u : DR_MISS};
-                    hit = dr_traverse<false>(S, r, h, stk, nv, nt);
-                    h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-                } else {
-                    // The volume search is bounded by the closest triangle,
-                    // wherever it lies, so the triangle query is unlimited.
-                    hit = sceneClosest<true>(S, o, d, stk, nv, nt, sc, h4, hn4);
-                }
-                if (!hit || traveled + h4.x >= maxDistance) {
-                    occluded = false;
-                    break;
-                }
-                Surf sh = surfaceAt<VOL>(S, o, d, h4, hn4);
-                Mat m = materialFor(S, sh);
-                if (m.base.w <= 0.001f) {
-                    traveled += sh.t;
-                    o = sh.position + d * 0.001f;
-                    continue;
-                }
-                float transmission = materialTransmission(m);
-                if (transmission <= 0.001f) break;
-                bool inside = !sh.frontFacing && !materialThinWalled(m);
-                vec3 medium = materialHasVolumeScattering(m) && inside ? materialVolumeTransmittance(m, sh.t)
-                                                                       : materialTransmissionAbsorption(m, sh.t, inside);
-                tr = tr * materialTransmissionTint(m, 0.35f) * medium * transmission;
-                if (maxComponent(tr) < 0.02f) break;
-                traveled += sh.t;
-                o = sh.position + d * 0.001f;
-            }
-        }
+        vec3 tr;
+        const bool occluded = shadowChain<VOL>(S, o, d, maxDistance, tr, stk, nq, nv, nt, sc);
         P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
     }
     if (statsOn) {
@@ -1082,6 +1093,229 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, nt);
         if constexpr (VOL) addSdfStats(P.stats, sc, DR_CNT_SDF_SHADOW_BASE);
     }
+}
+
+// ------------------------------------------------------------- preview --
+// The kernels of the preview and interactive qualities
+// (DenrimRenderer.primaryPipeline, DenrimRenderer.swift:197-215): one pixel
+// per thread, the pixel-centre ray, the accumulation and the AOVs written
+// (not averaged) each sample. flatPreviewKernel (PathTrace.metal:4728-4868):
+// base colour under a fixed key light plus emission; interactiveMaterialKernel
+// (:4870-5165): direct light from up to 12 emitters (centroid, shadow
+// chain), a six-lookup diffuse and five-lookup specular environment,
+// contact occlusion from four short closest-hit probes, and the material
+// terms of the reference.
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+__device__ __forceinline__ vec3 interactiveDiffuseEnvironment(const DrSceneDev &S, vec3 n) {  // :3805-3823
+    const vec3 up = sampleEnvironment(S, v3(0.0f, 1.0f, 0.0f));
+    const vec3 down = sampleEnvironment(S, v3(0.0f, -1.0f, 0.0f));
+    const vec3 side = (sampleEnvironment(S, v3(1.0f, 0.0f, 0.0f)) + sampleEnvironment(S, v3(-1.0f, 0.0f, 0.0f)) +
+                       sampleEnvironment(S, v3(0.0f, 0.0f, 1.0f)) + sampleEnvironment(S, v3(0.0f, 0.0f, -1.0f))) *
+                      0.25f;
+    const float upW = smoothstepf(-0.15f, 0.85f, n.y);
+    const float downW = smoothstepf(0.15f, -0.85f, n.y);
+    const float sideW = fmaxf(0.0f, 1.0f - upW - downW);
+    return up * upW + down * downW + side * sideW;
+}
+
+__device__ __forceinline__ vec3 interactiveHemisphereDirection(vec3 d, vec3 n) {  // :3825-3831
+    const float nd = dot(n, d);
+    if (nd < 0.0f) d = d - n * (2.0f * nd);
+    return normalize(d);
+}
+
+__device__ __forceinline__ vec3 interactiveSpecularEnvironment(const DrSceneDev &S, vec3 r, vec3 n,
+                                                               float roughness) {  // :3833-3880
+    r = interactiveHemisphereDirection(normalize(r), n);
+    roughness = clampf(roughness, 0.0f, 1.0f);
+    const float spread = roughness * roughness;
+    const vec3 helper = fabsf(r.y) < 0.999f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const vec3 t = normalize(cross(helper, r));
+    const vec3 b = cross(r, t);
+    vec3 cone = sampleEnvironment(S, r) * 0.42f;
+    cone += sampleEnvironment(S, interactiveHemisphereDirection(r + t * spread * 0.95f + n * spread * 0.18f, n)) * 0.145f;
+    cone += sampleEnvironment(S, interactiveHemisphereDirection(r - t * spread * 0.95f + n * spread * 0.18f, n)) * 0.145f;
+    cone += sampleEnvironment(S, interactiveHemisphereDirection(r + b * spread * 0.95f + n * spread * 0.18f, n)) * 0.145f;
+    cone += sampleEnvironment(S, interactiveHemisphereDirection(r - b * spread * 0.95f + n * spread * 0.18f, n)) * 0.145f;
+    return mixv(cone, interactiveDiffuseEnvironment(S, n), smoothstepf(0.62f, 0.95f, roughness));
+}
+
+template <bool VOL>
+__device__ __forceinline__ float interactiveContactOcclusion(const DrSceneDev &S, const Surf &h, const Mat &m, vec3 n,
+                                                             DrStack &stk, uint32_t &nq, uint32_t &nv, uint32_t &nt,
+                                                             DrSdfCounters &sc) {  // :3882-3944
+    const Frame f = makeTangentFrame(n, h.tangent, h.bitangent);
+    const vec3 local[4] = {v3(0.000f, 0.000f, 1.000f), v3(0.780f, 0.180f, 0.600f), v3(-0.420f, 0.720f, 0.552f),
+                           v3(-0.620f, -0.520f, 0.587f)};
+    const float radius = 0.34f;
+    float occlusion = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        const vec3 l = normalize(local[k]);
+        const vec3 dir = normalize(l.x * f.t + l.y * f.b + l.z * n);
+        const vec3 o = h.position + n * 0.003f;
+        float4 hr, hn;
+        ++nq;
+        if (!sceneClosest<VOL>(S, o, dir, stk, nv, nt, sc, hr, hn) || hr.x >= radius) continue;
+        const Surf oh = surfaceAt<VOL>(S, o, dir, hr, hn);
+        const Mat om = materialFor(S, oh);
+        const float blocker = clampf(om.base.w, 0.0f, 1.0f) * (1.0f - materialTransmission(om) * 0.72f);
+        occlusion += blocker * (1.0f - smoothstepf(0.0f, radius, oh.t));
+    }
+    const float strength = 0.72f + (0.36f - 0.72f) * materialTransmission(m);
+    return 1.0f - clampf((occlusion / 4.0f) * strength, 0.0f, 0.72f);
+}
+
+template <bool VOL, bool INTERACTIVE>
+__global__ void __launch_bounds__(DR_SHADE_BLOCK) k_preview(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
+                                                            DrPaths P, DrOutputs O) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
+    const uint32_t i = blockIdx.x * DR_SHADE_BLOCK + threadIdx.x;
+    DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + i, P.overflowStride};
+    if (i >= L.tileWidth * L.tileHeight) return;
+    const uint32_t gx = L.tileX + i % L.tileWidth, gy = L.tileY + i / L.tileWidth;
+    if (gx >= L.width || gy >= L.height) return;
+    vec3 ro, rd;
+    makeCameraRay(cam, L.width, L.height, gx, gy, 0.5f, 0.5f, 0.0f, 0.0f, ro, rd);
+    uint32_t nq = 0u, nv = 0u, nt = 0u;
+    DrSdfCounters sc{};
+    float4 hr, hn;
+    const bool hit = sceneClosest<VOL>(S, ro, rd, stk, nv, nt, sc, hr, hn);
+    const size_t pix = size_t(gy) * L.width + gx;
+    if (L.statsEnabled) {
+        atomicAdd(&P.stats[DR_CNT_PRIMARY], 1ull);
+        atomicAdd(&P.stats[DR_CNT_NODES], (unsigned long long)nv);
+        atomicAdd(&P.stats[DR_CNT_TRIS], (unsigned long long)nt);
+        if constexpr (VOL) addSdfStats(P.stats, sc);  // misses count their volume traversal too
+    }
+    nv = nt = 0u;
+    sc = DrSdfCounters{};
+    if (!hit) {
+        vec3 color = vmax(v3(L.backgroundColor[0], L.backgroundColor[1], L.backgroundColor[2]), v3s(0.0f));
+        if (!L.transparentBackground && L.showsEnvironmentBackground) color = sampleEnvironment(S, rd);
+        O.accumulation[pix] = make_float4(color.x, color.y, color.z, L.transparentBackground ? 0.0f : 1.0f);
+        O.depth[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.normal[pix] = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
+        O.albedo[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.materialID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.objectID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.motion[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        O.triangleID[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    Surf h = surfaceAt<VOL>(S, ro, rd, hr, hn);
+    Mat m = materialFor(S, h);
+    applyNormalMap(S, h, m, rd);
+    applyBaseColorTexture(S, m, h);
+    const vec3 n = normalize(h.normal);
+    const vec3 view = normalize(-rd);
+    const vec3 baseColor = vmax(xyz(m.base), v3s(0.0f));
+    vec3 color;
+    float alpha;
+    float4 albedo;
+    if constexpr (!INTERACTIVE) {  // :4838-4845
+        const vec3 key = normalize(v3(0.35f, 0.72f, 0.58f));
+        const float lambert = fmaxf(dot(n, key), 0.0f);
+        const float facing = fmaxf(dot(n, view), 0.0f);
+        const float rim = powf(1.0f - facing, 2.0f) * 0.08f;
+        const float shade = 0.32f + 0.68f * lambert + rim;
+        color = baseColor * shade + vmax(xyz(ms<M_EMISSION>(m)), v3s(0.0f));
+        alpha = L.transparentBackground ? clampf(m.base.w, 0.0f, 1.0f) : 1.0f;
+        albedo = m.base;
+    } else {  // :4967-5140
+        const float roughness = clampf(ms<M_PARAMS>(m).x, 0.02f, 1.0f);
+        const float metallic = clampf(ms<M_PARAMS>(m).y, 0.0f, 1.0f);
+        const float opacity = clampf(m.base.w, 0.0f, 1.0f);
+        const float specular = clampf(ms<M_PARAMS2>(m).x, 0.0f, 1.0f);
+        color = vmax(xyz(ms<M_EMISSION>(m)), v3s(0.0f));
+        const uint32_t nl = min(S.lightCount, 12u);
+        for (uint32_t li = 0; li < nl; ++li) {
+            const DrLight light = S.lights[li];
+            if (light.triangleIndex >= S.triangleCount || light.materialIndex >= S.materialCount) continue;
+            const DrTriangle &lt = S.triangles[light.triangleIndex];
+            const vec3 emission = vmax(xyz(S.materials[light.materialIndex].s[M_EMISSION]), v3s(0.0f));
+            if (maxComponent(emission) <= 0.0f || light.area <= 0.0f) continue;
+            const vec3 lp = (xyz(lt.v0) + xyz(lt.v1) + xyz(lt.v2)) / 3.0f;
+            const vec3 toLight = lp - h.position;
+            const float dist2 = fmaxf(dot(toLight, toLight), 1e-5f);
+            const float dist = sqrtf(dist2);
+            const vec3 ldir = toLight / dist;
+            const float cosS = fmaxf(dot(n, ldir), 0.0f);
+            const float cosL = fmaxf(dot(xyz(light.normal), -ldir), 0.0f);
+            if (cosS <= 0.0f || cosL <= 0.0f) continue;
+            vec3 tr;
+            if (shadowChain<VOL>(S, h.position + n * 0.001f, ldir, dist - 0.002f, tr, stk, nq, nv, nt, sc)) continue;
+            vec3 brdf = evaluateMaterialBRDF(m, n, h.tangent, h.bitangent, view, ldir);
+            const float ccGloss = materialClearcoat(m) * powf(1.0f - materialClearcoatRoughness(m), 2.0f);
+            const float gloss =
+                clampf(metallic + ccGloss + (1.0f - metallic) * powf(1.0f - roughness, 4.0f) * specular, 0.0f, 1.0f);
+            const vec3 diffuseOnly =
+                (1.0f - metallic) * baseColor * vmax(v3s(0.0f), v3s(1.0f) - materialF0(m)) * 0.31830988618f;
+            brdf = mixv(diffuseOnly, brdf, gloss);
+            color += brdf * emission * (cosS * cosL * light.area / dist2) * tr;
+        }
+        const vec3 envDiffuse = interactiveDiffuseEnvironment(S, n);
+        const vec3 refl = normalize(reflect(-view, n));
+        const vec3 baseRefl = interactiveSpecularEnvironment(S, refl, n, roughness);
+        const float ccRough = materialClearcoatRoughness(m);
+        const vec3 ccRefl = interactiveSpecularEnvironment(S, refl, n, ccRough);
+        const float nDotV = fmaxf(dot(n, view), 0.0f);
+        const vec3 fresnel = fresnelSchlickThinFilm(nDotV, materialF0(m), m);
+        const float glossyW = powf(1.0f - roughness, 4.0f);
+        const float metalW = metallic * powf(1.0f - roughness, 2.0f);
+        const float ccW = materialClearcoat(m) * powf(1.0f - ccRough, 3.0f);
+        const float dielW = (1.0f - metallic) * glossyW * specular;
+        const vec3 roughFresnel = mixv(fresnel, materialF0(m), smoothstepf(0.45f, 0.95f, roughness));
+        const vec3 ccFresnel = fresnelSchlickThinFilm(nDotV, materialClearcoatF0Color(m), m);
+        const float horizon = 0.58f + 0.42f * clampf(n.y * 0.5f + 0.5f, 0.0f, 1.0f);
+        const float contact = interactiveContactOcclusion<VOL>(S, h, m, n, stk, nq, nv, nt, sc);
+        const float ao = horizon * contact;
+        color += baseColor * envDiffuse * (0.58f * (1.0f - metallic) * ao);
+        color += baseRefl * roughFresnel * (metalW * 1.05f + dielW * 0.32f);
+        color += ccRefl * ccFresnel * ccW * 0.42f;
+        const vec3 key = normalize(v3(-0.32f, 0.78f, 0.52f));
+        color += baseColor * ((1.0f - metallic) * fmaxf(dot(n, key), 0.0f) * 0.16f * contact);
+        const float sss = materialSubsurface(m);
+        if (sss > 0.0f) {
+            const float wrap = powf(1.0f - nDotV, 2.0f);
+            color += materialSubsurfaceColor(m) * baseColor * envDiffuse * (sss * (0.12f + 0.22f * wrap) * contact);
+        }
+        const float transmission = materialTransmission(m);
+        if (transmission > 0.0f) {
+            const vec3 transmitted = sampleEnvironment(S, rd);
+            color = mixv(color, transmitted * materialTransmissionTint(m, 0.65f), transmission * 0.55f);
+            color += baseRefl * fresnel * (transmission * 0.35f);
+        }
+        color = vclamp(color, v3s(0.0f), v3s(fmaxf(L.sampleRadianceClamp, 1.0f)));
+        alpha = L.transparentBackground ? opacity : 1.0f;
+        albedo = make_float4(baseColor.x, baseColor.y, baseColor.z, opacity);
+        if (L.statsEnabled) {
+            atomicAdd(&P.stats[DR_CNT_SHADOW], (unsigned long long)nq);
+            atomicAdd(&P.stats[DR_CNT_SHADOW_NODES], (unsigned long long)nv);
+            atomicAdd(&P.stats[DR_CNT_SHADOW_TRIS], (unsigned long long)nt);
+        }
+    }
+    if constexpr (VOL) {
+        if (L.statsEnabled) addSdfStats(P.stats, sc);
+    }
+    O.accumulation[pix] = make_float4(color.x, color.y, color.z, alpha);
+    const vec3 en = n * 0.5f + v3s(0.5f);
+    O.depth[pix] = make_float4(h.t, h.t, h.t, 1.0f);
+    O.normal[pix] = make_float4(en.x, en.y, en.z, 1.0f);
+    O.albedo[pix] = albedo;
+    const float mid = (float)(h.materialID + 1u), oid = (float)(h.objectID + 1u);
+    O.materialID[pix] = make_float4(mid, mid, mid, 1.0f);
+    O.objectID[pix] = make_float4(oid, oid, oid, 1.0f);
+    vec2 cur, prv;
+    if (projectToScreen(cam, h.position, L.width, L.height, cur) && projectToScreen(prevCam, h.position, L.width, L.height, prv))
+        O.motion[pix] = make_float4(prv.x - cur.x, prv.y - cur.y, 0.0f, 1.0f);
+    else
+        O.motion[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float tid = __uint_as_float(h.primitiveID + 1u);
+    O.triangleID[pix] = make_float4(tid, tid, tid, tid);
 }
 
 // --------------------------------------------------------------- resolve --
@@ -1282,6 +1516,20 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
         else
             hipLaunchKernelGGL(k_shadow<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
         break;
+    case DR_STAGE_PREVIEW: {
+        const uint32_t px = L.tileWidth * L.tileHeight;
+        const dim3 grid((px + DR_SHADE_BLOCK - 1) / DR_SHADE_BLOCK), block(DR_SHADE_BLOCK);
+        const bool interactive = S.sdf.renderQuality == 1u;
+        if (vol && interactive)
+            hipLaunchKernelGGL((k_preview<true, true>), grid, block, 0, st, S, L, cam, prevCam, P, O);
+        else if (vol)
+            hipLaunchKernelGGL((k_preview<true, false>), grid, block, 0, st, S, L, cam, prevCam, P, O);
+        else if (interactive)
+            hipLaunchKernelGGL((k_preview<false, true>), grid, block, 0, st, S, L, cam, prevCam, P, O);
+        else
+            hipLaunchKernelGGL((k_preview<false, false>), grid, block, 0, st, S, L, cam, prevCam, P, O);
+        break;
+    }
     case DR_STAGE_RESOLVE:
         hipLaunchKernelGGL(k_resolve, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, P, bounce);
         break;

@@ -31,7 +31,7 @@ from . import _lib
 from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, compile_scene
 from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
 
-STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize", "sss", "shade_resume")
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize", "sss", "shade_resume", "preview")
 N_STATS = 30
 
 

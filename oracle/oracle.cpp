@@ -1593,6 +1593,191 @@ void pathTracePixel(Ctx<R> &c, uint32_t px, uint32_t py, uint32_t sampleIndex, R
     }
 }
 
+
+// The preview and interactive qualities' kernels: flatPreviewKernel
+// (PathTrace.metal:4728-4868) and interactiveMaterialKernel (:4870-5165),
+// selected by quality in DenrimRenderer.primaryPipeline
+// (DenrimRenderer.swift:197-215). Pixel-centre ray; accumulation and AOVs
+// are written, not averaged.
+template <class R> R smoothstepv(R e0, R e1, R x) {
+    R t = clampv((x - e0) / (e1 - e0), R(0), R(1));
+    return t * t * (R(3) - R(2) * t);
+}
+template <class R> V3<R> interactiveDiffuseEnvironment(const Ctx<R> &c, V3<R> n) {  // :3805-3823
+    V3<R> up = sampleEnvironment(c, V3<R>(0, 1, 0)), down = sampleEnvironment(c, V3<R>(0, -1, 0));
+    V3<R> side = (sampleEnvironment(c, V3<R>(1, 0, 0)) + sampleEnvironment(c, V3<R>(-1, 0, 0)) +
+                  sampleEnvironment(c, V3<R>(0, 0, 1)) + sampleEnvironment(c, V3<R>(0, 0, -1))) * R(0.25);
+    R upW = smoothstepv(R(-0.15), R(0.85), n.y), downW = smoothstepv(R(0.15), R(-0.85), n.y);
+    R sideW = fmx(R(0), R(1) - upW - downW);
+    return up * upW + down * downW + side * sideW;
+}
+template <class R> V3<R> interactiveHemisphereDirection(V3<R> d, V3<R> n) {  // :3825-3831
+    R nd = dot(n, d);
+    if (nd < R(0)) d = d - n * (R(2) * nd);
+    return normalize(d);
+}
+template <class R> V3<R> interactiveSpecularEnvironment(const Ctx<R> &c, V3<R> r, V3<R> n, R roughness) {  // :3833-3880
+    r = interactiveHemisphereDirection(normalize(r), n);
+    roughness = clampv(roughness, R(0), R(1));
+    R spread = roughness * roughness;
+    V3<R> helper = std::fabs(r.y) < R(0.999) ? V3<R>(0, 1, 0) : V3<R>(1, 0, 0);
+    V3<R> t = normalize(cross(helper, r)), b = cross(r, t);
+    V3<R> cone = sampleEnvironment(c, r) * R(0.42);
+    cone = cone + sampleEnvironment(c, interactiveHemisphereDirection(r + t * spread * R(0.95) + n * spread * R(0.18), n)) * R(0.145);
+    cone = cone + sampleEnvironment(c, interactiveHemisphereDirection(r - t * spread * R(0.95) + n * spread * R(0.18), n)) * R(0.145);
+    cone = cone + sampleEnvironment(c, interactiveHemisphereDirection(r + b * spread * R(0.95) + n * spread * R(0.18), n)) * R(0.145);
+    cone = cone + sampleEnvironment(c, interactiveHemisphereDirection(r - b * spread * R(0.95) + n * spread * R(0.18), n)) * R(0.145);
+    R w = smoothstepv(R(0.62), R(0.95), roughness);
+    return cone + (interactiveDiffuseEnvironment(c, n) - cone) * w;
+}
+template <class R> R interactiveContactOcclusion(Ctx<R> &c, const Hit<R> &h, const Mat<R> &m, V3<R> n) {  // :3882-3944
+    Frame<R> f = makeTangentFrame(n, h.tangent, h.bitangent);
+    const V3<R> local[4] = {V3<R>(R(0.000f), R(0.000f), R(1.000f)), V3<R>(R(0.780f), R(0.180f), R(0.600f)),
+                            V3<R>(R(-0.420f), R(0.720f), R(0.552f)), V3<R>(R(-0.620f), R(-0.520f), R(0.587f))};
+    const R radius = R(0.34f);
+    R occ = R(0);
+    for (int k = 0; k < 4; ++k) {
+        V3<R> l = normalize(local[k]);
+        V3<R> dir = normalize(l.x * f.tangent + l.y * f.bitangent + l.z * n);
+        Ray<R> ray{h.position + n * R(0.003), dir};
+        c.cnt[OR_CNT_SHADOW_QUERIES]++;
+        Hit<R> o = intersectScene(c, ray);
+        if (!o.hit || o.t >= radius) continue;
+        Mat<R> om = materialForHit(c, o);
+        R blocker = clampv(om.baseColor.w, R(0), R(1)) * (R(1) - materialTransmission(om) * R(0.72));
+        occ += blocker * (R(1) - smoothstepv(R(0), radius, o.t));
+    }
+    R strength = R(0.72) + (R(0.36) - R(0.72)) * materialTransmission(m);
+    return R(1) - clampv((occ / R(4)) * strength, R(0), R(0.72));
+}
+
+template <class R>
+void previewPixel(Ctx<R> &c, uint32_t px, uint32_t py, bool interactive, R *acc, R *aov, size_t planeStride) {
+    const or_settings &st = *c.st;
+    const uint32_t width = st.width, height = st.height;
+    Ray<R> ray = makeCameraRay<R>(*c.cam, width, height, px, py, R(0.5), R(0.5), R(0), R(0));
+    c.cnt[OR_CNT_PRIMARY_QUERIES]++;
+    Hit<R> h = intersectScene(c, ray);
+    size_t pix = (size_t(py) * width + px) * 4;
+    R *P[7] = {nullptr};
+    if (aov) for (int k = 0; k < 7; ++k) P[k] = aov + planeStride * k + pix;
+    auto put = [](R *p, R a, R b, R cc, R d) { if (p) { p[0] = a; p[1] = b; p[2] = cc; p[3] = d; } };
+    if (!h.hit) {
+        V3<R> color = vmax(V3<R>(R(st.backgroundColor[0]), R(st.backgroundColor[1]), R(st.backgroundColor[2])), V3<R>(0));
+        if (st.transparentBackground == 0u && st.showsEnvironmentBackground != 0u) color = sampleEnvironment(c, ray.direction);
+        put(acc + pix, color.x, color.y, color.z, st.transparentBackground != 0u ? R(0) : R(1));
+        put(P[0], 0, 0, 0, 0);
+        put(P[1], R(0.5), R(0.5), R(0.5), R(0));
+        for (int k = 2; k < 7; ++k) put(P[k], 0, 0, 0, 0);
+        return;
+    }
+    Mat<R> m = materialForHit(c, h);
+    applyNormalMap(c, h, m, ray.direction);
+    applyBaseColorTexture(c, m, h);
+    V3<R> n = normalize(h.normal), view = normalize(-ray.direction);
+    V3<R> baseColor = vmax(m.baseColor.xyz(), V3<R>(0));
+    V3<R> color;
+    R alpha;
+    V4<R> albedo;
+    if (!interactive) {  // :4838-4845
+        V3<R> key = normalize(V3<R>(R(0.35f), R(0.72f), R(0.58f)));
+        R lambert = fmx(dot(n, key), R(0));
+        R facing = fmx(dot(n, view), R(0));
+        R rim = std::pow(R(1) - facing, R(2)) * R(0.08);
+        R shade = R(0.32) + R(0.68) * lambert + rim;
+        color = baseColor * shade + vmax(m.emission.xyz(), V3<R>(0));
+        alpha = st.transparentBackground != 0u ? clampv(m.baseColor.w, R(0), R(1)) : R(1);
+        albedo = m.baseColor;
+    } else {  // :4967-5140
+        R roughness = clampv(m.parameters.x, R(0.02), R(1));
+        R metallic = clampv(m.parameters.y, R(0), R(1));
+        R opacity = clampv(m.baseColor.w, R(0), R(1));
+        R specular = clampv(m.parameters2.x, R(0), R(1));
+        color = vmax(m.emission.xyz(), V3<R>(0));
+        const or_scene &s = *c.s;
+        uint32_t nl = std::min<uint32_t>(uint32_t(s.lights.size()), 12u);
+        for (uint32_t li = 0; li < nl; ++li) {
+            const or_light &light = s.lights[li];
+            if (light.triangleIndex >= s.triangles.size() || light.materialIndex >= s.materials.size()) continue;
+            const or_triangle &lt = s.triangles[light.triangleIndex];
+            V3<R> emission = vmax(v4<R>(s.materials[light.materialIndex].emission).xyz(), V3<R>(0));
+            if (maxComponent(emission) <= R(0) || R(light.area) <= R(0)) continue;
+            V3<R> lp = (v3<R>(lt.v0) + v3<R>(lt.v1) + v3<R>(lt.v2)) / R(3);
+            V3<R> toLight = lp - h.position;
+            R dist2 = fmx(dot(toLight, toLight), R(1e-5f));
+            R dist = std::sqrt(dist2);
+            V3<R> ldir = toLight / dist;
+            R cosS = fmx(dot(n, ldir), R(0));
+            R cosL = fmx(dot(v4<R>(light.normal).xyz(), -ldir), R(0));
+            if (cosS <= R(0) || cosL <= R(0)) continue;
+            V3<R> tr(1);
+            if (shadowOccluded(c, Ray<R>{h.position + n * R(0.001), ldir}, dist - R(0.002), tr)) continue;
+            V3<R> brdf = evaluateMaterialBRDF(m, n, h.tangent, h.bitangent, view, ldir);
+            R ccGloss = materialClearcoat(m) * std::pow(R(1) - materialClearcoatRoughness(m), R(2));
+            R gloss = clampv(metallic + ccGloss + (R(1) - metallic) * std::pow(R(1) - roughness, R(4)) * specular, R(0), R(1));
+            V3<R> diffuseOnly = (R(1) - metallic) * baseColor * vmax(V3<R>(0), V3<R>(1) - materialF0(m)) * R(0.31830988618f);
+            brdf = diffuseOnly + (brdf - diffuseOnly) * gloss;
+            color = color + brdf * emission * (cosS * cosL * R(light.area) / dist2) * tr;
+        }
+        V3<R> envDiffuse = interactiveDiffuseEnvironment(c, n);
+        V3<R> refl = normalize(reflect(-view, n));
+        V3<R> baseRefl = interactiveSpecularEnvironment(c, refl, n, roughness);
+        R ccRough = materialClearcoatRoughness(m);
+        V3<R> ccRefl = interactiveSpecularEnvironment(c, refl, n, ccRough);
+        R nDotV = fmx(dot(n, view), R(0));
+        V3<R> fresnel = fresnelSchlickThinFilm(nDotV, materialF0(m), m);
+        R glossyW = std::pow(R(1) - roughness, R(4));
+        R metalW = metallic * std::pow(R(1) - roughness, R(2));
+        R ccW = materialClearcoat(m) * std::pow(R(1) - ccRough, R(3));
+        R dielW = (R(1) - metallic) * glossyW * specular;
+        R rw = smoothstepv(R(0.45), R(0.95), roughness);
+        V3<R> roughFresnel = fresnel + (materialF0(m) - fresnel) * rw;
+        V3<R> ccFresnel = fresnelSchlickThinFilm(nDotV, materialClearcoatF0Color(m), m);
+        R horizon = R(0.58) + R(0.42) * clampv(n.y * R(0.5) + R(0.5), R(0), R(1));
+        R contact = interactiveContactOcclusion(c, h, m, n);
+        R ao = horizon * contact;
+        color = color + baseColor * envDiffuse * (R(0.58) * (R(1) - metallic) * ao);
+        color = color + baseRefl * roughFresnel * (metalW * R(1.05) + dielW * R(0.32));
+        color = color + ccRefl * ccFresnel * ccW * R(0.42);
+        V3<R> key = normalize(V3<R>(R(-0.32f), R(0.78f), R(0.52f)));
+        color = color + baseColor * ((R(1) - metallic) * fmx(dot(n, key), R(0)) * R(0.16) * contact);
+        R sss = materialSubsurface(m);
+        if (sss > R(0)) {
+            R wrap = std::pow(R(1) - nDotV, R(2));
+            color = color + materialSubsurfaceColor(m) * baseColor * envDiffuse * (sss * (R(0.12) + R(0.22) * wrap) * contact);
+        }
+        R transmission = materialTransmission(m);
+        if (transmission > R(0)) {
+            V3<R> transmitted = sampleEnvironment(c, ray.direction);
+            V3<R> tt = transmitted * materialTransmissionTint(m, R(0.65));
+            color = color + (tt - color) * (transmission * R(0.55));
+            color = color + baseRefl * fresnel * (transmission * R(0.35));
+        }
+        color = vclamp(color, V3<R>(0), V3<R>(fmx(R(st.sampleRadianceClamp), R(1))));
+        alpha = st.transparentBackground != 0u ? opacity : R(1);
+        albedo = V4<R>{baseColor.x, baseColor.y, baseColor.z, opacity};
+    }
+    put(acc + pix, color.x, color.y, color.z, alpha);
+    if (!aov) return;
+    V3<R> en = n * R(0.5) + V3<R>(R(0.5));
+    put(P[0], h.t, h.t, h.t, R(1));
+    put(P[1], en.x, en.y, en.z, R(1));
+    put(P[2], albedo.x, albedo.y, albedo.z, albedo.w);
+    R mid = R(float(h.materialID + 1u)), oid = R(float(h.objectID + 1u));
+    put(P[3], mid, mid, mid, R(1));
+    put(P[4], oid, oid, oid, R(1));
+    V2<R> cur, prv;
+    if (projectToScreen(*c.cam, h.position, width, height, cur) && projectToScreen(*c.prevCam, h.position, width, height, prv))
+        put(P[5], prv.x - cur.x, prv.y - cur.y, R(0), R(1));
+    else
+        put(P[5], 0, 0, 0, 0);
+    uint32_t tid = h.primitiveID + 1u;
+    R tv;
+    if (sizeof(R) == 4) { float fv; std::memcpy(&fv, &tid, 4); tv = R(fv); }
+    else { tv = R(double(tid)); }
+    put(P[6], tv, tv, tv, tv);
+}
+
 template <class R>
 int renderT(const or_scene *s, const or_settings *st, const or_camera *cam, const or_camera *prev, uint32_t sampleBegin,
             uint32_t sampleCount, int traversal, R *acc, R *aov, uint64_t *counters, int threads) {
@@ -1616,6 +1801,10 @@ int renderT(const or_scene *s, const or_settings *st, const or_camera *cam, cons
         for (int64_t r = 0; r < rows; ++r) {
             uint32_t py = ty0 + uint32_t(r);
             for (uint32_t px = tx0; px < tx0 + tw; ++px) {
+                if (st->renderQuality < 2u) {  // preview / interactive: every sample writes the same image
+                    if (sampleCount) previewPixel<R>(c, px, py, st->renderQuality == 1u, acc, aov, planeStride);
+                    continue;
+                }
                 for (uint32_t k = 0; k < sampleCount; ++k) {
                     bool last = (k + 1 == sampleCount);
                     pathTracePixel<R>(c, px, py, sampleBegin + k, acc, last ? aov : nullptr, planeStride);

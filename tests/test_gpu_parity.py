@@ -255,3 +255,27 @@ def test_hip_render_png_matches_reference_metrics(renderer, name, tmp_path):
     sess.write_png(path)
     rgba8 = np.asarray(Image.open(path).convert("RGBA"))
     _assert_metrics(_image_metrics(rgba8), base)
+
+
+@pytest.mark.parametrize("quality", [RenderQuality.PREVIEW, RenderQuality.INTERACTIVE])
+@pytest.mark.parametrize("scene_name", ["room", "gallery"])
+def test_preview_and_interactive_kernels_parity(renderer, quality, scene_name):
+    # flatPreviewKernel / interactiveMaterialKernel (the kernels the reference
+    # runs for .preview / .interactive) against the oracle's restatement:
+    # IDs exact, AOVs within 1e-5, beauty within the 1-spp bound; a second
+    # sample rewrites the same image (no running mean).
+    scene = synthetic.cornell_room_scene() if scene_name == "room" else synthetic.layered_material_gallery()
+    W, H = 128, 72
+    build = compile_scene(scene)
+    st = RenderSettings(width=W, height=H, max_bounces=4, quality=quality, frame_seed=3)
+    sess = renderer.make_session(renderer.upload(build), st, camera=scene.camera)
+    sess.render(1)
+    ost = O.make_settings(build, W, H, max_bounces=4, frame_seed=3, clamp=st.resolved_sample_radiance_clamp,
+                          quality=quality.value)
+    ref = O.render(O.OracleScene(build), ost, scene.camera.gpu_camera(W, H), sample_count=1)
+    _check_aovs(sess, ref)
+    bad = _check_beauty(sess, ref)
+    first = sess.pixels(RenderOutput.BEAUTY)
+    sess.render(2)
+    assert np.array_equal(sess.pixels(RenderOutput.BEAUTY), first) and sess.sample_count == 3
+    print(f"{scene_name} {quality.name}: channels outside tol {bad:.4%}")
