@@ -25,10 +25,10 @@ sys.path.insert(0, HERE)
 
 METRIC = "Mrays/s & spp/s at 1080p; preview p50/p99 latency at equal relMSE"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per unit: a node fetch tests two 32-B child boxes, a
+# Algorithmic bytes per unit: a node fetch tests four child boxes (128 B), a
 # triangle test reads 36 B of positions, each query reads its 32-B ray and
 # writes a 16-B hit.
-NODE_BYTES, TRI_BYTES, RAY_BYTES = 64, 36, 48
+NODE_BYTES, TRI_BYTES, RAY_BYTES = 128, 36, 48
 # Sparse-volume units: a DDA cell reads its 4-B brick index and 4-B macro
 # word; a brick test reads the 64-B brick header; a march step reads the 8
 # FP16 corner samples (16 B); a hit adds 8 bisection lookups (8 x 16 B), two

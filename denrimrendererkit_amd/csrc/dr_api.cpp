@@ -82,7 +82,7 @@ struct PathLayout {
     size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, stats, base, ovf;
     size_t bytes;
 };
-static PathLayout pathLayout(size_t N, size_t stride) {
+static PathLayout pathLayout(size_t N, size_t stride, uint32_t stackMax) {
     PathLayout l{};
     size_t bytes = 0;
     auto take = [&](size_t n) {
@@ -95,7 +95,7 @@ static PathLayout pathLayout(size_t N, size_t stride) {
     l.shO = take(3 * N * 16), l.shD = take(3 * N * 16), l.shC = take(3 * N * 16), l.shR = take(3 * N * 16);
     l.q0 = take(N * 4), l.q1 = take(N * 4), l.sq = take(3 * N * 4), l.sss = take(N * 4), l.res = take(N * 4);
     l.ctr = take(DR_CTR_WORDS * 4), l.stats = take(DR_CNT_SLOTS * 8), l.base = take(4);
-    l.ovf = take(stride * (DR_STACK_MAX - DR_LDS_STACK) * 4);
+    l.ovf = take(stride * (stackMax > DR_LDS_STACK ? stackMax - DR_LDS_STACK : 0u) * 4);
     l.bytes = bytes;
     return l;
 }
@@ -224,11 +224,12 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
 
     auto *s = new dr_scene;
     s->device = device;
-    std::vector<DrNode> nodes;
+    std::vector<DrNode4> nodes;
     std::vector<float4> triPos;
     if (d->triangle_count)
         dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos,
                      &s->stats);
+    s->stackMax = 4u * s->stats.maxDepth + 2u;
 
     // Sparse volumes: compact brick headers, FP16 distances when every
     // distance is FP16-exact (else FP32), per-sample material pairs with the
@@ -455,7 +456,7 @@ dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t co
     DrSceneDev S = scene->dev;
     S.watertight = intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
     const size_t lanes = (size_t(count) + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK * DR_TRACE_BLOCK;
-    const size_t bytes = size_t(count) * 32 + size_t(count) * 8 + lanes * (DR_STACK_MAX - DR_LDS_STACK) * 4;
+    const size_t bytes = size_t(count) * 32 + size_t(count) * 8 + lanes * (scene->stackMax > DR_LDS_STACK ? scene->stackMax - DR_LDS_STACK : 0u) * 4;
     char *buf = nullptr;
     DR_HIP(hipMalloc(&buf, bytes), "intersect buffers");
     float4 *dRays = reinterpret_cast<float4 *>(buf);
@@ -513,7 +514,7 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
     const uint32_t full = uint32_t(std::max(1, scene->device->cuCount)) * 16u;
     sh.traceGrid = std::max(1u, std::min(full, (sh.nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
     sh.stride = std::max(sh.nSlots, sh.traceGrid * DR_TRACE_BLOCK);
-    sh.lay = pathLayout(sh.nSlots, sh.stride);
+    sh.lay = pathLayout(sh.nSlots, sh.stride, scene->stackMax);
     if (st->denoiser == DR_DENOISER_APPLE_SVGF)  // RenderSession.swift:575-576
         return fail(DR_ERR_INVALID_SCENE, "Apple SVGF denoising is not available on this platform.");
     if (st->denoiser > DR_DENOISER_LEARNED) return fail(DR_ERR_INVALID_ARGUMENT, "unknown denoiser");

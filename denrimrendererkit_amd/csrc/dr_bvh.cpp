@@ -4,8 +4,10 @@
 // index array (Acceleration/BVHBuilder.swift:4-135, FlatBVH.swift:13-33).
 // Closest-hit results with the lowest-index tie-break do not depend on the
 // tree, so this backend builds its own: binned SAH (32 bins, 3 axes), leaves
-// of at most `maxLeaf` triangles, written as two-box nodes in depth-first
-// order with leaf triangles stored contiguously in leaf order.
+// of at most `maxLeaf` triangles, collapsed into four-wide nodes (each node
+// takes its two children and opens its largest inner grandchildren until it
+// has four) in depth-first order, leaf triangles stored contiguously in leaf
+// order.
 #include "dr_bvh.h"
 
 #include <algorithm>
@@ -203,16 +205,18 @@ struct Flattener {
     const std::vector<TNode> &tn;
     const std::vector<uint32_t> &idx;
     const DrTriangle *tris;
-    std::vector<DrNode> &out;
+    std::vector<DrNode4> &out;
     std::vector<float4> &pos;
     uint32_t leaves = 0;
+    uint32_t depth = 0;
     double sah = 0.0;
     double rootArea = 1.0;
 
-    void emitLeafTris(const TNode &n, uint32_t &first) {
+    // Triangles [start, start + count) of the index list, appended in leaf order.
+    void emitLeafTris(const TNode &n, uint32_t start, uint32_t count, uint32_t &first) {
         first = uint32_t(pos.size() / 3);
-        for (uint32_t k = 0; k < n.count; ++k) {
-            uint32_t orig = idx[n.first + k];
+        for (uint32_t k = 0; k < count; ++k) {
+            uint32_t orig = idx[start + k];
             const DrTriangle &t = tris[orig];
             float4 v0 = t.v0, v1 = t.v1, v2 = t.v2;
             uint32_t bits = orig;
@@ -223,41 +227,114 @@ struct Flattener {
             pos.push_back(make_float4(v2.x, v2.y, v2.z, 0.f));
         }
         leaves++;
-        sah += double(n.box.area()) / rootArea * kTriCost * n.count;
+        sah += double(n.box.area()) / rootArea * kTriCost * count;
     }
 
-    // Returns the index of the emitted node.
-    uint32_t emit(int32_t ni) {
-        const TNode &n = tn[ni];
-        uint32_t me = uint32_t(out.size());
-        out.push_back(DrNode{});
-        sah += double(n.box.area()) / rootArea * kNodeCost;
-        int32_t ch[2] = {n.left, n.right};
-        uint32_t ref[2], cnt[2];
-        float bx[2][6];
-        for (int c = 0; c < 2; ++c) {
+    // A leaf of more triangles than a node's 8-bit count holds (only where
+    // the depth cap forced it): a node of up to four leaf chunks of <= 255
+    // triangles, nested further when needed; every chunk has the leaf's box.
+    uint32_t emitBigLeaf(const TNode &leaf, uint32_t start, uint32_t count, uint32_t level) {
+        depth = std::max(depth, level);
+        const uint32_t me = uint32_t(out.size());
+        out.push_back(DrNode4{});
+        float bx[6];
+        childBox(leaf.box, bx);
+        uint32_t ref[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0}, valid = 0;
+        const uint32_t per = count <= 4u * 255u ? 255u : (count + 3u) / 4u;
+        for (int c = 0; c < 4 && count > 0; ++c) {
+            const uint32_t n = std::min(per, count);
+            valid |= 1u << c;
+            if (n <= 255u) {
+                emitLeafTris(leaf, start, n, ref[c]);
+                cnt[c] = n;
+            } else {
+                ref[c] = emitBigLeaf(leaf, start, n, level + 1);
+            }
+            start += n;
+            count -= n;
+        }
+        DrNode4 &o = out[me];
+        o.lox = make_float4(bx[0], bx[0], bx[0], bx[0]);
+        o.loy = make_float4(bx[1], bx[1], bx[1], bx[1]);
+        o.loz = make_float4(bx[2], bx[2], bx[2], bx[2]);
+        o.hix = make_float4(bx[3], bx[3], bx[3], bx[3]);
+        o.hiy = make_float4(bx[4], bx[4], bx[4], bx[4]);
+        o.hiz = make_float4(bx[5], bx[5], bx[5], bx[5]);
+        o.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid, 0u, 0u);
+        return me;
+    }
+
+    // Up to four children of binary node ni: its two children, then the
+    // largest-area inner child opened into its two, while there is room.
+    int gather(int32_t ni, int32_t ch[4]) const {
+        int n = 0;
+        ch[n++] = tn[ni].left;
+        ch[n++] = tn[ni].right;
+        while (n < 4) {
+            int best = -1;
+            float area = -1.0f;
+            for (int i = 0; i < n; ++i) {
+                const TNode &c = tn[ch[i]];
+                if (c.count == 0 && c.box.area() > area) {
+                    area = c.box.area();
+                    best = i;
+                }
+            }
+            if (best < 0) break;
+            const int32_t e = ch[best];
+            ch[best] = tn[e].left;
+            ch[n++] = tn[e].right;
+        }
+        return n;
+    }
+
+    // Children are given as binary-tree nodes (n of them, 1...4).
+    uint32_t emitNode(const int32_t *ch, int n, uint32_t level) {
+        depth = std::max(depth, level);
+        const uint32_t me = uint32_t(out.size());
+        out.push_back(DrNode4{});
+        float lo[3][4], hi[3][4];
+        uint32_t ref[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0}, valid = 0;
+        for (int c = 0; c < 4; ++c)
+            for (int a = 0; a < 3; ++a) lo[a][c] = hi[a][c] = 0.f;
+        for (int c = 0; c < n; ++c) {
             const TNode &cn = tn[ch[c]];
-            childBox(cn.box, bx[c]);
-            if (cn.count > 0) {
-                emitLeafTris(cn, ref[c]);
+            float bx[6];
+            childBox(cn.box, bx);
+            for (int a = 0; a < 3; ++a) {
+                lo[a][c] = bx[a];
+                hi[a][c] = bx[3 + a];
+            }
+            valid |= 1u << c;
+            if (cn.count > 255u) {
+                ref[c] = emitBigLeaf(cn, cn.first, cn.count, level + 1);
+            } else if (cn.count > 0) {
+                emitLeafTris(cn, cn.first, cn.count, ref[c]);
                 cnt[c] = cn.count;
             } else {
-                ref[c] = emit(ch[c]);
-                cnt[c] = 0;
+                int32_t g[4];
+                const int gn = gather(ch[c], g);
+                sah += double(cn.box.area()) / rootArea * kNodeCost;
+                ref[c] = emitNode(g, gn, level + 1);
             }
         }
-        DrNode &o = out[me];
-        o.a = make_float4(bx[0][0], bx[0][1], bx[0][2], bx[0][3]);
-        o.b = make_float4(bx[0][4], bx[0][5], bx[1][0], bx[1][1]);
-        o.c = make_float4(bx[1][2], bx[1][3], bx[1][4], bx[1][5]);
-        o.d = make_uint4(ref[0], ref[1], cnt[0], cnt[1]);
+        DrNode4 &o = out[me];
+        o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        o.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid, 0u, 0u);
         return me;
     }
 };
 
 }  // namespace
 
-void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode> &nodes,
+void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode4> &nodes,
               std::vector<float4> &triPos, BvhBuildStats *stats) {
     auto t0 = std::chrono::steady_clock::now();
     nodes.clear();
@@ -283,29 +360,23 @@ void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vec
     b.build(root, 0, count, 0);
 
     triPos.reserve(size_t(count) * 3);
-    nodes.reserve(count);
+    nodes.reserve(count / 2 + 1);
     Flattener f{tn, idx, tris, nodes, triPos};
     f.rootArea = std::max(double(tn[root].box.area()), 1e-30);
+    f.sah = kNodeCost;
     if (tn[root].count > 0) {
-        // Whole scene in one leaf: a root whose second child is empty.
-        DrNode r{};
-        float bx[6];
-        childBox(tn[root].box, bx);
-        uint32_t first;
-        nodes.push_back(r);
-        f.emitLeafTris(tn[root], first);
-        DrNode &o = nodes[0];
-        o.a = make_float4(bx[0], bx[1], bx[2], bx[3]);
-        o.b = make_float4(bx[4], bx[5], 0.f, 0.f);
-        o.c = make_float4(0.f, 0.f, 0.f, 0.f);
-        o.d = make_uint4(first, DR_EMPTY_CHILD, tn[root].count, 0);
+        // Whole scene in one leaf: a root with one leaf child.
+        const int32_t only = int32_t(root);
+        f.emitNode(&only, 1, 1);
     } else {
-        f.emit(int32_t(root));
+        int32_t g[4];
+        const int gn = f.gather(int32_t(root), g);
+        f.emitNode(g, gn, 1);
     }
     if (stats) {
         stats->nodeCount = uint32_t(nodes.size());
         stats->leafCount = f.leaves;
-        stats->maxDepth = b.maxDepth.load();
+        stats->maxDepth = f.depth;
         stats->sahCost = f.sah;
         stats->buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }

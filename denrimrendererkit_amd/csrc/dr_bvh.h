@@ -17,8 +17,10 @@ struct BvhBuildStats {
 };
 
 // Builds a binned-SAH binary BVH over the triangles' world-space vertices
-// (reference GPUTriangle records) and writes the traversal layout:
-//   nodes  -- DrNode per inner node, depth-first pre-order, root at 0;
+// (reference GPUTriangle records), collapses it to a 4-wide BVH (each node
+// takes up to four children by repeatedly opening its largest-area inner
+// child) and writes the traversal layout:
+//   nodes  -- DrNode4 per inner node, depth-first pre-order, root at 0;
 //   triPos -- 3 float4 per triangle in leaf order: {v0, bits(orig index)},
 //             {v1, 0}, {v2, 0} (exact vertices: the watertight test needs
 //             them; the Moller-Trumbore test forms v1 - v0, v2 - v0 itself,
@@ -26,14 +28,12 @@ struct BvhBuildStats {
 // Child boxes are widened outward by a few ulps of their magnitude so the
 // fused-multiply-add slab test can never cull a box that the exact test
 // would keep; that keeps the closest hit identical to a brute-force search.
-// Depth is capped at DR_MAX_DEPTH so the traversal stack can never overflow.
-void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode> &nodes,
+// Depth is capped at DR_MAX_DEPTH; stats->maxDepth is the 4-wide tree's
+// depth, and a traversal lane needs at most 4 stack entries per level
+// (three inner children and one group of leaf children) plus two.
+void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode4> &nodes,
               std::vector<float4> &triPos, BvhBuildStats *stats);
 
 }  // namespace dr
 
 #define DR_MAX_DEPTH 64
-// Traversal stack capacity: one entry per inner level plus two leaf
-// references pushed at the last level.
-#define DR_STACK_MAX (DR_MAX_DEPTH + 2)
-#define DR_EMPTY_CHILD 0xffffffffu

@@ -21,7 +21,8 @@ struct dr_device {
 struct dr_scene {
     dr_device *device = nullptr;
     DrSceneDev dev{};
-    DrNode *nodes = nullptr;
+    DrNode4 *nodes = nullptr;
+    uint32_t stackMax = 2;  // traversal-stack entries a lane can need in this scene's BVH
     float4 *triPos = nullptr;
     DrTriangle *triangles = nullptr;
     DrMaterial *materials = nullptr;

@@ -10,7 +10,8 @@
 #define DR_SHADE_BLOCK 128
 #define DR_MAX_BOUNCES 64
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
-// per-lane global area of (DR_STACK_MAX - DR_LDS_STACK) words.
+// per-lane global area of (stackMax - DR_LDS_STACK) words, stackMax being
+// the scene's bound of 4 entries per BVH level plus two.
 #define DR_LDS_STACK 24
 
 // Device counter words: queue lengths per bounce (index b = queue consumed

@@ -41,16 +41,18 @@ struct DrLight {
 };
 static_assert(sizeof(DrLight) == 32, "GPULightRecord layout");
 
-// Traversal node: both children's boxes plus child references.
-//   a = (c0.min.xyz, c0.max.x)  b = (c0.max.yz, c1.min.xy)
-//   c = (c1.min.z, c1.max.xyz)  d = (child0, child1, count0, count1)
-// count > 0: the child is a leaf, child = first triangle in leaf order;
-// count == 0: the child is an inner node index.
-struct DrNode {
-    float4 a, b, c;
-    uint4 d;
+// Traversal node of the 4-wide BVH, one 128-B L2 line: the four child
+// boxes as planes of x/y/z bounds (component c = child c), the child
+// references, then meta.x = the four leaf triangle counts (8 bits each,
+// child c at bits 8c; 0 = inner node) and meta.y = the mask of present
+// children. A leaf child's reference is its first triangle in leaf order,
+// an inner child's its node index.
+struct DrNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 child;
+    uint4 meta;
 };
-static_assert(sizeof(DrNode) == 64, "node layout");
+static_assert(sizeof(DrNode4) == 128, "node layout");
 
 // Compact brick header (64 B).
 struct DrBrick {
@@ -101,7 +103,7 @@ struct DrSdfDev {
 };
 
 struct DrSceneDev {
-    const DrNode *nodes;
+    const DrNode4 *nodes;
     const float4 *triPos;        // 3 float4 per triangle in leaf order
     const DrTriangle *triangles; // caller's records, original order
     const DrMaterial *materials;

@@ -1,11 +1,12 @@
 // dr_traverse.h -- software BVH traversal and triangle test for gfx950.
 //
 // CDNA4 has no ray-tracing units, so traversal is plain VALU work:
-//   * one node fetch (4 x dwordx4, 64 B) tests both child boxes with a
-//     fused-multiply-add slab test against precomputed 1/d and o/d;
-//   * while-while order: inner nodes (nearer child first, the farther one
-//     pushed) until every lane of the wave holds a hit leaf, then the
-//     leaves' triangles, converged;
+//   * 4-wide BVH: one node fetch (8 x dwordx4, 128 B) tests four child
+//     boxes with a fused-multiply-add slab test against precomputed 1/d and
+//     o/d, so a ray needs about half the dependent fetches of a binary tree;
+//   * while-while order: inner nodes (nearest hit child next, the others
+//     pushed far to near) until every lane of the wave holds a hit leaf,
+//     then the leaves' triangles, converged;
 //   * the traversal stack lives in LDS (DR_LDS_STACK entries per lane,
 //     lane-interleaved so a wave's pushes hit 64 distinct banks) and spills
 //     to a per-lane global area only past that depth.
@@ -163,9 +164,11 @@ struct DrHitRec {
 // visited until the lane holds a pending leaf (or finishes); leaves are
 // tested when every lane of the wave holds one, so triangle tests run
 // converged instead of serializing the wave once per lane.
-// Stack entries: an inner node index, or DR_LEAF_REF | node << 1 | child for
-// a hit leaf child waiting to be tested (its range is read back from the
-// node). The tree depth cap bounds the stack: one push per visited node.
+// Stack entries: an inner node index, or DR_LEAF_REF | node << 4 | mask for
+// the hit leaf children (mask bit c = child c) of a node still to be tested;
+// popping one takes its first child and pushes the rest back. A node visit
+// pushes at most three inner children and one leaf group, so a lane needs
+// at most 4 entries per tree level plus two (the scene's stackMax).
 #define DR_POP 0xffffffffu
 #define DR_LEAF_REF 0x80000000u
 
@@ -185,6 +188,25 @@ __device__ __forceinline__ void dr_trav_init(DrTrav &s) {
     s.done = false;
 }
 
+__device__ __forceinline__ uint32_t dr_sel4(uint4 v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
+
+// Child of `mask` with the smallest (or largest) entry distance; ties go
+// to the lower child.
+__device__ __forceinline__ int dr_pick(const float t[4], uint32_t mask, bool nearest) {
+    int best = -1;
+    float bt = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if ((mask >> c) & 1u) {
+            if (best < 0 || (nearest ? t[c] < bt : t[c] > bt)) {
+                best = c;
+                bt = t[c];
+            }
+        }
+    }
+    return best;
+}
+
 // One inner-node step; requires !s.done and either no pending leaf, or a
 // known next node (speculative step: leaves found then go to the stack).
 __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre &r, float tmax, DrTrav &s,
@@ -196,46 +218,56 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
         }
         uint32_t e = stk.get(--s.sp);
         if (e & DR_LEAF_REF) {
-            uint32_t node = (e & ~DR_LEAF_REF) >> 1;
-            uint4 d = S.nodes[node].d;
-            bool c1 = (e & 1u) != 0u;
-            s.leafFirst = c1 ? d.y : d.x;
-            s.leafCount = c1 ? d.w : d.z;
+            const uint32_t node = (e & ~DR_LEAF_REF) >> 4;
+            uint32_t mask = e & 15u;
+            const int c = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint4 ch = S.nodes[node].child;
+            const uint32_t counts = S.nodes[node].meta.x;
+            s.leafFirst = dr_sel4(ch, c);
+            s.leafCount = (counts >> (8 * c)) & 0xffu;
+            if (mask) stk.put(s.sp++, DR_LEAF_REF | node << 4 | mask);
             return;
         }
         s.cur = e;
     }
     const uint32_t ni = s.cur;
-    const DrNode n = S.nodes[ni];
+    const DrNode4 n = S.nodes[ni];
     ++nv;
-    float t0 = dr_box(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tmax);
-    float t1 = (n.d.y == DR_EMPTY_CHILD) ? __builtin_inff() : dr_box(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tmax);
-    bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
-    bool l0 = h0 && n.d.z != 0u, l1 = h1 && n.d.w != 0u;
-    bool i0 = h0 && n.d.z == 0u, i1 = h1 && n.d.w == 0u;
-    bool near0 = t0 <= t1;
-    if (s.leafCount != 0u) {
-        // Speculative: keep the pending leaf, stack the new ones (nearer on top).
-        if (l0 && l1) stk.put(s.sp++, DR_LEAF_REF | ni << 1 | (near0 ? 1u : 0u));
-        if (l0 || l1) stk.put(s.sp++, DR_LEAF_REF | ni << 1 | ((l0 && l1) ? (near0 ? 0u : 1u) : (l0 ? 0u : 1u)));
-    } else if (l0 && l1) {
-        s.leafFirst = near0 ? n.d.x : n.d.y;
-        s.leafCount = near0 ? n.d.z : n.d.w;
-        stk.put(s.sp++, DR_LEAF_REF | ni << 1 | (near0 ? 1u : 0u));
-    } else if (l0) {
-        s.leafFirst = n.d.x;
-        s.leafCount = n.d.z;
-    } else if (l1) {
-        s.leafFirst = n.d.y;
-        s.leafCount = n.d.w;
+    float t[4];
+    t[0] = dr_box(r, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmax);
+    t[1] = dr_box(r, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmax);
+    t[2] = dr_box(r, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmax);
+    t[3] = dr_box(r, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmax);
+    const uint32_t counts = n.meta.x, valid = n.meta.y;
+    uint32_t leafMask = 0u, innerMask = 0u;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (((valid >> c) & 1u) && t[c] != __builtin_inff()) {
+            if ((counts >> (8 * c)) & 0xffu) leafMask |= 1u << c;
+            else innerMask |= 1u << c;
+        }
     }
-    if (i0 && i1) {
-        stk.put(s.sp++, near0 ? n.d.y : n.d.x);
-        s.cur = near0 ? n.d.x : n.d.y;
-    } else if (i0) {
-        s.cur = n.d.x;
-    } else if (i1) {
-        s.cur = n.d.y;
+    if (leafMask) {
+        if (s.leafCount == 0u) {
+            const int c = dr_pick(t, leafMask, true);
+            s.leafFirst = dr_sel4(n.child, c);
+            s.leafCount = (counts >> (8 * c)) & 0xffu;
+            leafMask &= ~(1u << c);
+        }
+        if (leafMask) stk.put(s.sp++, DR_LEAF_REF | ni << 4 | leafMask);
+    }
+    if (innerMask) {
+        // The nearest inner child is next; the others go to the stack, the
+        // farthest first so that nearer ones are popped first.
+        const int nc = dr_pick(t, innerMask, true);
+        innerMask &= ~(1u << nc);
+        while (innerMask) {
+            const int fc = dr_pick(t, innerMask, false);
+            stk.put(s.sp++, dr_sel4(n.child, fc));
+            innerMask &= ~(1u << fc);
+        }
+        s.cur = dr_sel4(n.child, nc);
     } else {
         s.cur = DR_POP;
     }
