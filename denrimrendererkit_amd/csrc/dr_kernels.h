@@ -27,7 +27,12 @@
 #define DR_CTR_SSSQ (DR_CTR_SHADOW + DR_MAX_BOUNCES)
 #define DR_CTR_SSS (DR_CTR_SSSQ + DR_MAX_BOUNCES)
 #define DR_CTR_RESUMEQ (DR_CTR_SSS + DR_MAX_BOUNCES)
-#define DR_CTR_WORDS (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
+// First shadow-queue entry of the bounce left to k_shadow: the triangle-only
+// k_sss traces the entries queued before it ran (those of k_shade) beside
+// its walks and stores their count here; k_shadow takes the rest (those of
+// k_shade<RESUME>). Zero when k_sss does not trace shadow rays.
+#define DR_CTR_SHBEGIN (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
+#define DR_CTR_WORDS (DR_CTR_SHBEGIN + DR_MAX_BOUNCES)
 
 struct DrPaths {
     float4 *rayO;   // origin.xyz, previousBSDFPDF

@@ -35,6 +35,14 @@
 #define DR_FLAG_PRIMARY 1u
 #define DR_FLAG_NEE 2u
 
+#ifndef DR_TRACE_WAVES
+#define DR_TRACE_WAVES 1
+#endif
+#define DR_TRACE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_TRACE_WAVES)))
+#ifndef DR_SSS_WAVES
+#define DR_SSS_WAVES 4
+#endif
+#define DR_SSS_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SSS_WAVES)))
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
 // Sample index of the batch's first sample: a launch argument, or a device
@@ -146,7 +154,7 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
 
 // ---------------------------------------------------------------- extend --
 template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_extend(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t *queue = P.queue[bounce & 1u];
     const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
@@ -770,105 +778,238 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
 // the nested closest-hit queries run converged. A successful walk exits the
 // surface (then RR, :5359-5363); a failed one goes to k_shade<RESUME> with
 // its throughput and RNG state.
+// ------------------------------------------------ refill-tracer items --
+// A work item of the refill tracer: its current ray and the state it carries
+// between queries. Shadow rays and subsurface walks share the fields so that
+// one kernel can trace both without holding two sets of registers.
+struct DrItem {
+    uint32_t id;    // shadow record index, or the path slot of a walk
+    uint32_t step;  // queries done
+    vec3 o, d;      // current ray
+    vec3 w;         // shadow: transmittance; walk: path throughput
+    float a, b;     // shadow: maxDistance, traveled; walk: anisotropy g, output alpha
+};
+
+// shadowOccluded (PathTrace.metal:4196-4274) for one queued NEE ray as an
+// item: a chain of closest hits, one query per step (anyHit = false), or one
+// any-hit query below maxDistance when no material can let light through
+// (anyHit = true; the limit is exclusive: id 0 never wins a tie). The
+// transmittance (0 if occluded) goes to shR[k].
+// Hits at or beyond maxDistance - traveled end the chain, so each query is
+// limited there: exactly (exclusive limit) on the first step, with a
+// relative margin afterwards where the reference test
+// traveled + t >= maxDistance decides.
+__device__ __forceinline__ DrHitRec shadowLimit(const DrItem &it) {
+    return it.step == 0u ? DrHitRec{it.a, 0.0f, 0.0f, 0u}
+                         : DrHitRec{(it.a - it.b) * 1.000001f + 1e-30f, 0.0f, 0.0f, DR_MISS};
+}
+__device__ __forceinline__ void shadowBegin(const DrPaths &P, uint32_t k, DrItem &it, DrRayPre &r, DrHitRec &best) {
+    float4 o4 = P.shO[k], d4 = P.shD[k];
+    it.id = k;
+    it.step = 0u;
+    it.o = v3(o4.x, o4.y, o4.z);
+    it.d = v3(d4.x, d4.y, d4.z);
+    it.w = v3s(1.0f);
+    it.a = o4.w;
+    it.b = 0.0f;
+    r = dr_prep(it.o, it.d);
+    best = shadowLimit(it);
+}
+// Consumes a query's result; true when the chain continues with the next
+// query (set up in r / best).
+__device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P, bool anyHit, const DrHitRec &h,
+                                           bool hit, DrItem &it, DrRayPre &r, DrHitRec &best, uint32_t &nq) {
+    if (anyHit) {
+        P.shR[it.id] = hit ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        return false;
+    }
+    bool occluded = true, next = false;
+    if (!hit || it.b + h.t >= it.a) {
+        occluded = false;
+    } else {
+        float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        Surf sh = surfaceAt<false>(S, it.o, it.d, h4, h4);
+        Mat m = materialFor(S, sh);
+        if (m.base.w <= 0.001f) {
+            it.b += sh.t;
+            it.o = sh.position + it.d * 0.001f;
+            next = true;
+        } else {
+            float transmission = materialTransmission(m);
+            if (transmission > 0.001f) {
+                bool inside = !sh.frontFacing && !materialThinWalled(m);
+                vec3 medium = materialHasVolumeScattering(m) && inside ? materialVolumeTransmittance(m, sh.t)
+                                                                       : materialTransmissionAbsorption(m, sh.t, inside);
+                it.w = it.w * materialTransmissionTint(m, 0.35f) * medium * transmission;
+                if (maxComponent(it.w) >= 0.02f) {
+                    it.b += sh.t;
+                    it.o = sh.position + it.d * 0.001f;
+                    next = true;
+                }
+            }
+        }
+    }
+    if (next && ++it.step < 8u) {
+        r = dr_prep(it.o, it.d);
+        best = shadowLimit(it);
+        ++nq;
+        return true;
+    }
+    P.shR[it.id] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(it.w.x, it.w.y, it.w.z, 1.0f);
+    return false;
+}
+
+// continueSubsurfaceRandomWalk (PathTrace.metal:3525-3621) on triangles as
+// an item: one closest-hit query per walk segment.
+struct DrWalk {
+    vec3 sigmaT, albedo;
+    uint32_t state;
+    uint4 meta;
+};
+__device__ __forceinline__ void walkBegin(const DrSceneDev &S, const DrPaths &P, uint32_t slot, DrItem &it, DrWalk &wk,
+                                          DrRayPre &r, DrHitRec &best) {
+    float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
+    wk.meta = P.meta[slot];
+    float4 h4 = P.hit[slot];
+    vec3 hitO = v3(o4.x, o4.y, o4.z), hitD = v3(d4.x, d4.y, d4.z);
+    Surf h = surfaceAt<false>(S, hitO, hitD, h4, h4);
+    Mat m = materialFor(S, h);
+    applyNormalMap(S, h, m, hitD);
+    applyBaseColorTexture(S, m, h);
+    it.id = slot;
+    it.w = v3(t4.x, t4.y, t4.z);
+    it.b = t4.w;
+    wk.state = wk.meta.z;
+    // Entry (:3534-3548): cosine-weighted into the surface.
+    float e0 = randomFloat(wk.state);
+    float e1 = randomFloat(wk.state);
+    vec3 entryDir = orientHemisphere(cosineHemisphere(e0, e1), -h.normal);
+    it.o = h.position + entryDir * 0.001f;
+    it.d = entryDir;
+    wk.sigmaT = v3s(1.0f) / materialSubsurfaceRadius(m);
+    wk.albedo = materialSubsurfaceColor(m);
+    it.a = materialSubsurfaceAnisotropy(m);
+    it.step = 0u;
+    r = dr_prep(it.o, it.d);
+    best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+}
+// Consumes a segment's query; true when the walk continues (next segment
+// set up in r / best). A finished walk writes the path back and queues it:
+// for the next bounce when it exits alive, for k_shade<RESUME> when it fails.
+__device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L, const DrPaths &P, uint32_t bounce,
+                                         const DrHitRec &hr, bool hit, DrItem &it, DrWalk &wk, DrRayPre &r,
+                                         DrHitRec &best, uint32_t &cBounce, uint32_t &cSss) {
+    bool exited = false, more = false;
+    vec3 ro = it.o, rd = it.d;
+    float prevPdf = 0.0f;
+    if (hit) {
+        float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
+        Surf b = surfaceAt<false>(S, it.o, it.d, h4, h4);
+        float f0 = randomFloat(wk.state);
+        float f1 = randomFloat(wk.state);
+        float ff = sampleFreeFlight(wk.sigmaT, f0, f1);
+        if (ff < b.t) {
+            it.o += it.d * ff;
+            float h0 = randomFloat(wk.state);
+            float h1 = randomFloat(wk.state);
+            it.d = sampleHenyeyGreenstein(h0, h1, it.d, it.a);
+            vec3 tr = vexp((-wk.sigmaT) * ff);
+            it.w *= wk.albedo * wk.sigmaT * tr / collisionPDF(wk.sigmaT, ff);
+            more = !(maxComponent(it.w) <= 1e-4f || ++it.step >= 8u);
+            ro = it.o;
+            rd = it.d;
+        } else {
+            vec3 exitN = normalize(b.frontFacing ? b.normal : -b.normal);
+            float x0 = randomFloat(wk.state);
+            float x1 = randomFloat(wk.state);
+            rd = orientHemisphere(cosineHemisphere(x0, x1), exitN);
+            ro = b.position + exitN * 0.001f;
+            vec3 tr = vexp((-wk.sigmaT) * b.t);
+            it.w *= tr / boundaryPDF(wk.sigmaT, b.t);
+            prevPdf = fmaxf(dot(exitN, rd), 0.0f) * 0.31830988618f;
+            exited = true;
+        }
+    }
+    if (more) {
+        cBounce++;
+        cSss++;
+        r = dr_prep(it.o, it.d);
+        best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+        return true;
+    }
+    const uint32_t slot = it.id;
+    bool alive = false;
+    if (exited) {
+        alive = !terminatePathWithRussianRoulette(bounce, it.w, wk.state) && (bounce + 1u < L.maxBounces);
+        P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
+        P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+    }
+    P.thr[slot] = make_float4(it.w.x, it.w.y, it.w.z, it.b);
+    P.meta[slot] = make_uint4(wk.meta.x, wk.meta.y, wk.state, wk.meta.w);
+    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+    if (alive) P.queue[(bounce + 1u) & 1u][pos] = slot;
+    const bool resume = !exited;
+    uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
+    if (resume) P.resumeQueue[rpos] = slot;
+    return false;
+}
+
+// ------------------------------------------------------------ subsurface --
+// The bounce's subsurface walks. On triangles the kernel also traces the
+// bounce's shadow rays queued so far (by k_shade): walks are claimed first
+// and are chains of up to 8 dependent queries, so the shadow rays fill the
+// lanes around them instead of waiting for the longest walk; k_shadow then
+// traces only the shadow rays k_shade<RESUME> adds after the failed walks.
+// Results are the same as tracing them in separate kernels: each item
+// writes only its own records.
 template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L, DrPaths P, uint32_t bounce) {
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S, DrLaunch L, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t count = P.counters[DR_CTR_SSSQ + bounce];
     uint32_t *claim = &P.counters[DR_CTR_SSS + bounce];
-    uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
+    uint32_t shQ = 0u, shNodes = 0u, shTris = 0u;
     DrSdfCounters sc{};
     if constexpr (!VOL) {
-        // continueSubsurfaceRandomWalk (PathTrace.metal:3525-3621) as a
-        // chain of closest-hit queries, one per walk segment.
-        uint32_t slot = 0, state = 0, step = 0;
-        vec3 mo, md, T, sigmaT, albedo;
-        float g = 0.0f, alpha = 0.0f;
-        uint4 meta;
-        auto finish = [&](bool exited, vec3 ro, vec3 rd, float prevPdf) {
-            bool alive = false;
-            if (exited) {
-                alive = !terminatePathWithRussianRoulette(bounce, T, state) && (bounce + 1u < L.maxBounces);
-                P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
-                P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
-            }
-            P.thr[slot] = make_float4(T.x, T.y, T.z, alpha);
-            P.meta[slot] = make_uint4(meta.x, meta.y, state, meta.w);
-            uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
-            if (alive) queueOut[pos] = slot;
-            bool resume = !exited;
-            uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
-            if (resume) P.resumeQueue[rpos] = slot;
-        };
+        const uint32_t nShadow = P.counters[DR_CTR_SHADOWQ + bounce];
+        if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[DR_CTR_SHBEGIN + bounce] = nShadow;
+        const bool anyShadow = S.opaqueOnly != 0u;
+        DrItem it{};
+        DrWalk wk{};
+        bool walk = false, laneAny = false;
+        uint32_t markV = 0u, markT = 0u;
         dr_trace_queue<false>(
-            S, count, claim, stk, cNodes, cTris,
+            S, count + nShadow, claim, stk, cNodes, cTris,
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                slot = P.sssQueue[qi];
-                float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
-                meta = P.meta[slot];
-                float4 h4 = P.hit[slot];
-                vec3 hitO = v3(o4.x, o4.y, o4.z), hitD = v3(d4.x, d4.y, d4.z);
-                Surf h = surfaceAt<false>(S, hitO, hitD, h4, h4);
-                Mat m = materialFor(S, h);
-                applyNormalMap(S, h, m, hitD);
-                applyBaseColorTexture(S, m, h);
-                T = v3(t4.x, t4.y, t4.z);
-                alpha = t4.w;
-                state = meta.z;
-                // Entry (:3534-3548): cosine-weighted into the surface.
-                float e0 = randomFloat(state);
-                float e1 = randomFloat(state);
-                vec3 entryDir = orientHemisphere(cosineHemisphere(e0, e1), -h.normal);
-                mo = h.position + entryDir * 0.001f;
-                md = entryDir;
-                sigmaT = v3s(1.0f) / materialSubsurfaceRadius(m);
-                albedo = materialSubsurfaceColor(m);
-                g = materialSubsurfaceAnisotropy(m);
-                step = 0;
-                cBounce++;
-                cSss++;
-                r = dr_prep(mo, md);
-                best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
-            },
-            [&](const DrHitRec &hr, bool hit, DrRayPre &r, DrHitRec &best) {
-                if (!hit) {
-                    finish(false, mo, md, 0.0f);
-                    return false;
-                }
-                float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
-                Surf b = surfaceAt<false>(S, mo, md, h4, h4);
-                float f0 = randomFloat(state);
-                float f1 = randomFloat(state);
-                float ff = sampleFreeFlight(sigmaT, f0, f1);
-                if (ff < b.t) {
-                    mo += md * ff;
-                    float h0 = randomFloat(state);
-                    float h1 = randomFloat(state);
-                    md = sampleHenyeyGreenstein(h0, h1, md, g);
-                    vec3 tr = vexp((-sigmaT) * ff);
-                    T *= albedo * sigmaT * tr / collisionPDF(sigmaT, ff);
-                    if (maxComponent(T) <= 1e-4f || ++step >= 8u) {
-                        finish(false, mo, md, 0.0f);
-                        return false;
-                    }
+                walk = qi < count;
+                if (walk) {
+                    walkBegin(S, P, P.sssQueue[qi], it, wk, r, best);
                     cBounce++;
                     cSss++;
-                    r = dr_prep(mo, md);
-                    best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
-                    return true;
+                    laneAny = false;
+                } else {
+                    shadowBegin(P, P.shadowQueue[qi - count], it, r, best);
+                    ++shQ;
+                    laneAny = anyShadow;
                 }
-                vec3 exitN = normalize(b.frontFacing ? b.normal : -b.normal);
-                float x0 = randomFloat(state);
-                float x1 = randomFloat(state);
-                vec3 rd = orientHemisphere(cosineHemisphere(x0, x1), exitN);
-                vec3 ro = b.position + exitN * 0.001f;
-                vec3 tr = vexp((-sigmaT) * b.t);
-                T *= tr / boundaryPDF(sigmaT, b.t);
-                finish(true, ro, rd, fmaxf(dot(exitN, rd), 0.0f) * 0.31830988618f);
-                return false;
-            });
+                markV = cNodes;
+                markT = cTris;
+            },
+            [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
+                if (!walk) {
+                    shNodes += cNodes - markV;
+                    shTris += cTris - markT;
+                }
+                markV = cNodes;
+                markT = cTris;
+                return walk ? walkStep(S, L, P, bounce, h, hit, it, wk, r, best, cBounce, cSss)
+                            : shadowStep(S, P, anyShadow, h, hit, it, r, best, shQ);
+            },
+            &laneAny);
+        cNodes -= shNodes;
+        cTris -= shTris;
     }
     while (VOL) {
         uint32_t base = 0;
@@ -907,7 +1048,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L
             P.meta[slot] = make_uint4(meta.x, meta.y, state, meta.w);
         }
         uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
-        if (alive) queueOut[pos] = slot;
+        if (alive) P.queue[(bounce + 1u) & 1u][pos] = slot;
         uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
         if (resume) P.resumeQueue[rpos] = slot;
     }
@@ -916,6 +1057,9 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_sss(DrSceneDev S, DrLaunch L
         waveAddStat(P.stats, DR_CNT_SSS_NODES, cNodes);
         waveAddStat(P.stats, DR_CNT_SSS_TRIS, cTris);
         waveAddStat(P.stats, DR_CNT_SSS, cSss);
+        waveAddStat(P.stats, DR_CNT_SHADOW, shQ);
+        waveAddStat(P.stats, DR_CNT_SHADOW_NODES, shNodes);
+        waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, shTris);
         if constexpr (VOL) addSdfStats(P.stats, sc);
     }
 }
@@ -977,100 +1121,38 @@ __device__ __forceinline__ bool shadowChain(const DrSceneDev &S, vec3 o, vec3 d,
 }
 
 // ---------------------------------------------------------------- shadow --
-// shadowOccluded (PathTrace.metal:4196-4274) for one NEE ray; writes the
-// transmittance (0 if occluded) into shC[k].w's companion record shR[k].
+// shadowOccluded (PathTrace.metal:4196-4274) for the bounce's NEE rays not
+// traced by k_sss (on triangles: those k_shade<RESUME> queued; with volumes:
+// all of them); writes each ray's transmittance (0 if occluded) to shR[k].
 template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
-    const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce];
+    const uint32_t begin = P.counters[DR_CTR_SHBEGIN + bounce];
+    const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce] - begin;
+    const uint32_t *queue = P.shadowQueue + begin;
     uint32_t *steal = &P.counters[DR_CTR_SHADOW + bounce];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
     DrSdfCounters sc{};
-    if (!VOL && S.opaqueOnly) {
-        // No material lets light through: occluded iff any triangle lies
-        // below maxDistance (the limit is exclusive: id 0 never wins a tie).
-        uint32_t k = 0;
-        dr_trace_queue<true>(
-            S, count, steal, stk, nv, nt,
-            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                k = P.shadowQueue[qi];
-                float4 o4 = P.shO[k], d4 = P.shD[k];
-                r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
-                best = DrHitRec{o4.w, 0.0f, 0.0f, 0u};
-                ++nq;
-            },
-            [&](const DrHitRec &, bool occluded, DrRayPre &, DrHitRec &) {
-                P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-                return false;
-            });
-    } else if (!VOL) {
-        // The chain of closest hits, one query per step; a lane continues
-        // its chain without waiting for the wave.
-        uint32_t k = 0, step = 0;
-        vec3 o, d, tr;
-        float maxDistance = 0.0f, traveled = 0.0f;
-        // Hits at or beyond maxDistance - traveled end the chain, so each
-        // query is limited there: exactly (exclusive limit) on the first
-        // step, with a relative margin afterwards where the reference test
-        // traveled + t >= maxDistance decides.
-        auto limitFor = [&](uint32_t st) {
-            return st == 0u ? DrHitRec{maxDistance, 0.0f, 0.0f, 0u}
-                            : DrHitRec{(maxDistance - traveled) * 1.000001f + 1e-30f, 0.0f, 0.0f, DR_MISS};
+    if constexpr (!VOL) {
+        DrItem it{};
+        auto fetch = [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+            shadowBegin(P, queue[qi], it, r, best);
+            ++nq;
         };
-        dr_trace_queue<false>(
-            S, count, steal, stk, nv, nt,
-            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                k = P.shadowQueue[qi];
-                float4 o4 = P.shO[k], d4 = P.shD[k];
-                o = v3(o4.x, o4.y, o4.z);
-                d = v3(d4.x, d4.y, d4.z);
-                maxDistance = o4.w;
-                traveled = 0.0f;
-                tr = v3s(1.0f);
-                step = 0;
-                r = dr_prep(o, d);
-                best = limitFor(0);
-                ++nq;
-            },
-            [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
-                bool occluded = true, next = false;
-                if (!hit || traveled + h.t >= maxDistance) {
-                    occluded = false;
-                } else {
-                    float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-                    Surf sh = surfaceAt<false>(S, o, d, h4, h4);
-                    Mat m = materialFor(S, sh);
-                    if (m.base.w <= 0.001f) {
-                        traveled += sh.t;
-                        o = sh.position + d * 0.001f;
-                        next = true;
-                    } else {
-                        float transmission = materialTransmission(m);
-                        if (transmission > 0.001f) {
-                            bool inside = !sh.frontFacing && !materialThinWalled(m);
-                            vec3 medium = materialHasVolumeScattering(m) && inside
-                                              ? materialVolumeTransmittance(m, sh.t)
-                                              : materialTransmissionAbsorption(m, sh.t, inside);
-                            tr = tr * materialTransmissionTint(m, 0.35f) * medium * transmission;
-                            if (maxComponent(tr) >= 0.02f) {
-                                traveled += sh.t;
-                                o = sh.position + d * 0.001f;
-                                next = true;
-                            }
-                        }
-                    }
-                }
-                if (next && ++step < 8u) {
-                    r = dr_prep(o, d);
-                    best = limitFor(step);
-                    ++nq;
-                    return true;
-                }
-                P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
-                return false;
-            });
+        if (S.opaqueOnly) {
+            dr_trace_queue<true>(S, count, steal, stk, nv, nt, fetch,
+                                 [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
+                                     return shadowStep(S, P, true, h, hit, it, r, best, nq);
+                                 });
+        } else {
+            // A lane continues its chain without waiting for the wave.
+            dr_trace_queue<false>(S, count, steal, stk, nv, nt, fetch,
+                                  [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
+                                      return shadowStep(S, P, false, h, hit, it, r, best, nq);
+                                  });
+        }
     }
     while (VOL) {
         uint32_t base = 0;
@@ -1079,7 +1161,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) k_shadow(DrSceneDev S, DrPaths
         if (base >= count) break;
         uint32_t qi = base + laneId();
         if (qi >= count) continue;
-        uint32_t k = P.shadowQueue[qi];
+        uint32_t k = queue[qi];
         float4 o4 = P.shO[k], d4 = P.shD[k];
         vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
         float maxDistance = o4.w;

@@ -342,12 +342,17 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
 // returns true (a shadow chain step, a subsurface-walk segment), or
 // returns false when the item is done. All 64 lanes of the wave must enter;
 // the loop exits when the queue is drained and every lane is idle.
+// laneAny, when given, points to a per-lane flag that fetch / retire set:
+// the lane's current query is then an any-hit query (ends at its first hit)
+// even where ANY is false, so closest-hit and any-hit items can share a
+// queue.
 #ifndef DR_REFILL_MIN
 #define DR_REFILL_MIN 16
 #endif
 template <bool ANY, class Fetch, class Retire>
 __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t count, uint32_t *claim, DrStack &stk,
-                                               uint32_t &nv, uint32_t &nt, Fetch fetch, Retire retire) {
+                                               uint32_t &nv, uint32_t &nt, Fetch fetch, Retire retire,
+                                               const bool *laneAny = nullptr) {
     const uint32_t lane = __lane_id();
     // A wave claims at most its fair share of a short queue, so that short
     // queues (subsurface walks, late bounces) spread over every CU instead
@@ -394,7 +399,7 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
         while (__any(active && s.leafCount != 0u)) {
             if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
                 found = true;
-                if (ANY) {
+                if (ANY || (laneAny && *laneAny)) {
                     s.done = true;
                     s.leafCount = 0;
                 }
