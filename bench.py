@@ -393,10 +393,12 @@ def main():
     # Roofline of the dominant kernel (largest share of stage time).
     st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
     dom = max(("extend", "shadow", "shade", "sss"), key=lambda k: st_ms[k])
+    # On triangle scenes k_sss also traces the shadow rays queued by k_shade
+    # (counters 30-32); k_shadow traces the rest.
     if dom == "shadow":
-        nodes, tris, rays, sdf = c[5], c[6], c[2], sdf_sh
-    elif dom == "sss":  # subsurface-walk queries
-        nodes, tris, rays, sdf = c[28], c[29], c[7], sdf_ext * 0.0
+        nodes, tris, rays, sdf = c[5] - c[31], c[6] - c[32], c[2] - c[30], sdf_sh
+    elif dom == "sss":  # subsurface-walk queries + the shadow rays traced beside them
+        nodes, tris, rays, sdf = c[28] + c[31], c[29] + c[32], c[7] + c[30], sdf_ext * 0.0
     else:
         nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
     launches_per_step = max(1, st_n[dom] // max(1, args.steps))

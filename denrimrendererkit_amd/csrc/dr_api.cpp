@@ -52,10 +52,10 @@ template <class T> hipError_t devAlloc(T **p, size_t count) {
     return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
 }
 
-template <class T> hipError_t upload(T **p, const void *src, size_t count) {
+template <class T> hipError_t upload(T **p, const void *src, size_t count, hipStream_t stream) {
     hipError_t e = devAlloc(p, count);
     if (e != hipSuccess || count == 0) return e;
-    return hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    return drCopy(*p, src, count * sizeof(T), hipMemcpyHostToDevice, stream);
 }
 
 void devFree(void *p) {
@@ -337,28 +337,30 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
         for (uint32_t i = 0; i < n; ++i) std::memcpy(&alias[i].x, &prob[i], 4);
     }
 
-    hipError_t e = hipSuccess;
+    hipStream_t up = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
     auto chain = [&](hipError_t x) {
         if (e == hipSuccess) e = x;
     };
-    chain(upload(&s->nodes, nodes.data(), nodes.size()));
-    chain(upload(&s->envAlias, alias.data(), alias.size()));
-    chain(upload(&s->triPos, triPos.data(), triPos.size()));
-    chain(upload(&s->triangles, d->triangles, d->triangle_count));
-    chain(upload(&s->materials, d->materials, d->material_count));
-    chain(upload(&s->lights, d->lights, d->light_count));
-    chain(upload(&s->env, d->environment_samples, d->environment_sample_count));
-    chain(upload(&s->envGuide, guide.data(), guide.size()));
-    chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count));
-    chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count));
-    chain(upload(&s->volumes, d->volumes, d->volume_count));
-    chain(upload(&s->denseSamples, d->volume_samples, d->volume_count ? d->volume_sample_count : 0u));
-    chain(upload(&s->bricks, bricks.data(), bricks.size()));
-    chain(upload(&s->dist16, dist16.data(), dist16.size()));
-    chain(upload(&s->dist32, dist32.data(), dist32.size()));
-    chain(upload(&s->sampleMaterial, sampleMat.data(), sampleMat.size()));
-    chain(upload(&s->grids, d->volume_brick_grids, d->volume_brick_grid_count));
-    chain(upload(&s->gridIndices, d->volume_brick_grid_indices, d->volume_brick_grid_index_count));
+    chain(upload(&s->nodes, nodes.data(), nodes.size(), up));
+    chain(upload(&s->envAlias, alias.data(), alias.size(), up));
+    chain(upload(&s->triPos, triPos.data(), triPos.size(), up));
+    chain(upload(&s->triangles, d->triangles, d->triangle_count, up));
+    chain(upload(&s->materials, d->materials, d->material_count, up));
+    chain(upload(&s->lights, d->lights, d->light_count, up));
+    chain(upload(&s->env, d->environment_samples, d->environment_sample_count, up));
+    chain(upload(&s->envGuide, guide.data(), guide.size(), up));
+    chain(upload(&s->texDesc, d->texture_descriptors, d->texture_count, up));
+    chain(upload(&s->texels, d->texture_pixels, d->texture_pixel_count, up));
+    chain(upload(&s->volumes, d->volumes, d->volume_count, up));
+    chain(upload(&s->denseSamples, d->volume_samples, d->volume_count ? d->volume_sample_count : 0u, up));
+    chain(upload(&s->bricks, bricks.data(), bricks.size(), up));
+    chain(upload(&s->dist16, dist16.data(), dist16.size(), up));
+    chain(upload(&s->dist32, dist32.data(), dist32.size(), up));
+    chain(upload(&s->sampleMaterial, sampleMat.data(), sampleMat.size(), up));
+    chain(upload(&s->grids, d->volume_brick_grids, d->volume_brick_grid_count, up));
+    chain(upload(&s->gridIndices, d->volume_brick_grid_indices, d->volume_brick_grid_index_count, up));
+    if (up) (void)hipStreamDestroy(up);
     if (e != hipSuccess) {
         freeScene(s);
         delete s;
@@ -463,10 +465,13 @@ dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t co
     float *dT = reinterpret_cast<float *>(buf + size_t(count) * 32);
     uint32_t *dIds = reinterpret_cast<uint32_t *>(buf + size_t(count) * 36);
     uint32_t *dOvf = reinterpret_cast<uint32_t *>(buf + size_t(count) * 40);
-    hipError_t e = hipMemcpy(dRays, rays, size_t(count) * 32, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = dr::launchIntersect(S, dRays, count, dT, dIds, dOvf, nullptr);
-    if (e == hipSuccess) e = hipMemcpy(t, dT, size_t(count) * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(ids, dIds, size_t(count) * 4, hipMemcpyDeviceToHost);
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = drCopy(dRays, rays, size_t(count) * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = dr::launchIntersect(S, dRays, count, dT, dIds, dOvf, st);
+    if (e == hipSuccess) e = drCopy(t, dT, size_t(count) * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = drCopy(ids, dIds, size_t(count) * 4, hipMemcpyDeviceToHost, st);
+    if (st) (void)hipStreamDestroy(st);
     (void)hipFree(buf);
     DR_HIP(e, "intersect");
     return DR_OK;
@@ -934,7 +939,7 @@ dr_status dr_hip_session_read_output(dr_session *s, dr_output kind, float *rgba,
     DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
     const float4 *planes[8] = {denoised ? s->denoised : s->out.accumulation, s->out.depth, s->out.normal,
                                s->out.albedo, s->out.materialID, s->out.objectID, s->out.motion, s->out.triangleID};
-    DR_HIP(hipMemcpy(rgba, planes[int(kind)], pixels * 16, hipMemcpyDeviceToHost), "readback");
+    DR_HIP(drCopy(rgba, planes[int(kind)], pixels * 16, hipMemcpyDeviceToHost, s->stream), "readback");
     return DR_OK;
 }
 
@@ -942,7 +947,7 @@ dr_status dr_hip_session_stats(dr_session *s, uint64_t *counters, size_t count) 
     if (!s || !counters) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
     unsigned long long tmp[DR_CNT_SLOTS];
     DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
-    DR_HIP(hipMemcpy(tmp, s->paths.stats, sizeof(tmp), hipMemcpyDeviceToHost), "stats readback");
+    DR_HIP(drCopy(tmp, s->paths.stats, sizeof(tmp), hipMemcpyDeviceToHost, s->stream), "stats readback");
     for (size_t i = 0; i < count && i < DR_CNT_SLOTS; ++i) counters[i] = tmp[i];
     return DR_OK;
 }
@@ -950,7 +955,8 @@ dr_status dr_hip_session_stats(dr_session *s, uint64_t *counters, size_t count) 
 dr_status dr_hip_session_reset_stats(dr_session *s) {
     if (!s) return fail(DR_ERR_INVALID_ARGUMENT, "session is NULL");
     DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
-    DR_HIP(hipMemset(s->paths.stats, 0, DR_CNT_SLOTS * 8), "stats reset");
+    DR_HIP(hipMemsetAsync(s->paths.stats, 0, DR_CNT_SLOTS * 8, s->stream), "stats reset");
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
     for (int i = 0; i < DR_STAGE_COUNT; ++i) {
         s->stageMs[i] = 0.0;
         s->stageLaunches[i] = 0;

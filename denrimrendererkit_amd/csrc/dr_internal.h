@@ -12,6 +12,16 @@
 #include "dr_kernels.h"
 #include "dr_scene.h"
 
+// Host<->device copy that is synchronous for the caller but runs on
+// `stream`, never on the legacy null stream: another thread may be capturing
+// a batch graph on its own stream, and the null stream's implicit
+// synchronization with other streams is refused while a capture is open.
+inline hipError_t drCopy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t stream) {
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
+    return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+}
+
 struct dr_device {
     int ordinal = 0;
     int cuCount = 0;

@@ -1060,6 +1060,9 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         waveAddStat(P.stats, DR_CNT_SHADOW, shQ);
         waveAddStat(P.stats, DR_CNT_SHADOW_NODES, shNodes);
         waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, shTris);
+        waveAddStat(P.stats, DR_CNT_SSS_SHADOW, shQ);
+        waveAddStat(P.stats, DR_CNT_SSS_SHADOW_NODES, shNodes);
+        waveAddStat(P.stats, DR_CNT_SSS_SHADOW_TRIS, shTris);
         if constexpr (VOL) addSdfStats(P.stats, sc);
     }
 }

@@ -164,5 +164,10 @@ enum {
     // NODES / TRIS count extension and AOV queries.
     DR_CNT_SSS_NODES = 28,
     DR_CNT_SSS_TRIS = 29,
-    DR_CNT_SLOTS = 30
+    // The part of the SHADOW / SHADOW_NODES / SHADOW_TRIS counts traced
+    // inside k_sss (the shadow rays queued by k_shade on triangle scenes).
+    DR_CNT_SSS_SHADOW = 30,
+    DR_CNT_SSS_SHADOW_NODES = 31,
+    DR_CNT_SSS_SHADOW_TRIS = 32,
+    DR_CNT_SLOTS = 33
 };

@@ -401,14 +401,14 @@ dr_status dr_hip_unet_param_count(const dr_unet *u, uint64_t *count) {
 dr_status dr_hip_unet_get_params(dr_unet *u, float *p, size_t count) {
     if (!u || !p || count < u->paramCount) return ufail(DR_ERR_INVALID_ARGUMENT, "parameter buffer too small");
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
-    DR_UHIP(hipMemcpy(p, u->params, u->paramCount * 4, hipMemcpyDeviceToHost), "read parameters");
+    DR_UHIP(drCopy(p, u->params, u->paramCount * 4, hipMemcpyDeviceToHost, u->stream), "read parameters");
     return DR_OK;
 }
 
 dr_status dr_hip_unet_set_params(dr_unet *u, const float *p, size_t count) {
     if (!u || !p || count != u->paramCount) return ufail(DR_ERR_INVALID_ARGUMENT, "parameter count mismatch");
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
-    DR_UHIP(hipMemcpy(u->params, p, u->paramCount * 4, hipMemcpyHostToDevice), "write parameters");
+    DR_UHIP(drCopy(u->params, p, u->paramCount * 4, hipMemcpyHostToDevice, u->stream), "write parameters");
     DR_UHIP(dr::launchFill(u->m, u->paramCount, 0.f, u->stream), "reset adam");
     DR_UHIP(dr::launchFill(u->v, u->paramCount, 0.f, u->stream), "reset adam");
     u->step = 0;
@@ -428,12 +428,12 @@ dr_status dr_hip_unet_set_batch(dr_unet *u, const float *features, const float *
     const size_t px = size_t(n) * h * w;
     float *tmp = nullptr;
     DR_UHIP(hipMalloc(&tmp, px * 16 * 4), "staging");
-    hipError_t e = hipMemcpy(tmp, features, px * 16 * 4, hipMemcpyHostToDevice);
+    hipError_t e = drCopy(tmp, features, px * 16 * 4, hipMemcpyHostToDevice, u->stream);
     if (e == hipSuccess) e = dr::launchToBf16(tmp, u->train.act[X0], px * 16, u->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(u->stream);
     (void)hipFree(tmp);
     DR_UHIP(e, "features upload");
-    DR_UHIP(hipMemcpy(u->train.target, targets, px * 4 * 4, hipMemcpyHostToDevice), "targets upload");
+    DR_UHIP(drCopy(u->train.target, targets, px * 4 * 4, hipMemcpyHostToDevice, u->stream), "targets upload");
     u->batchReady = true;
     return DR_OK;
 }
@@ -447,14 +447,14 @@ dr_status dr_hip_unet_forward_backward(dr_unet *u, float *loss) {
     rc = forwardBackward(u, 0);
     if (rc != DR_OK) return rc;
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
-    if (loss) DR_UHIP(hipMemcpy(loss, u->loss, 4, hipMemcpyDeviceToHost), "loss readback");
+    if (loss) DR_UHIP(drCopy(loss, u->loss, 4, hipMemcpyDeviceToHost, u->stream), "loss readback");
     return DR_OK;
 }
 
 dr_status dr_hip_unet_get_grads(dr_unet *u, float *g, size_t count) {
     if (!u || !g || count < u->paramCount) return ufail(DR_ERR_INVALID_ARGUMENT, "gradient buffer too small");
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
-    DR_UHIP(hipMemcpy(g, u->grads, u->paramCount * 4, hipMemcpyDeviceToHost), "read gradients");
+    DR_UHIP(drCopy(g, u->grads, u->paramCount * 4, hipMemcpyDeviceToHost, u->stream), "read gradients");
     return DR_OK;
 }
 
@@ -463,7 +463,7 @@ dr_status dr_hip_unet_get_output(dr_unet *u, float *out, size_t floats) {
     const size_t n = size_t(u->train.n) * u->train.H * u->train.W * 16;
     if (floats < n) return ufail(DR_ERR_INVALID_ARGUMENT, "output buffer too small");
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
-    DR_UHIP(hipMemcpy(out, u->train.y, n * 4, hipMemcpyDeviceToHost), "read output");
+    DR_UHIP(drCopy(out, u->train.y, n * 4, hipMemcpyDeviceToHost, u->stream), "read output");
     return DR_OK;
 }
 
@@ -546,7 +546,7 @@ dr_status dr_hip_unet_train_dataset(dr_unet *u, uint32_t steps, uint32_t batch, 
     }
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
     u->batchReady = true;
-    if (losses) DR_UHIP(hipMemcpy(losses, u->loss, size_t(steps) * 4, hipMemcpyDeviceToHost), "loss readback");
+    if (losses) DR_UHIP(drCopy(losses, u->loss, size_t(steps) * 4, hipMemcpyDeviceToHost, u->stream), "loss readback");
     return DR_OK;
 }
 

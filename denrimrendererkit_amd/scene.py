@@ -458,6 +458,7 @@ class RenderScene:
         self.mesh_instances: List[MeshInstance] = []
         self.sparse_volume_instances: list = []
         self.volume_instances: list = []
+        self.render_defaults = None  # scenescript.RenderDefaults of a parsed script
 
     def add_material(self, material: Material) -> int:
         self.materials.append(material)

@@ -187,7 +187,12 @@ typedef enum {
     DR_STAT_SHADOW_SDF_BASE = 18,
     DR_STAT_SUBSURFACE_NODE_VISITS = 28, /* subsurface-walk queries */
     DR_STAT_SUBSURFACE_TRIANGLE_TESTS = 29,
-    DR_STAT_COUNT = 30
+    /* 30-32: the part of the shadow counters (2, 5, 6) traced by the
+     * subsurface-walk kernel's launch (shadow rays and walks share it). */
+    DR_STAT_SHADOW_QUERIES_IN_WALK_KERNEL = 30,
+    DR_STAT_SHADOW_NODE_VISITS_IN_WALK_KERNEL = 31,
+    DR_STAT_SHADOW_TRIANGLE_TESTS_IN_WALK_KERNEL = 32,
+    DR_STAT_COUNT = 33
 } dr_stat;
 
 /* SDF primitive for the volume builders (SDFPrimitive / CompiledPrimitive,
