@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_extend(DrSceneD
     const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
     uint32_t *steal = &P.counters[DR_CTR_EXTEND + bounce];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0;
     DrSdfCounters sc{};
     if constexpr (!VOL) {
@@ -968,7 +968,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
     const uint32_t count = P.counters[DR_CTR_SSSQ + bounce];
     uint32_t *claim = &P.counters[DR_CTR_SSS + bounce];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
     uint32_t shQ = 0u, shNodes = 0u, shTris = 0u;
     DrSdfCounters sc{};
@@ -1135,7 +1135,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_shadow(DrSceneD
     const uint32_t *queue = P.shadowQueue + begin;
     uint32_t *steal = &P.counters[DR_CTR_SHADOW + bounce];
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
     DrSdfCounters sc{};
     if constexpr (!VOL) {
@@ -1259,7 +1259,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_preview(DrSceneDev S, DrLaun
                                                             DrPaths P, DrOutputs O) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_SHADE_BLOCK];
     const uint32_t i = blockIdx.x * DR_SHADE_BLOCK + threadIdx.x;
-    DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + i, P.overflowStride};
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_SHADE_BLOCK, P.overflow + i, P.overflowStride};
     if (i >= L.tileWidth * L.tileHeight) return;
     const uint32_t gx = L.tileX + i % L.tileWidth, gy = L.tileY + i / L.tileWidth;
     if (gx >= L.width || gy >= L.height) return;
@@ -1482,7 +1482,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_finalize(DrSceneDev S, DrLau
     O.accumulation[pix] = acc;
     bool has = (lastFlags & DR_FLAG_PRIMARY) != 0u;
     if (L.denoiserEnabled != 0u) {
-        DrStack stk{&ldsStack[threadIdx.x], DR_SHADE_BLOCK, P.overflow + p, P.overflowStride};
+        DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_SHADE_BLOCK, P.overflow + p, P.overflowStride};
         vec3 o, d;
         makeCameraRay(cam, L.width, L.height, px, py, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
         Surf h;
@@ -1529,7 +1529,7 @@ __global__ __launch_bounds__(DR_TRACE_BLOCK) void k_intersect(DrSceneDev S, cons
                                                               uint32_t *__restrict__ idOut, uint32_t *overflow) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t i = blockIdx.x * DR_TRACE_BLOCK + threadIdx.x;
-    DrStack stk{&ldsStack[threadIdx.x], DR_TRACE_BLOCK, overflow + i, gridDim.x * DR_TRACE_BLOCK};
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, overflow + i, gridDim.x * DR_TRACE_BLOCK};
     const bool live = i < count;
     // Lanes past the end trace a ray that misses every box.
     const float4 o = live ? rays[2 * i] : make_float4(3e38f, 3e38f, 3e38f, 0.f);

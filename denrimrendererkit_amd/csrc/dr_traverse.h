@@ -24,10 +24,18 @@
 #define DR_LDS_STACK 24
 #endif
 
+// The LDS part is addressed through an LDS (address space 3) pointer: with a
+// generic pointer the compiler folds the LDS and the spill access into one
+// flat instruction, which is slower than ds_read/ds_write and is counted
+// with the vector-memory loads the node fetches wait on.
+typedef __attribute__((address_space(3))) uint32_t dr_lds_u32;
+
+__device__ __forceinline__ dr_lds_u32 *dr_lds_ptr(uint32_t *p) { return (dr_lds_u32 *)p; }
+
 struct DrStack {
-    uint32_t *lds;   // &lds[threadIdx.x], stride blockDim.x
+    dr_lds_u32 *lds;  // &lds[threadIdx.x], stride blockDim.x
     uint32_t ldsStride;
-    uint32_t *glob;  // &overflow[globalLane], stride gStride
+    uint32_t *glob;   // &overflow[globalLane], stride gStride
     uint32_t gStride;
 
     __device__ __forceinline__ void put(uint32_t sp, uint32_t v) {
