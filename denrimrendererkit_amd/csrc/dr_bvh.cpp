@@ -261,7 +261,9 @@ struct Flattener {
         o.hiy = make_float4(bx[4], bx[4], bx[4], bx[4]);
         o.hiz = make_float4(bx[5], bx[5], bx[5], bx[5]);
         o.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid, 0u, 0u);
+        uint32_t leafBits = 0;
+        for (int c = 0; c < 4; ++c) leafBits |= (cnt[c] ? 1u : 0u) << c;
+        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid | leafBits << 4, 0u, 0u);
         return me;
     }
 
@@ -327,7 +329,9 @@ struct Flattener {
         o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
         o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
         o.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid, 0u, 0u);
+        uint32_t leafBits = 0;
+        for (int c = 0; c < 4; ++c) leafBits |= (cnt[c] ? 1u : 0u) << c;
+        o.meta = make_uint4(cnt[0] | cnt[1] << 8 | cnt[2] << 16 | cnt[3] << 24, valid | leafBits << 4, 0u, 0u);
         return me;
     }
 };

@@ -45,8 +45,9 @@ static_assert(sizeof(DrLight) == 32, "GPULightRecord layout");
 // boxes as planes of x/y/z bounds (component c = child c), the child
 // references, then meta.x = the four leaf triangle counts (8 bits each,
 // child c at bits 8c; 0 = inner node) and meta.y = the mask of present
-// children. A leaf child's reference is its first triangle in leaf order,
-// an inner child's its node index.
+// children (bits 0-3) and of leaf children (bits 4-7). A leaf child's
+// reference is its first triangle in leaf order, an inner child's its node
+// index.
 struct DrNode4 {
     float4 lox, hix, loy, hiy, loz, hiz;
     uint4 child;

@@ -198,21 +198,15 @@ __device__ __forceinline__ void dr_trav_init(DrTrav &s) {
 
 __device__ __forceinline__ uint32_t dr_sel4(uint4 v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
 
-// Child of `mask` with the smallest (or largest) entry distance; ties go
-// to the lower child.
-__device__ __forceinline__ int dr_pick(const float t[4], uint32_t mask, bool nearest) {
-    int best = -1;
-    float bt = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if ((mask >> c) & 1u) {
-            if (best < 0 || (nearest ? t[c] < bt : t[c] > bt)) {
-                best = c;
-                bt = t[c];
-            }
-        }
-    }
-    return best;
+// Orders (ka, pa), (kb, pb) by key; equal keys keep their order.
+__device__ __forceinline__ void dr_cswap(float &ka, uint32_t &pa, float &kb, uint32_t &pb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    kb = sw ? ka : kb;
+    ka = k;
+    const uint32_t q = sw ? pb : pa;
+    pb = sw ? pa : pb;
+    pa = q;
 }
 
 // One inner-node step; requires !s.done and either no pending leaf, or a
@@ -242,42 +236,59 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
     const uint32_t ni = s.cur;
     const DrNode4 n = S.nodes[ni];
     ++nv;
-    float t[4];
-    t[0] = dr_box(r, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmax);
-    t[1] = dr_box(r, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmax);
-    t[2] = dr_box(r, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmax);
-    t[3] = dr_box(r, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmax);
-    const uint32_t counts = n.meta.x, valid = n.meta.y;
-    uint32_t leafMask = 0u, innerMask = 0u;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if (((valid >> c) & 1u) && t[c] != __builtin_inff()) {
-            if ((counts >> (8 * c)) & 0xffu) leafMask |= 1u << c;
-            else innerMask |= 1u << c;
-        }
-    }
+    const float inf = __builtin_inff();
+    float t0 = dr_box(r, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmax);
+    float t1 = dr_box(r, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmax);
+    float t2 = dr_box(r, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmax);
+    float t3 = dr_box(r, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmax);
+    const uint32_t hit = ((t0 != inf ? 1u : 0u) | (t1 != inf ? 2u : 0u) | (t2 != inf ? 4u : 0u) |
+                          (t3 != inf ? 8u : 0u)) & n.meta.y;
+    const uint32_t leafBits = (n.meta.y >> 4) & 15u;
+    uint32_t leafMask = hit & leafBits;
+    const uint32_t innerMask = hit & ~leafBits;
     if (leafMask) {
         if (s.leafCount == 0u) {
-            const int c = dr_pick(t, leafMask, true);
-            s.leafFirst = dr_sel4(n.child, c);
-            s.leafCount = (counts >> (8 * c)) & 0xffu;
-            leafMask &= ~(1u << c);
+            // The nearest hit leaf (ties to the lower child) becomes pending.
+            float lk = inf;
+            uint32_t lc = 0u;
+            if ((leafMask & 1u) && t0 < lk) lk = t0, lc = 0u;
+            if ((leafMask & 2u) && t1 < lk) lk = t1, lc = 1u;
+            if ((leafMask & 4u) && t2 < lk) lk = t2, lc = 2u;
+            if ((leafMask & 8u) && t3 < lk) lk = t3, lc = 3u;
+            s.leafFirst = dr_sel4(n.child, int(lc));
+            s.leafCount = (n.meta.x >> (8u * lc)) & 0xffu;
+            leafMask &= ~(1u << lc);
         }
         if (leafMask) stk.put(s.sp++, DR_LEAF_REF | ni << 4 | leafMask);
     }
-    if (innerMask) {
-        // The nearest inner child is next; the others go to the stack, the
-        // farthest first so that nearer ones are popped first.
-        const int nc = dr_pick(t, innerMask, true);
-        innerMask &= ~(1u << nc);
-        while (innerMask) {
-            const int fc = dr_pick(t, innerMask, false);
-            stk.put(s.sp++, dr_sel4(n.child, fc));
-            innerMask &= ~(1u << fc);
+    // Inner children in order of entry distance (a 5-comparator network over
+    // (distance, reference), misses keyed +inf): the nearest is next, the
+    // others are pushed far to near so that nearer ones are popped first.
+    float k0 = (innerMask & 1u) ? t0 : inf, k1 = (innerMask & 2u) ? t1 : inf;
+    float k2 = (innerMask & 4u) ? t2 : inf, k3 = (innerMask & 8u) ? t3 : inf;
+    uint32_t p0 = n.child.x, p1 = n.child.y, p2 = n.child.z, p3 = n.child.w;
+    dr_cswap(k0, p0, k1, p1);
+    dr_cswap(k2, p2, k3, p3);
+    dr_cswap(k0, p0, k2, p2);
+    dr_cswap(k1, p1, k3, p3);
+    dr_cswap(k1, p1, k2, p2);
+    const uint32_t nIn = __builtin_popcount(innerMask);
+    s.cur = nIn ? p0 : DR_POP;
+    if (nIn > 1u) {
+        const uint32_t e0 = nIn == 4u ? p3 : (nIn == 3u ? p2 : p1);
+        const uint32_t e1 = nIn == 4u ? p2 : p1;
+        if (s.sp + 3u <= DR_LDS_STACK) {
+            // All three slots in LDS: write them unconditionally (slots
+            // past the new top are scratch) and advance by nIn - 1.
+            stk.lds[s.sp * stk.ldsStride] = e0;
+            stk.lds[(s.sp + 1u) * stk.ldsStride] = e1;
+            stk.lds[(s.sp + 2u) * stk.ldsStride] = p1;
+            s.sp += nIn - 1u;
+        } else {
+            stk.put(s.sp++, e0);
+            if (nIn >= 3u) stk.put(s.sp++, e1);
+            if (nIn == 4u) stk.put(s.sp++, p1);
         }
-        s.cur = dr_sel4(n.child, nc);
-    } else {
-        s.cur = DR_POP;
     }
 }
 
