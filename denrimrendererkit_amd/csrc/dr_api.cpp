@@ -196,6 +196,8 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     if ((d->triangle_count == 0 || !d->triangles) && d->volume_count == 0)
         return fail(DR_ERR_INVALID_SCENE, "Scene contains no renderable geometry.");
     if (d->material_count == 0 || !d->materials) return fail(DR_ERR_INVALID_SCENE, "Scene contains no materials.");
+    if (d->triangle_count >= DR_MAX_TRIANGLES)
+        return fail(DR_ERR_INVALID_SCENE, "Scene has more triangles than the traversal can index (2^27).");
     if (d->volume_count &&
         (!d->volumes || (d->volume_sample_count && !d->volume_samples) || (d->volume_brick_count && !d->volume_bricks) ||
          (d->volume_brick_sample_count && !d->volume_brick_samples) ||
@@ -229,7 +231,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     if (d->triangle_count)
         dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos,
                      &s->stats);
-    s->stackMax = 4u * s->stats.maxDepth + 2u;
+    s->stackMax = 7u * s->stats.maxDepth + 2u;
 
     // Sparse volumes: compact brick headers, FP16 distances when every
     // distance is FP16-exact (else FP32), per-sample material pairs with the

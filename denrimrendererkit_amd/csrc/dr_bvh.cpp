@@ -230,8 +230,9 @@ struct Flattener {
         sah += double(n.box.area()) / rootArea * kTriCost * count;
     }
 
-    // A leaf of more triangles than a node's 8-bit count holds (only where
-    // the depth cap forced it): a node of up to four leaf chunks of <= 255
+    // A leaf of more than DR_LEAF_MAX triangles (a traversal stack entry
+    // holds a leaf's count in 4 bits; only a large maxLeaf or the depth cap
+    // makes such leaves): a node of up to four leaf chunks of <= DR_LEAF_MAX
     // triangles, nested further when needed; every chunk has the leaf's box.
     uint32_t emitBigLeaf(const TNode &leaf, uint32_t start, uint32_t count, uint32_t level) {
         depth = std::max(depth, level);
@@ -240,11 +241,11 @@ struct Flattener {
         float bx[6];
         childBox(leaf.box, bx);
         uint32_t ref[4] = {0, 0, 0, 0}, cnt[4] = {0, 0, 0, 0}, valid = 0;
-        const uint32_t per = count <= 4u * 255u ? 255u : (count + 3u) / 4u;
+        const uint32_t per = count <= 4u * DR_LEAF_MAX ? DR_LEAF_MAX : (count + 3u) / 4u;
         for (int c = 0; c < 4 && count > 0; ++c) {
             const uint32_t n = std::min(per, count);
             valid |= 1u << c;
-            if (n <= 255u) {
+            if (n <= DR_LEAF_MAX) {
                 emitLeafTris(leaf, start, n, ref[c]);
                 cnt[c] = n;
             } else {
@@ -309,7 +310,7 @@ struct Flattener {
                 hi[a][c] = bx[3 + a];
             }
             valid |= 1u << c;
-            if (cn.count > 255u) {
+            if (cn.count > DR_LEAF_MAX) {
                 ref[c] = emitBigLeaf(cn, cn.first, cn.count, level + 1);
             } else if (cn.count > 0) {
                 emitLeafTris(cn, cn.first, cn.count, ref[c]);

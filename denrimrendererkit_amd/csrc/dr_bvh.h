@@ -29,11 +29,16 @@ struct BvhBuildStats {
 // fused-multiply-add slab test can never cull a box that the exact test
 // would keep; that keeps the closest hit identical to a brute-force search.
 // Depth is capped at DR_MAX_DEPTH; stats->maxDepth is the 4-wide tree's
-// depth, and a traversal lane needs at most 4 stack entries per level
-// (three inner children and one group of leaf children) plus two.
+// depth, and a traversal lane needs at most 7 stack entries per level
+// (three inner children and four leaf children) plus two. Leaves hold at
+// most DR_LEAF_MAX triangles (larger ones become nodes of leaf chunks).
+// A traversal stack entry of a leaf packs its count (4 bits) and first
+// triangle (27 bits), so a scene holds fewer than DR_MAX_TRIANGLES.
 void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vector<DrNode4> &nodes,
               std::vector<float4> &triPos, BvhBuildStats *stats);
 
 }  // namespace dr
 
 #define DR_MAX_DEPTH 64
+#define DR_LEAF_MAX 15u
+#define DR_MAX_TRIANGLES (1u << 27)

@@ -172,11 +172,11 @@ struct DrHitRec {
 // visited until the lane holds a pending leaf (or finishes); leaves are
 // tested when every lane of the wave holds one, so triangle tests run
 // converged instead of serializing the wave once per lane.
-// Stack entries: an inner node index, or DR_LEAF_REF | node << 4 | mask for
-// the hit leaf children (mask bit c = child c) of a node still to be tested;
-// popping one takes its first child and pushes the rest back. A node visit
-// pushes at most three inner children and one leaf group, so a lane needs
-// at most 4 entries per tree level plus two (the scene's stackMax).
+// Stack entries: an inner node index, or DR_LEAF_REF | count << 27 | first
+// for a hit leaf child still to be tested (count <= DR_LEAF_MAX, first <
+// DR_MAX_TRIANGLES), so a popped leaf needs no memory access. A node visit
+// pushes at most three inner and four leaf children, so a lane needs at
+// most 7 entries per tree level plus two (the scene's stackMax).
 #define DR_POP 0xffffffffu
 #define DR_LEAF_REF 0x80000000u
 
@@ -218,17 +218,10 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
             s.done = true;
             return;
         }
-        uint32_t e = stk.get(--s.sp);
+        const uint32_t e = stk.get(--s.sp);
         if (e & DR_LEAF_REF) {
-            const uint32_t node = (e & ~DR_LEAF_REF) >> 4;
-            uint32_t mask = e & 15u;
-            const int c = __builtin_ctz(mask);
-            mask &= mask - 1u;
-            const uint4 ch = S.nodes[node].child;
-            const uint32_t counts = S.nodes[node].meta.x;
-            s.leafFirst = dr_sel4(ch, c);
-            s.leafCount = (counts >> (8 * c)) & 0xffu;
-            if (mask) stk.put(s.sp++, DR_LEAF_REF | node << 4 | mask);
+            s.leafFirst = e & (DR_MAX_TRIANGLES - 1u);
+            s.leafCount = (e >> 27) & 15u;
             return;
         }
         s.cur = e;
@@ -247,6 +240,7 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
     uint32_t leafMask = hit & leafBits;
     const uint32_t innerMask = hit & ~leafBits;
     if (leafMask) {
+        const uint32_t cnt = n.meta.x;
         if (s.leafCount == 0u) {
             // The nearest hit leaf (ties to the lower child) becomes pending.
             float lk = inf;
@@ -256,10 +250,14 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
             if ((leafMask & 4u) && t2 < lk) lk = t2, lc = 2u;
             if ((leafMask & 8u) && t3 < lk) lk = t3, lc = 3u;
             s.leafFirst = dr_sel4(n.child, int(lc));
-            s.leafCount = (n.meta.x >> (8u * lc)) & 0xffu;
+            s.leafCount = (cnt >> (8u * lc)) & 0xffu;
             leafMask &= ~(1u << lc);
         }
-        if (leafMask) stk.put(s.sp++, DR_LEAF_REF | ni << 4 | leafMask);
+        // The other hit leaves are pushed in child order.
+        if (leafMask & 1u) stk.put(s.sp++, DR_LEAF_REF | (cnt & 0xffu) << 27 | n.child.x);
+        if (leafMask & 2u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 8 & 0xffu) << 27 | n.child.y);
+        if (leafMask & 4u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 16 & 0xffu) << 27 | n.child.z);
+        if (leafMask & 8u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 24) << 27 | n.child.w);
     }
     // Inner children in order of entry distance (a 5-comparator network over
     // (distance, reference), misses keyed +inf): the nearest is next, the
