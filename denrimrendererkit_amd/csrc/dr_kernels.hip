@@ -35,13 +35,20 @@
 #define DR_FLAG_PRIMARY 1u
 #define DR_FLAG_NEE 2u
 
-#ifndef DR_TRACE_WAVES
-#define DR_TRACE_WAVES 1
+// Minimum waves per SIMD the register allocator is held to for the
+// triangle-only tracers (1 = no constraint); the volume variants are left
+// free.
+#ifndef DR_EXTEND_WAVES
+#define DR_EXTEND_WAVES 1
 #endif
-#define DR_TRACE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_TRACE_WAVES)))
+#ifndef DR_SHADOW_WAVES
+#define DR_SHADOW_WAVES 1
+#endif
 #ifndef DR_SSS_WAVES
 #define DR_SSS_WAVES 4
 #endif
+#define DR_EXTEND_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_EXTEND_WAVES)))
+#define DR_SHADOW_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADOW_WAVES)))
 #define DR_SSS_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SSS_WAVES)))
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
@@ -153,8 +160,10 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
 }
 
 // ---------------------------------------------------------------- extend --
-template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+// STATS: the counting variant (node, triangle and query counters); the
+// other one keeps no counters at all.
+template <bool VOL, bool STATS>
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t *queue = P.queue[bounce & 1u];
     const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
@@ -194,7 +203,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_extend(DrSceneD
             }
         }
     }
-    if (statsOn) {
+    if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_NODES, nv);
         waveAddStat(P.stats, DR_CNT_TRIS, nt);
         if constexpr (VOL) addSdfStats(P.stats, sc);
@@ -785,10 +794,10 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch
 struct DrItem {
     uint32_t id;    // shadow record index, or the path slot of a walk
     uint32_t step;  // queries done
-    vec3 o, d;      // current ray
     vec3 w;         // shadow: transmittance; walk: path throughput
-    float a, b;     // shadow: maxDistance, traveled; walk: anisotropy g, output alpha
+    float a, b;     // shadow: maxDistance, traveled; walk: anisotropy g
 };
+// The item's current ray is the tracer's DrRayPre (r.o, r.d).
 
 // shadowOccluded (PathTrace.metal:4196-4274) for one queued NEE ray as an
 // item: a chain of closest hits, one query per step (anyHit = false), or one
@@ -807,12 +816,10 @@ __device__ __forceinline__ void shadowBegin(const DrPaths &P, uint32_t k, DrItem
     float4 o4 = P.shO[k], d4 = P.shD[k];
     it.id = k;
     it.step = 0u;
-    it.o = v3(o4.x, o4.y, o4.z);
-    it.d = v3(d4.x, d4.y, d4.z);
     it.w = v3s(1.0f);
     it.a = o4.w;
     it.b = 0.0f;
-    r = dr_prep(it.o, it.d);
+    r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
     best = shadowLimit(it);
 }
 // Consumes a query's result; true when the chain continues with the next
@@ -824,15 +831,17 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
         return false;
     }
     bool occluded = true, next = false;
+    vec3 o = r.o;
+    const vec3 d = r.d;
     if (!hit || it.b + h.t >= it.a) {
         occluded = false;
     } else {
         float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-        Surf sh = surfaceAt<false>(S, it.o, it.d, h4, h4);
+        Surf sh = surfaceAt<false>(S, o, d, h4, h4);
         Mat m = materialFor(S, sh);
         if (m.base.w <= 0.001f) {
             it.b += sh.t;
-            it.o = sh.position + it.d * 0.001f;
+            o = sh.position + d * 0.001f;
             next = true;
         } else {
             float transmission = materialTransmission(m);
@@ -843,14 +852,14 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
                 it.w = it.w * materialTransmissionTint(m, 0.35f) * medium * transmission;
                 if (maxComponent(it.w) >= 0.02f) {
                     it.b += sh.t;
-                    it.o = sh.position + it.d * 0.001f;
+                    o = sh.position + d * 0.001f;
                     next = true;
                 }
             }
         }
     }
     if (next && ++it.step < 8u) {
-        r = dr_prep(it.o, it.d);
+        r = dr_prep(o, d);
         best = shadowLimit(it);
         ++nq;
         return true;
@@ -860,16 +869,15 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
 }
 
 // continueSubsurfaceRandomWalk (PathTrace.metal:3525-3621) on triangles as
-// an item: one closest-hit query per walk segment.
+// an item: one closest-hit query per walk segment. The path's other state
+// (output alpha, meta words) stays in memory until the walk ends.
 struct DrWalk {
     vec3 sigmaT, albedo;
     uint32_t state;
-    uint4 meta;
 };
 __device__ __forceinline__ void walkBegin(const DrSceneDev &S, const DrPaths &P, uint32_t slot, DrItem &it, DrWalk &wk,
                                           DrRayPre &r, DrHitRec &best) {
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
-    wk.meta = P.meta[slot];
     float4 h4 = P.hit[slot];
     vec3 hitO = v3(o4.x, o4.y, o4.z), hitD = v3(d4.x, d4.y, d4.z);
     Surf h = surfaceAt<false>(S, hitO, hitD, h4, h4);
@@ -878,19 +886,16 @@ __device__ __forceinline__ void walkBegin(const DrSceneDev &S, const DrPaths &P,
     applyBaseColorTexture(S, m, h);
     it.id = slot;
     it.w = v3(t4.x, t4.y, t4.z);
-    it.b = t4.w;
-    wk.state = wk.meta.z;
+    wk.state = P.meta[slot].z;
     // Entry (:3534-3548): cosine-weighted into the surface.
     float e0 = randomFloat(wk.state);
     float e1 = randomFloat(wk.state);
     vec3 entryDir = orientHemisphere(cosineHemisphere(e0, e1), -h.normal);
-    it.o = h.position + entryDir * 0.001f;
-    it.d = entryDir;
     wk.sigmaT = v3s(1.0f) / materialSubsurfaceRadius(m);
     wk.albedo = materialSubsurfaceColor(m);
     it.a = materialSubsurfaceAnisotropy(m);
     it.step = 0u;
-    r = dr_prep(it.o, it.d);
+    r = dr_prep(h.position + entryDir * 0.001f, entryDir);
     best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
 }
 // Consumes a segment's query; true when the walk continues (next segment
@@ -900,24 +905,22 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
                                          const DrHitRec &hr, bool hit, DrItem &it, DrWalk &wk, DrRayPre &r,
                                          DrHitRec &best, uint32_t &cBounce, uint32_t &cSss) {
     bool exited = false, more = false;
-    vec3 ro = it.o, rd = it.d;
+    vec3 ro = r.o, rd = r.d;
     float prevPdf = 0.0f;
     if (hit) {
         float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
-        Surf b = surfaceAt<false>(S, it.o, it.d, h4, h4);
+        Surf b = surfaceAt<false>(S, ro, rd, h4, h4);
         float f0 = randomFloat(wk.state);
         float f1 = randomFloat(wk.state);
         float ff = sampleFreeFlight(wk.sigmaT, f0, f1);
         if (ff < b.t) {
-            it.o += it.d * ff;
+            ro += rd * ff;
             float h0 = randomFloat(wk.state);
             float h1 = randomFloat(wk.state);
-            it.d = sampleHenyeyGreenstein(h0, h1, it.d, it.a);
+            rd = sampleHenyeyGreenstein(h0, h1, rd, it.a);
             vec3 tr = vexp((-wk.sigmaT) * ff);
             it.w *= wk.albedo * wk.sigmaT * tr / collisionPDF(wk.sigmaT, ff);
             more = !(maxComponent(it.w) <= 1e-4f || ++it.step >= 8u);
-            ro = it.o;
-            rd = it.d;
         } else {
             vec3 exitN = normalize(b.frontFacing ? b.normal : -b.normal);
             float x0 = randomFloat(wk.state);
@@ -933,7 +936,7 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
     if (more) {
         cBounce++;
         cSss++;
-        r = dr_prep(it.o, it.d);
+        r = dr_prep(ro, rd);
         best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
         return true;
     }
@@ -944,8 +947,10 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
         P.rayO[slot] = make_float4(ro.x, ro.y, ro.z, prevPdf);
         P.rayD[slot] = make_float4(rd.x, rd.y, rd.z, 0.0f);
     }
-    P.thr[slot] = make_float4(it.w.x, it.w.y, it.w.z, it.b);
-    P.meta[slot] = make_uint4(wk.meta.x, wk.meta.y, wk.state, wk.meta.w);
+    const float alpha = P.thr[slot].w;
+    const uint4 meta = P.meta[slot];
+    P.thr[slot] = make_float4(it.w.x, it.w.y, it.w.z, alpha);
+    P.meta[slot] = make_uint4(meta.x, meta.y, wk.state, meta.w);
     uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
     if (alive) P.queue[(bounce + 1u) & 1u][pos] = slot;
     const bool resume = !exited;
@@ -962,7 +967,7 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
 // traces only the shadow rays k_shade<RESUME> adds after the failed walks.
 // Results are the same as tracing them in separate kernels: each item
 // writes only its own records.
-template <bool VOL>
+template <bool VOL, bool STATS>
 __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S, DrLaunch L, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t count = P.counters[DR_CTR_SSSQ + bounce];
@@ -1052,7 +1057,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
         if (resume) P.resumeQueue[rpos] = slot;
     }
-    if (L.statsEnabled) {
+    if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
         waveAddStat(P.stats, DR_CNT_SSS_NODES, cNodes);
         waveAddStat(P.stats, DR_CNT_SSS_TRIS, cTris);
@@ -1127,8 +1132,8 @@ __device__ __forceinline__ bool shadowChain(const DrSceneDev &S, vec3 o, vec3 d,
 // shadowOccluded (PathTrace.metal:4196-4274) for the bounce's NEE rays not
 // traced by k_sss (on triangles: those k_shade<RESUME> queued; with volumes:
 // all of them); writes each ray's transmittance (0 if occluded) to shR[k].
-template <bool VOL>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce, uint32_t statsOn) {
+template <bool VOL, bool STATS>
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t begin = P.counters[DR_CTR_SHBEGIN + bounce];
     const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce] - begin;
@@ -1172,7 +1177,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_TRACE_OCC k_shadow(DrSceneD
         const bool occluded = shadowChain<VOL>(S, o, d, maxDistance, tr, stk, nq, nv, nt, sc);
         P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
     }
-    if (statsOn) {
+    if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_SHADOW, nq);
         waveAddStat(P.stats, DR_CNT_SHADOW_NODES, nv);
         waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, nt);
@@ -1570,10 +1575,14 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
         hipLaunchKernelGGL(k_raygen, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, cam, P, nSlots);
         break;
     case DR_STAGE_EXTEND:
-        if (vol)
-            hipLaunchKernelGGL(k_extend<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        if (vol && L.statsEnabled)
+            hipLaunchKernelGGL((k_extend<true, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+        else if (vol)
+            hipLaunchKernelGGL((k_extend<true, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+        else if (L.statsEnabled)
+            hipLaunchKernelGGL((k_extend<false, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
         else
-            hipLaunchKernelGGL(k_extend<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+            hipLaunchKernelGGL((k_extend<false, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
         break;
     case DR_STAGE_SHADE:
     case DR_STAGE_RESUME: {
@@ -1590,16 +1599,24 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
         break;
     }
     case DR_STAGE_SSS:
-        if (vol)
-            hipLaunchKernelGGL(k_sss<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
+        if (vol && L.statsEnabled)
+            hipLaunchKernelGGL((k_sss<true, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
+        else if (vol)
+            hipLaunchKernelGGL((k_sss<true, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
+        else if (L.statsEnabled)
+            hipLaunchKernelGGL((k_sss<false, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
         else
-            hipLaunchKernelGGL(k_sss<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
+            hipLaunchKernelGGL((k_sss<false, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, P, bounce);
         break;
     case DR_STAGE_SHADOW:
-        if (vol)
-            hipLaunchKernelGGL(k_shadow<true>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+        if (vol && L.statsEnabled)
+            hipLaunchKernelGGL((k_shadow<true, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+        else if (vol)
+            hipLaunchKernelGGL((k_shadow<true, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+        else if (L.statsEnabled)
+            hipLaunchKernelGGL((k_shadow<false, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
         else
-            hipLaunchKernelGGL(k_shadow<false>, dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce, L.statsEnabled);
+            hipLaunchKernelGGL((k_shadow<false, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
         break;
     case DR_STAGE_PREVIEW: {
         const uint32_t px = L.tileWidth * L.tileHeight;
