@@ -367,7 +367,8 @@ def main():
     sdf_ext, sdf_sh = c[8:18], c[18:28]
     queries_per_step = c[0] + c[1] + c[2]
     log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
-        f"shadow {c[2]:.0f}), node fetches {c[3]:.0f} + shadow {c[5]:.0f} + subsurface {c[28]:.0f}, "
+        f"shadow {c[2]:.0f}: {c[30]:.0f} beside the walks, {c[33]:.0f} beside the next extension), "
+        f"node fetches {c[3]:.0f} + shadow {c[5]:.0f} + subsurface {c[28]:.0f}, "
         f"tri tests {c[4]:.0f} + shadow {c[6]:.0f} + subsurface {c[29]:.0f}")
 
     sess = renderer.make_session(gscene, settings, camera=scene.camera, samples_per_launch=1)
@@ -394,11 +395,14 @@ def main():
     st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
     dom = max(("extend", "shadow", "shade", "sss"), key=lambda k: st_ms[k])
     # On triangle scenes k_sss also traces the shadow rays queued by k_shade
-    # (counters 30-32); k_shadow traces the rest.
+    # (counters 30-32) and the next bounce's k_extend those of resumed paths
+    # (33-35); k_shadow traces the last bounce's rest.
     if dom == "shadow":
-        nodes, tris, rays, sdf = c[5] - c[31], c[6] - c[32], c[2] - c[30], sdf_sh
+        nodes, tris, rays, sdf = c[5] - c[31] - c[34], c[6] - c[32] - c[35], c[2] - c[30] - c[33], sdf_sh
     elif dom == "sss":  # subsurface-walk queries + the shadow rays traced beside them
         nodes, tris, rays, sdf = c[28] + c[31], c[29] + c[32], c[7] + c[30], sdf_ext * 0.0
+    elif dom == "extend":  # extension queries + the previous bounce's remaining shadow rays
+        nodes, tris, rays, sdf = c[3] + c[34], c[4] + c[35], c[0] + c[1] - c[7] + c[33], sdf_ext
     else:
         nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
     launches_per_step = max(1, st_n[dom] // max(1, args.steps))

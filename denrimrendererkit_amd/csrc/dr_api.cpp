@@ -719,14 +719,26 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
         DR_HIP(run(DR_STAGE_PREVIEW, 0), "preview launch");
         return DR_OK;
     }
+    // Triangle scenes: k_sss traces the shadow rays k_shade queued and the
+    // next bounce's k_extend the rest, so k_resolve of bounce b follows
+    // k_extend of b + 1 (k_shadow runs only after the last bounce). With
+    // volumes every bounce ends with k_shadow and k_resolve.
+    const bool vol = s->sceneDev.sdf.volumeCount != 0u;
     DR_HIP(run(DR_STAGE_RAYGEN, 0), "raygen launch");
     for (uint32_t b = 0; b < L.maxBounces; ++b) {
         DR_HIP(run(DR_STAGE_EXTEND, b), "extend launch");
+        if (!vol && b > 0) DR_HIP(run(DR_STAGE_RESOLVE, b - 1), "resolve launch");
         DR_HIP(run(DR_STAGE_SHADE, b), "shade launch");
         DR_HIP(run(DR_STAGE_SSS, b), "subsurface launch");
         DR_HIP(run(DR_STAGE_RESUME, b), "shade resume launch");
-        DR_HIP(run(DR_STAGE_SHADOW, b), "shadow launch");
-        DR_HIP(run(DR_STAGE_RESOLVE, b), "resolve launch");
+        if (vol) {
+            DR_HIP(run(DR_STAGE_SHADOW, b), "shadow launch");
+            DR_HIP(run(DR_STAGE_RESOLVE, b), "resolve launch");
+        }
+    }
+    if (!vol && L.maxBounces > 0) {
+        DR_HIP(run(DR_STAGE_SHADOW, L.maxBounces - 1), "shadow launch");
+        DR_HIP(run(DR_STAGE_RESOLVE, L.maxBounces - 1), "resolve launch");
     }
     DR_HIP(run(DR_STAGE_FINALIZE, 0), "finalize launch");
     return DR_OK;
@@ -740,6 +752,7 @@ static void collectStageTimes(dr_session *s) {
     size_t i = 0;
     const uint32_t mb = std::max(1u, s->settings.max_bounces);
     const bool preview = s->settings.quality < 2u;
+    const bool vol = s->sceneDev.sdf.volumeCount != 0u;
     while (i + 1 < s->stageEvents.size()) {
         auto add = [&](DrStage st) {
             float ms = 0.f;
@@ -752,11 +765,18 @@ static void collectStageTimes(dr_session *s) {
             continue;
         }
         add(DR_STAGE_RAYGEN);
-        for (uint32_t b = 0; b < mb; ++b) {
+        for (uint32_t b = 0; b < mb; ++b) {  // the launch order of enqueueBatch
             add(DR_STAGE_EXTEND);
+            if (!vol && b > 0) add(DR_STAGE_RESOLVE);
             add(DR_STAGE_SHADE);
             add(DR_STAGE_SSS);
             add(DR_STAGE_RESUME);
+            if (vol) {
+                add(DR_STAGE_SHADOW);
+                add(DR_STAGE_RESOLVE);
+            }
+        }
+        if (!vol) {
             add(DR_STAGE_SHADOW);
             add(DR_STAGE_RESOLVE);
         }

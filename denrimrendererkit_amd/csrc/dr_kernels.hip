@@ -159,57 +159,6 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
     P.queue[0][i] = i;
 }
 
-// ---------------------------------------------------------------- extend --
-// STATS: the counting variant (node, triangle and query counters); the
-// other one keeps no counters at all.
-template <bool VOL, bool STATS>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce) {
-    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
-    const uint32_t *queue = P.queue[bounce & 1u];
-    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
-    uint32_t *steal = &P.counters[DR_CTR_EXTEND + bounce];
-    uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
-    uint32_t nv = 0, nt = 0;
-    DrSdfCounters sc{};
-    if constexpr (!VOL) {
-        uint32_t slot = 0;
-        dr_trace_queue<false>(
-            S, count, steal, stk, nv, nt,
-            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
-                slot = queue[qi];
-                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
-                r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
-                best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
-            },
-            [&](const DrHitRec &h, bool, DrRayPre &, DrHitRec &) {
-                P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-                return false;
-            });
-    } else {
-        while (true) {
-            uint32_t base = 0;
-            if (laneId() == 0) base = atomicAdd(steal, 64u);
-            base = __shfl(base, 0);
-            if (base >= count) break;
-            uint32_t qi = base + laneId();
-            if (qi < count) {
-                uint32_t slot = queue[qi];
-                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
-                float4 h, hn;
-                sceneClosest<true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, nv, nt, sc, h, hn);
-                P.hit[slot] = h;
-                if (__float_as_uint(h.w) != DR_MISS && (__float_as_uint(h.w) & DR_VOL_HIT)) P.hitN[slot] = hn;
-            }
-        }
-    }
-    if constexpr (STATS) {
-        waveAddStat(P.stats, DR_CNT_NODES, nv);
-        waveAddStat(P.stats, DR_CNT_TRIS, nt);
-        if constexpr (VOL) addSdfStats(P.stats, sc);
-    }
-}
-
 // Fills the surface record of a hit (attribute half of intersectTriangle,
 // PathTrace.metal:1841-1850, and the Hit fields set at :3372-3384).
 struct Surf {
@@ -866,6 +815,97 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
     }
     P.shR[it.id] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(it.w.x, it.w.y, it.w.z, 1.0f);
     return false;
+}
+
+// ---------------------------------------------------------------- extend --
+// STATS: the counting variant (node, triangle and query counters); the
+// other one keeps no counters at all.
+//
+// On triangles the launch also traces the previous bounce's remaining shadow
+// rays (those k_shade<RESUME> queued after k_sss), claimed after the
+// extension rays: they are short chains whose latency would otherwise set
+// the length of a launch of their own. Their k_resolve runs after this
+// kernel and before the next k_shade, so radiance is summed in the same
+// order as before.
+template <bool VOL, bool STATS>
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce) {
+    __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
+    const uint32_t *queue = P.queue[bounce & 1u];
+    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    uint32_t *steal = &P.counters[DR_CTR_EXTEND + bounce];
+    uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
+    uint32_t nv = 0, nt = 0;
+    uint32_t shQ = 0u, shNodes = 0u, shTris = 0u;
+    DrSdfCounters sc{};
+    if constexpr (!VOL) {
+        const uint32_t shBegin = bounce ? P.counters[DR_CTR_SHBEGIN + bounce - 1u] : 0u;
+        const uint32_t nShadow = bounce ? P.counters[DR_CTR_SHADOWQ + bounce - 1u] - shBegin : 0u;
+        const uint32_t *shQueue = P.shadowQueue + shBegin;
+        const bool anyShadow = S.opaqueOnly != 0u;
+        uint32_t slot = 0, markV = 0u, markT = 0u;
+        bool shadow = false, laneAny = false;
+        DrItem it{};
+        dr_trace_queue<false>(
+            S, count + nShadow, steal, stk, nv, nt,
+            [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
+                shadow = qi >= count;
+                if (!shadow) {
+                    slot = queue[qi];
+                    float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                    r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                    best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
+                    laneAny = false;
+                } else {
+                    shadowBegin(P, shQueue[qi - count], it, r, best);
+                    ++shQ;
+                    laneAny = anyShadow;
+                }
+                markV = nv;
+                markT = nt;
+            },
+            [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
+                if (!shadow) {
+                    P.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+                    return false;
+                }
+                shNodes += nv - markV;
+                shTris += nt - markT;
+                markV = nv;
+                markT = nt;
+                return shadowStep(S, P, anyShadow, h, hit, it, r, best, shQ);
+            },
+            &laneAny);
+        nv -= shNodes;
+        nt -= shTris;
+    } else {
+        while (true) {
+            uint32_t base = 0;
+            if (laneId() == 0) base = atomicAdd(steal, 64u);
+            base = __shfl(base, 0);
+            if (base >= count) break;
+            uint32_t qi = base + laneId();
+            if (qi < count) {
+                uint32_t slot = queue[qi];
+                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                float4 h, hn;
+                sceneClosest<true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, nv, nt, sc, h, hn);
+                P.hit[slot] = h;
+                if (__float_as_uint(h.w) != DR_MISS && (__float_as_uint(h.w) & DR_VOL_HIT)) P.hitN[slot] = hn;
+            }
+        }
+    }
+    if constexpr (STATS) {
+        waveAddStat(P.stats, DR_CNT_NODES, nv);
+        waveAddStat(P.stats, DR_CNT_TRIS, nt);
+        waveAddStat(P.stats, DR_CNT_SHADOW, shQ);
+        waveAddStat(P.stats, DR_CNT_SHADOW_NODES, shNodes);
+        waveAddStat(P.stats, DR_CNT_SHADOW_TRIS, shTris);
+        waveAddStat(P.stats, DR_CNT_EXT_SHADOW, shQ);
+        waveAddStat(P.stats, DR_CNT_EXT_SHADOW_NODES, shNodes);
+        waveAddStat(P.stats, DR_CNT_EXT_SHADOW_TRIS, shTris);
+        if constexpr (VOL) addSdfStats(P.stats, sc);
+    }
 }
 
 // continueSubsurfaceRandomWalk (PathTrace.metal:3525-3621) on triangles as

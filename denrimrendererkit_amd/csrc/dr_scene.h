@@ -170,5 +170,10 @@ enum {
     DR_CNT_SSS_SHADOW = 30,
     DR_CNT_SSS_SHADOW_NODES = 31,
     DR_CNT_SSS_SHADOW_TRIS = 32,
-    DR_CNT_SLOTS = 33
+    // The part traced inside k_extend (the previous bounce's shadow rays
+    // queued by k_shade<RESUME>, triangle scenes).
+    DR_CNT_EXT_SHADOW = 33,
+    DR_CNT_EXT_SHADOW_NODES = 34,
+    DR_CNT_EXT_SHADOW_TRIS = 35,
+    DR_CNT_SLOTS = 36
 };

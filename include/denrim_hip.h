@@ -192,7 +192,12 @@ typedef enum {
     DR_STAT_SHADOW_QUERIES_IN_WALK_KERNEL = 30,
     DR_STAT_SHADOW_NODE_VISITS_IN_WALK_KERNEL = 31,
     DR_STAT_SHADOW_TRIANGLE_TESTS_IN_WALK_KERNEL = 32,
-    DR_STAT_COUNT = 33
+    /* 33-35: the part traced by the extension kernel's launch (the previous
+     * bounce's shadow rays of paths whose subsurface walk failed). */
+    DR_STAT_SHADOW_QUERIES_IN_EXTEND_KERNEL = 33,
+    DR_STAT_SHADOW_NODE_VISITS_IN_EXTEND_KERNEL = 34,
+    DR_STAT_SHADOW_TRIANGLE_TESTS_IN_EXTEND_KERNEL = 35,
+    DR_STAT_COUNT = 36
 } dr_stat;
 
 /* SDF primitive for the volume builders (SDFPrimitive / CompiledPrimitive,
