@@ -785,16 +785,19 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
     if (!hit || it.b + h.t >= it.a) {
         occluded = false;
     } else {
-        float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-        Surf sh = surfaceAt<false>(S, o, d, h4, h4);
-        Mat m = materialFor(S, sh);
+        // materialForHit of a triangle hit is its material record; the
+        // interpolated surface is only needed when light can pass.
+        const DrMaterial *mp = &S.materials[min(S.triangles[h.id].materialID, S.materialCount - 1u)];
+        const Mat m{mp, mp, 0.0f, mp->s[M_BASE], 0u, {}, {}, {}};
         if (m.base.w <= 0.001f) {
-            it.b += sh.t;
-            o = sh.position + d * 0.001f;
+            it.b += h.t;
+            o = (o + h.t * d) + d * 0.001f;
             next = true;
         } else {
             float transmission = materialTransmission(m);
             if (transmission > 0.001f) {
+                float4 h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+                Surf sh = surfaceAt<false>(S, o, d, h4, h4);
                 bool inside = !sh.frontFacing && !materialThinWalled(m);
                 vec3 medium = materialHasVolumeScattering(m) && inside ? materialVolumeTransmittance(m, sh.t)
                                                                        : materialTransmissionAbsorption(m, sh.t, inside);
@@ -948,12 +951,10 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
     vec3 ro = r.o, rd = r.d;
     float prevPdf = 0.0f;
     if (hit) {
-        float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
-        Surf b = surfaceAt<false>(S, ro, rd, h4, h4);
         float f0 = randomFloat(wk.state);
         float f1 = randomFloat(wk.state);
         float ff = sampleFreeFlight(wk.sigmaT, f0, f1);
-        if (ff < b.t) {
+        if (ff < hr.t) {  // a collision inside: the boundary surface is not needed
             ro += rd * ff;
             float h0 = randomFloat(wk.state);
             float h1 = randomFloat(wk.state);
@@ -962,6 +963,8 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
             it.w *= wk.albedo * wk.sigmaT * tr / collisionPDF(wk.sigmaT, ff);
             more = !(maxComponent(it.w) <= 1e-4f || ++it.step >= 8u);
         } else {
+            float4 h4 = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
+            Surf b = surfaceAt<false>(S, ro, rd, h4, h4);
             vec3 exitN = normalize(b.frontFacing ? b.normal : -b.normal);
             float x0 = randomFloat(wk.state);
             float x1 = randomFloat(wk.state);
