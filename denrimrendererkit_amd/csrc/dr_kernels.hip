@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         DrWalk wk{};
         bool walk = false, laneAny = false;
         uint32_t markV = 0u, markT = 0u;
-        dr_trace_queue<false>(
+        dr_trace_queue<false, true>(
             S, count + nShadow, claim, stk, cNodes, cTris,
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
                 walk = qi < count;
@@ -1193,13 +1193,13 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrScene
             ++nq;
         };
         if (S.opaqueOnly) {
-            dr_trace_queue<true>(S, count, steal, stk, nv, nt, fetch,
+            dr_trace_queue<true, true>(S, count, steal, stk, nv, nt, fetch,
                                  [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
                                      return shadowStep(S, P, true, h, hit, it, r, best, nq);
                                  });
         } else {
             // A lane continues its chain without waiting for the wave.
-            dr_trace_queue<false>(S, count, steal, stk, nv, nt, fetch,
+            dr_trace_queue<false, true>(S, count, steal, stk, nv, nt, fetch,
                                   [&](const DrHitRec &h, bool hit, DrRayPre &r, DrHitRec &best) {
                                       return shadowStep(S, P, false, h, hit, it, r, best, nq);
                                   });

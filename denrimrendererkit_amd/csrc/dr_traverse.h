@@ -366,7 +366,15 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
 #ifndef DR_REFILL_MIN
 #define DR_REFILL_MIN 16
 #endif
-template <bool ANY, class Fetch, class Retire>
+#ifndef DR_RETIRE_MIN
+#define DR_RETIRE_MIN 8
+#endif
+// STEPWISE selects the loop shape: false = while-while (node phase until
+// every lane holds a leaf, then the leaf phase; best for the extension
+// queue, whose items are single queries), true = one node or triangle step
+// per lane and iteration with batched retires (best for queues of chained
+// queries -- shadow chains, subsurface walks -- whose lanes desynchronise).
+template <bool ANY, bool STEPWISE = false, class Fetch, class Retire>
 __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t count, uint32_t *claim, DrStack &stk,
                                                uint32_t &nv, uint32_t &nt, Fetch fetch, Retire retire,
                                                const bool *laneAny = nullptr) {
@@ -410,15 +418,38 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
             if (!more) break;
             continue;
         }
-        while (__any(active && !s.done && s.leafCount == 0u)) {
-            if (active && !s.done && (s.leafCount == 0u || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
-        }
-        while (__any(active && s.leafCount != 0u)) {
-            if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
-                found = true;
-                if (ANY || (laneAny && *laneAny)) {
-                    s.done = true;
-                    s.leafCount = 0;
+        if constexpr (STEPWISE) {
+            // One step per lane and iteration: a triangle of the pending leaf,
+            // else an inner node; finished lanes retire together once
+            // DR_RETIRE_MIN of them are waiting (or nothing else is tracing).
+            const bool tracing = active && !(s.done && s.leafCount == 0u);
+            if (tracing) {
+                if (s.leafCount != 0u) {
+                    if (dr_leaf_step(S, r, s, best, nt)) {
+                        found = true;
+                        if (ANY || (laneAny && *laneAny)) {
+                            s.done = true;
+                            s.leafCount = 0;
+                        }
+                    }
+                } else {
+                    dr_node_step(S, r, best.t, s, stk, nv);
+                }
+            }
+            const bool waiting = active && s.done && s.leafCount == 0u;
+            const uint64_t wmask = __ballot(waiting);
+            if (!wmask || (__popcll(wmask) < DR_RETIRE_MIN && __any(active && !(s.done && s.leafCount == 0u)))) continue;
+        } else {
+            while (__any(active && !s.done && s.leafCount == 0u)) {
+                if (active && !s.done && (s.leafCount == 0u || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
+            }
+            while (__any(active && s.leafCount != 0u)) {
+                if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
+                    found = true;
+                    if (ANY || (laneAny && *laneAny)) {
+                        s.done = true;
+                        s.leafCount = 0;
+                    }
                 }
             }
         }
