@@ -50,6 +50,10 @@
 #define DR_EXTEND_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_EXTEND_WAVES)))
 #define DR_SHADOW_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADOW_WAVES)))
 #define DR_SSS_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SSS_WAVES)))
+#ifndef DR_SHADE_WAVES
+#define DR_SHADE_WAVES 1
+#endif
+#define DR_SHADE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADE_WAVES)))
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
 // Sample index of the batch's first sample: a launch argument, or a device
@@ -457,7 +461,7 @@ __device__ __forceinline__ void stashWalkRay(const DrPaths &P, uint32_t slot, ve
 // resumeQueue continue at continueTransmissiveSurface with the throughput
 // and RNG state their failed walk left.
 template <bool VOL, bool RESUME>
-__global__ void __launch_bounds__(DR_SHADE_BLOCK) k_shade(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
+__global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDev S, DrLaunch L, DrCamera cam, DrCamera prevCam,
                                                          DrPaths P, DrOutputs O, uint32_t bounce) {
     const uint32_t *queue = RESUME ? P.resumeQueue : P.queue[bounce & 1u];
     uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
