@@ -254,10 +254,27 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
             leafMask &= ~(1u << lc);
         }
         // The other hit leaves are pushed in child order.
-        if (leafMask & 1u) stk.put(s.sp++, DR_LEAF_REF | (cnt & 0xffu) << 27 | n.child.x);
-        if (leafMask & 2u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 8 & 0xffu) << 27 | n.child.y);
-        if (leafMask & 4u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 16 & 0xffu) << 27 | n.child.z);
-        if (leafMask & 8u) stk.put(s.sp++, DR_LEAF_REF | (cnt >> 24) << 27 | n.child.w);
+        const uint32_t l0 = DR_LEAF_REF | (cnt & 0xffu) << 27 | n.child.x;
+        const uint32_t l1 = DR_LEAF_REF | (cnt >> 8 & 0xffu) << 27 | n.child.y;
+        const uint32_t l2 = DR_LEAF_REF | (cnt >> 16 & 0xffu) << 27 | n.child.z;
+        const uint32_t l3 = DR_LEAF_REF | (cnt >> 24) << 27 | n.child.w;
+        if (s.sp + 4u <= DR_LDS_STACK) {
+            // Compaction by unconditional writes in child order: entry c
+            // goes to sp + (hit leaves before c); a child that is not pushed
+            // writes the slot the next pushed one then overwrites, or a
+            // slot past the new top.
+            const uint32_t b1 = leafMask & 1u, b2 = b1 + ((leafMask >> 1) & 1u), b3 = b2 + ((leafMask >> 2) & 1u);
+            stk.lds[s.sp * stk.ldsStride] = l0;
+            stk.lds[(s.sp + b1) * stk.ldsStride] = l1;
+            stk.lds[(s.sp + b2) * stk.ldsStride] = l2;
+            stk.lds[(s.sp + b3) * stk.ldsStride] = l3;
+            s.sp += __builtin_popcount(leafMask);
+        } else {
+            if (leafMask & 1u) stk.put(s.sp++, l0);
+            if (leafMask & 2u) stk.put(s.sp++, l1);
+            if (leafMask & 4u) stk.put(s.sp++, l2);
+            if (leafMask & 8u) stk.put(s.sp++, l3);
+        }
     }
     // Inner children in order of entry distance (a 5-comparator network over
     // (distance, reference), misses keyed +inf): the nearest is next, the
