@@ -47,8 +47,11 @@
 #ifndef DR_SSS_WAVES
 #define DR_SSS_WAVES 4
 #endif
-#define DR_EXTEND_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_EXTEND_WAVES)))
-#define DR_SHADOW_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADOW_WAVES)))
+#ifndef DR_VOL_WAVES
+#define DR_VOL_WAVES 3
+#endif
+#define DR_EXTEND_OCC __attribute__((amdgpu_waves_per_eu(VOL ? DR_VOL_WAVES : DR_EXTEND_WAVES)))
+#define DR_SHADOW_OCC __attribute__((amdgpu_waves_per_eu(VOL ? DR_VOL_WAVES : DR_SHADOW_WAVES)))
 #define DR_SSS_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SSS_WAVES)))
 #ifndef DR_SHADE_WAVES
 #define DR_SHADE_WAVES 1
