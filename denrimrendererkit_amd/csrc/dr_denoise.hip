@@ -181,6 +181,93 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs c) {
     }
 }
 
+// The same convolution with the input tile staged in LDS: per 32-channel
+// chunk the workgroup loads its (4 * MB + 2) x 18 pixels (tile plus 1-pixel
+// halo) once, and the nine taps read their A operands from LDS instead of
+// re-reading global memory nine times. Pixels are 80 B apart in LDS (32
+// channels + 8 of padding), so the 16 lanes of a channel group read 16
+// disjoint 4-bank spans. Accumulation runs chunk-major (tap-major inside),
+// which only reorders the FP32 sums of inputs wider than 32 channels.
+template <int NB, int MB>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
+    constexpr int TH = 4 * MB, LR = TH + 2, LC = 18, PIX = 40;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[LR * LC * PIX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tilesX = (c.W + 15) >> 4, tilesY = (c.H + TH - 1) / TH;
+    int t = blockIdx.x;
+    const int img = t / (tilesX * tilesY);
+    t -= img * tilesX * tilesY;
+    const int ty = t / tilesX, tx = t - ty * tilesX;
+    const int row0 = ty * TH + wave * MB;
+    const int kg = (lane >> 4) * 8;
+    const int cin = c.ca + c.cb;
+    const int Hb = c.H >> 1, Wb = c.W >> 1;
+    f4 acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
+    const bf8 zero8 = {};
+    for (int kc = 0; kc < cin; kc += 32) {
+        __syncthreads();  // the previous chunk's reads are done
+        for (int v = threadIdx.x; v < LR * LC * 4; v += 256) {
+            const int pix = v >> 2, q = v & 3;
+            const int ly = pix / LC, lx = pix - ly * LC;
+            const int sy = ty * TH - 1 + ly, sx = tx * 16 - 1 + lx;
+            const int k0 = kc + q * 8;
+            bf8 val = zero8;
+            if (k0 < cin && sy >= 0 && sy < c.H && sx >= 0 && sx < c.W) {
+                const uint16_t *src =
+                    k0 < c.ca ? c.a + ((size_t(img) * c.H + sy) * c.W + sx) * c.ca + k0
+                              : c.b + ((size_t(img) * Hb + (sy >> 1)) * Wb + (sx >> 1)) * c.cb + (k0 - c.ca);
+                val = *reinterpret_cast<const bf8 *>(src);
+            }
+            *reinterpret_cast<bf8 *>(&tile[pix * PIX + q * 8]) = val;
+        }
+        __syncthreads();
+        const int k0 = kc + kg;
+        const bool kin = k0 < cin;
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin;
+            bf8 bv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                bv[nb] = kin ? *reinterpret_cast<const bf8 *>(wt + size_t(nb) * 16 * cin + k0) : zero8;
+            const int lx = (lane & 15) + dx + 1;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const int ly = wave * MB + mb + dy + 1;
+                const bf8 av = *reinterpret_cast<const bf8 *>(&tile[(ly * LC + lx) * PIX + kg]);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
+    }
+    // D: lane holds rows (pixels) 4 * (lane / 16) + j, column (channel) lane % 16.
+    const int co0 = lane & 15;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int oy = row0 + mb;
+        if (oy >= c.H) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int ox = tx * 16 + (lane >> 4) * 4 + j;
+            if (ox >= c.W) continue;
+            const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + co0;
+                float v = acc[mb][nb][j] + (c.bias ? c.bias[co] : 0.f);
+                if (c.relu) v = fmaxf(v, 0.f);
+                if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
+                else static_cast<uint16_t *>(c.out)[o + co] = f2bf(v);
+            }
+        }
+    }
+}
+
 constexpr int kWgRow = 64 + 4;  // LDS row: 64 pixels + 4 elements of padding
 constexpr int kWgMaxQ = 16;     // weight tiles per wave (cout/16 * cin/16 <= 64)
 
@@ -463,10 +550,14 @@ hipError_t launchAtrous(const float4 *src, const float4 *depth, const float4 *no
     return hipGetLastError();
 }
 
+#ifndef DR_CONV_LDS
+#define DR_CONV_LDS 1
+#endif
 template <int NB, int MB>
 hipError_t convLaunch(const ConvArgs &c, hipStream_t stream) {
     const unsigned grid = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 4 * MB - 1) / (4 * MB)));
-    if (grid) hipLaunchKernelGGL((k_conv3x3<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
+    if (grid && DR_CONV_LDS) hipLaunchKernelGGL((k_conv3x3_lds<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
+    else if (grid) hipLaunchKernelGGL((k_conv3x3<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
     return hipGetLastError();
 }
 
