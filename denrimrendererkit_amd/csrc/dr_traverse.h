@@ -4,12 +4,14 @@
 //   * 4-wide BVH: one node fetch (8 x dwordx4, 128 B) tests four child
 //     boxes with a fused-multiply-add slab test against precomputed 1/d and
 //     o/d, so a ray needs about half the dependent fetches of a binary tree;
-//   * while-while order: inner nodes (nearest hit child next, the others
-//     pushed far to near) until every lane of the wave holds a hit leaf,
-//     then the leaves' triangles, converged;
+//   * the nearest hit inner child is visited next, the others are pushed far
+//     to near; hit leaves become pending (one per lane) or stack entries;
+//   * two loop shapes (dr_trace_queue): while-while (inner nodes until every
+//     lane of the wave holds a leaf, then the leaves converged) and stepwise
+//     (one node or triangle per lane and iteration);
 //   * the traversal stack lives in LDS (DR_LDS_STACK entries per lane,
-//     lane-interleaved so a wave's pushes hit 64 distinct banks) and spills
-//     to a per-lane global area only past that depth.
+//     lane-interleaved) and spills to a per-lane global area only past that
+//     depth.
 // The triangle test restates PathTrace.metal:1801-1852 operation for
 // operation (Moeller-Trumbore, |det| < 1e-7 reject, t > 0.0005) with no
 // contraction, and the closest-hit update takes the lower triangle index on
@@ -81,9 +83,9 @@ __device__ __forceinline__ float dr_box(const DrRayPre &r, float x0, float y0, f
     return tn <= tf ? tn : __builtin_inff();
 }
 
-// Moeller-Trumbore with precomputed edges (e1 = v1 - v0, e2 = v2 - v0 are
-// computed on the host with the same float subtraction the reference does).
-// The IEEE reciprocal is the expensive step, so the sign tests run first on
+// Moeller-Trumbore from the exact vertices (e1 = v1 - v0, e2 = v2 - v0 with
+// the same float subtractions as the reference). The IEEE reciprocal is the
+// expensive step, so the sign tests run first on
 // the numerators: u = inv * un is negative exactly when un and det differ in
 // sign and the product does not underflow to -0; only rejections that are
 // certain (|un| > 2^-100 |det|) are taken early, everything else falls
