@@ -211,22 +211,33 @@ __device__ __forceinline__ void dr_cswap(float &ka, uint32_t &pa, float &kb, uin
     pa = q;
 }
 
-// One inner-node step; requires !s.done and either no pending leaf, or a
-// known next node (speculative step: leaves found then go to the stack).
+// One inner-node step; requires !s.done. With a leaf pending the step is
+// speculative: it visits the known next node, or pops an inner node from
+// the top of the stack, and the hit leaves it finds go to the stack.
 __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre &r, float tmax, DrTrav &s,
                                              DrStack &stk, uint32_t &nv) {
     if (s.cur == DR_POP) {
-        if (s.sp == 0) {
-            s.done = true;
-            return;
+        if (s.leafCount != 0u) {
+            // Speculative step with a leaf pending: only an inner node on
+            // top of the stack is taken; a leaf entry waits.
+            if (s.sp == 0u) return;
+            const uint32_t e = stk.get(s.sp - 1u);
+            if (e & DR_LEAF_REF) return;
+            --s.sp;
+            s.cur = e;
+        } else {
+            if (s.sp == 0) {
+                s.done = true;
+                return;
+            }
+            const uint32_t e = stk.get(--s.sp);
+            if (e & DR_LEAF_REF) {
+                s.leafFirst = e & (DR_MAX_TRIANGLES - 1u);
+                s.leafCount = (e >> 27) & 15u;
+                return;
+            }
+            s.cur = e;
         }
-        const uint32_t e = stk.get(--s.sp);
-        if (e & DR_LEAF_REF) {
-            s.leafFirst = e & (DR_MAX_TRIANGLES - 1u);
-            s.leafCount = (e >> 27) & 15u;
-            return;
-        }
-        s.cur = e;
     }
     const uint32_t ni = s.cur;
     const DrNode4 n = S.nodes[ni];
@@ -349,9 +360,11 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
     bool found = false;
     while (true) {
         // Node phase: until every lane holds a leaf; lanes that already hold
-        // one keep descending (speculatively) while they know a next node.
+        // one keep descending (speculatively) while they know a next node or
+        // an inner node is on top of their stack.
         while (__any(!s.done && s.leafCount == 0)) {
-            if (!s.done && (s.leafCount == 0 || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
+            if (!s.done && (s.leafCount == 0 || s.cur != DR_POP || s.sp != 0u))
+                dr_node_step(S, r, best.t, s, stk, nv);
         }
         if (!__any(s.leafCount != 0u)) break;
         while (__any(s.leafCount != 0u)) {
@@ -460,7 +473,8 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
             if (!wmask || (__popcll(wmask) < DR_RETIRE_MIN && __any(active && !(s.done && s.leafCount == 0u)))) continue;
         } else {
             while (__any(active && !s.done && s.leafCount == 0u)) {
-                if (active && !s.done && (s.leafCount == 0u || s.cur != DR_POP)) dr_node_step(S, r, best.t, s, stk, nv);
+                if (active && !s.done && (s.leafCount == 0u || s.cur != DR_POP || s.sp != 0u))
+                    dr_node_step(S, r, best.t, s, stk, nv);
             }
             while (__any(active && s.leafCount != 0u)) {
                 if (active && s.leafCount != 0u && dr_leaf_step(S, r, s, best, nt)) {
