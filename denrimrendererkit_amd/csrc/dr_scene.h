@@ -5,10 +5,10 @@
 //     GPUMaterial 272 B, GPULightRecord 32 B, GPUEnvironmentSample 8 B,
 //     texture descriptors and RGBA32F texels; Metal/GPUTypes.swift), read
 //     only at shading time;
-//   * a traversal form built by this library: a binary BVH whose nodes hold
-//     both child boxes (64 B per node, one 4 x dwordx4 fetch tests two
-//     boxes) and a position-only triangle stream in leaf order, 48 B per
-//     triangle {v0 | original index, v1 - v0, v2 - v0}, so a leaf's triangles
+//   * a traversal form built by this library: a four-wide BVH whose nodes
+//     hold their four child boxes (DrNode4, 128 B, one 8 x dwordx4 fetch
+//     tests four boxes) and a position-only triangle stream in leaf order,
+//     48 B per triangle {v0 | original index, v1, v2}, so a leaf's triangles
 //     are contiguous and the traversal never touches the 192-B records.
 #pragma once
 
