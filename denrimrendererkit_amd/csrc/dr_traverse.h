@@ -45,7 +45,11 @@ struct DrStack {
         else glob[(sp - DR_LDS_STACK) * gStride] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t sp) const {
-        return sp < DR_LDS_STACK ? lds[sp * ldsStride] : glob[(sp - DR_LDS_STACK) * gStride];
+        // The LDS read is unconditional (clamped slot); only a spilled entry
+        // takes the global load.
+        uint32_t v = lds[min(sp, uint32_t(DR_LDS_STACK - 1)) * ldsStride];
+        if (sp >= DR_LDS_STACK) v = glob[(sp - DR_LDS_STACK) * gStride];
+        return v;
     }
 };
 
@@ -217,27 +221,22 @@ __device__ __forceinline__ void dr_cswap(float &ka, uint32_t &pa, float &kb, uin
 __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre &r, float tmax, DrTrav &s,
                                              DrStack &stk, uint32_t &nv) {
     if (s.cur == DR_POP) {
-        if (s.leafCount != 0u) {
-            // Speculative step with a leaf pending: only an inner node on
-            // top of the stack is taken; a leaf entry waits.
-            if (s.sp == 0u) return;
-            const uint32_t e = stk.get(s.sp - 1u);
-            if (e & DR_LEAF_REF) return;
-            --s.sp;
-            s.cur = e;
-        } else {
-            if (s.sp == 0) {
-                s.done = true;
-                return;
-            }
-            const uint32_t e = stk.get(--s.sp);
-            if (e & DR_LEAF_REF) {
-                s.leafFirst = e & (DR_MAX_TRIANGLES - 1u);
-                s.leafCount = (e >> 27) & 15u;
-                return;
-            }
-            s.cur = e;
+        // With a leaf pending (speculative step) only an inner node on top
+        // of the stack is taken; a leaf entry waits.
+        if (s.sp == 0u) {
+            if (s.leafCount == 0u) s.done = true;
+            return;
         }
+        const uint32_t e = stk.get(s.sp - 1u);
+        const bool leaf = (e & DR_LEAF_REF) != 0u;
+        if (leaf && s.leafCount != 0u) return;
+        --s.sp;
+        if (leaf) {
+            s.leafFirst = e & (DR_MAX_TRIANGLES - 1u);
+            s.leafCount = (e >> 27) & 15u;
+            return;
+        }
+        s.cur = e;
     }
     const uint32_t ni = s.cur;
     const DrNode4 n = S.nodes[ni];
