@@ -5,12 +5,13 @@
 // Closest-hit results with the lowest-index tie-break do not depend on the
 // tree, so this backend builds its own: binned SAH (32 bins, 3 axes), leaves
 // of at most `maxLeaf` triangles, collapsed into four-wide nodes (each node
-// takes its two children and opens its largest inner grandchildren until it
-// has four) in depth-first order, leaf triangles stored contiguously in leaf
-// order.
+// takes the cut of its binary subtree into at most four nodes of least SAH
+// cost, chosen by a bottom-up dynamic program) in depth-first order, leaf
+// triangles stored contiguously in leaf order.
 #include "dr_bvh.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -268,27 +269,69 @@ struct Flattener {
         return me;
     }
 
-    // Up to four children of binary node ni: its two children, then the
-    // largest-area inner child opened into its two, while there is room.
-    int gather(int32_t ni, int32_t ch[4]) const {
-        int n = 0;
-        ch[n++] = tn[ni].left;
-        ch[n++] = tn[ni].right;
-        while (n < 4) {
-            int best = -1;
-            float area = -1.0f;
-            for (int i = 0; i < n; ++i) {
-                const TNode &c = tn[ch[i]];
-                if (c.count == 0 && c.box.area() > area) {
-                    area = c.box.area();
-                    best = i;
-                }
+    // Cost-optimal collapse: cut[n][k - 1] is the least SAH cost of
+    // covering binary subtree n with at most k children of one wide node
+    // (n itself, or cuts of its two subtrees); split[n][k - 1] is the number
+    // of those children given to the left subtree (0: n itself). A wide
+    // node at binary node n costs A(n) kNodeCost plus its children's cut
+    // for four slots; a leaf costs A(n) kTriCost per triangle.
+    std::vector<std::array<float, 4>> cut;
+    std::vector<std::array<uint8_t, 4>> split;
+
+    void planCollapse(uint32_t used) {
+        cut.assign(used, {});
+        split.assign(used, {});
+        // Children are allocated after their parent: a reverse sweep sees
+        // both children of a node before the node.
+        for (uint32_t i = used; i-- > 0;) {
+            const TNode &n = tn[i];
+            const float area = n.box.area();
+            if (n.count > 0 || n.left < 0) {
+                const float c = area * kTriCost * float(n.count);
+                for (int k = 0; k < 4; ++k) cut[i][k] = c, split[i][k] = 0;
+                continue;
             }
-            if (best < 0) break;
-            const int32_t e = ch[best];
-            ch[best] = tn[e].left;
-            ch[n++] = tn[e].right;
+            const auto &L = cut[n.left], &R = cut[n.right];
+            float best4 = std::numeric_limits<float>::infinity();
+            for (int l = 1; l <= 3; ++l) best4 = std::min(best4, L[l - 1] + R[4 - l - 1]);
+            const float self = area * kNodeCost + best4;
+            cut[i][0] = self;
+            split[i][0] = 0;
+            for (int k = 2; k <= 4; ++k) {
+                float b = self;
+                uint8_t s = 0;
+                for (int l = 1; l < k; ++l) {
+                    const float c = L[l - 1] + R[k - l - 1];
+                    if (c < b) b = c, s = uint8_t(l);
+                }
+                cut[i][k - 1] = b;
+                split[i][k - 1] = s;
+            }
         }
+    }
+
+    void expand(int32_t ni, int k, int32_t ch[4], int &n) const {
+        const int l = split[ni][k - 1];
+        if (l == 0) {
+            ch[n++] = ni;
+            return;
+        }
+        expand(tn[ni].left, l, ch, n);
+        expand(tn[ni].right, k - l, ch, n);
+    }
+
+    // The two to four children of the wide node made at inner binary node
+    // ni: the cheapest cut of its two subtrees into at most four nodes.
+    int gather(int32_t ni, int32_t ch[4]) const {
+        const TNode &t = tn[ni];
+        const auto &L = cut[t.left], &R = cut[t.right];
+        int bl = 1;
+        float best = std::numeric_limits<float>::infinity();
+        for (int l = 1; l <= 3; ++l)
+            if (L[l - 1] + R[4 - l - 1] < best) best = L[l - 1] + R[4 - l - 1], bl = l;
+        int n = 0;
+        expand(t.left, bl, ch, n);
+        expand(t.right, 4 - bl, ch, n);
         return n;
     }
 
@@ -367,6 +410,7 @@ void buildBvh(const DrTriangle *tris, uint32_t count, uint32_t maxLeaf, std::vec
     triPos.reserve(size_t(count) * 3);
     nodes.reserve(count / 2 + 1);
     Flattener f{tn, idx, tris, nodes, triPos};
+    f.planCollapse(b.next.load());
     f.rootArea = std::max(double(tn[root].box.area()), 1e-30);
     f.sah = kNodeCost;
     if (tn[root].count > 0) {
