@@ -123,7 +123,8 @@ __device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d
     h = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
     if constexpr (VOL) {
         DrVolHit vh;
-        if (dr_trace_volumes(S.sdf, o, d, hr.t, sc, vh)) {
+        if (S.sdf.distHalf ? dr_trace_volumes<true>(S.sdf, o, d, hr.t, sc, vh)
+                          : dr_trace_volumes<false>(S.sdf, o, d, hr.t, sc, vh)) {
             uint32_t id = DR_VOL_HIT | (vh.frontFacing ? DR_VOL_FRONT : 0u) | vh.volume;
             if (vh.dense)
                 h = make_float4(vh.t, __uint_as_float(vh.denseSample), 0.0f, __uint_as_float(id | DR_VOL_DENSE));

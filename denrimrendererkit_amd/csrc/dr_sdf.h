@@ -78,9 +78,10 @@ __device__ __forceinline__ DrBrickCtx brickCtx(const DrSdfDev &V, const DrBrick 
     return c;
 }
 
+template <bool HALF>
 __device__ __forceinline__ float sdAt(const DrSdfDev &V, const DrBrickCtx &c, uint32_t x, uint32_t y, uint32_t z) {
     uint32_t i = c.offset + x + y * c.strideY + z * c.strideZ;
-    return V.distHalf ? dr_half(V.dist16[i]) : V.dist32[i];
+    return HALF ? dr_half(V.dist16[i]) : V.dist32[i];
 }
 
 __device__ __forceinline__ void brickGrid(const DrBrickCtx &c, vec3 p, uint32_t b[3], uint32_t n[3], vec3 &f) {
@@ -96,15 +97,16 @@ __device__ __forceinline__ void brickGrid(const DrBrickCtx &c, vec3 p, uint32_t 
     f = v3(fr[0], fr[1], fr[2]);
 }
 
+template <bool HALF>
 __device__ __forceinline__ float brickDistance(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2096-2127
     if (!c.valid) return __builtin_inff();
     uint32_t b[3], n[3];
     vec3 f;
     brickGrid(c, p, b, n, f);
-    float c00 = mixf(sdAt(V, c, b[0], b[1], b[2]), sdAt(V, c, n[0], b[1], b[2]), f.x);
-    float c10 = mixf(sdAt(V, c, b[0], n[1], b[2]), sdAt(V, c, n[0], n[1], b[2]), f.x);
-    float c01 = mixf(sdAt(V, c, b[0], b[1], n[2]), sdAt(V, c, n[0], b[1], n[2]), f.x);
-    float c11 = mixf(sdAt(V, c, b[0], n[1], n[2]), sdAt(V, c, n[0], n[1], n[2]), f.x);
+    float c00 = mixf(sdAt<HALF>(V, c, b[0], b[1], b[2]), sdAt<HALF>(V, c, n[0], b[1], b[2]), f.x);
+    float c10 = mixf(sdAt<HALF>(V, c, b[0], n[1], b[2]), sdAt<HALF>(V, c, n[0], n[1], b[2]), f.x);
+    float c01 = mixf(sdAt<HALF>(V, c, b[0], b[1], n[2]), sdAt<HALF>(V, c, n[0], b[1], n[2]), f.x);
+    float c11 = mixf(sdAt<HALF>(V, c, b[0], n[1], n[2]), sdAt<HALF>(V, c, n[0], n[1], n[2]), f.x);
     float c0 = mixf(c00, c10, f.y);
     float c1 = mixf(c01, c11, f.y);
     return mixf(c0, c1, f.z);
@@ -115,16 +117,17 @@ struct DrDG {
     vec3 gradient;
 };
 
+template <bool HALF>
 __device__ __forceinline__ DrDG brickDistanceGradient(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2161-2219
     DrDG r{__builtin_inff(), v3s(0.0f)};
     if (!c.valid) return r;
     uint32_t b[3], n[3];
     vec3 f;
     brickGrid(c, p, b, n, f);
-    float d000 = sdAt(V, c, b[0], b[1], b[2]), d100 = sdAt(V, c, n[0], b[1], b[2]);
-    float d010 = sdAt(V, c, b[0], n[1], b[2]), d110 = sdAt(V, c, n[0], n[1], b[2]);
-    float d001 = sdAt(V, c, b[0], b[1], n[2]), d101 = sdAt(V, c, n[0], b[1], n[2]);
-    float d011 = sdAt(V, c, b[0], n[1], n[2]), d111 = sdAt(V, c, n[0], n[1], n[2]);
+    float d000 = sdAt<HALF>(V, c, b[0], b[1], b[2]), d100 = sdAt<HALF>(V, c, n[0], b[1], b[2]);
+    float d010 = sdAt<HALF>(V, c, b[0], n[1], b[2]), d110 = sdAt<HALF>(V, c, n[0], n[1], b[2]);
+    float d001 = sdAt<HALF>(V, c, b[0], b[1], n[2]), d101 = sdAt<HALF>(V, c, n[0], b[1], n[2]);
+    float d011 = sdAt<HALF>(V, c, b[0], n[1], n[2]), d111 = sdAt<HALF>(V, c, n[0], n[1], n[2]);
     float c00 = mixf(d000, d100, f.x), c10 = mixf(d010, d110, f.x);
     float c01 = mixf(d001, d101, f.x), c11 = mixf(d011, d111, f.x);
     float c0 = mixf(c00, c10, f.y), c1 = mixf(c01, c11, f.y);
@@ -149,6 +152,7 @@ __device__ __forceinline__ float catmullRomD(float p0, float p1, float p2, float
                    3.0f * (-p0 + 3.0f * p1 - 3.0f * p2 + p3) * t2);
 }
 
+template <bool HALF>
 __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2257-2314
     DrDG inv{__builtin_inff(), v3s(0.0f)};
     if (!c.valid) return inv;
@@ -156,7 +160,7 @@ __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, co
     float g[3] = {uvw.x * (float)(c.dim[0] - 1u), uvw.y * (float)(c.dim[1] - 1u), uvw.z * (float)(c.dim[2] - 1u)};
     uint32_t b[3] = {(uint32_t)floorf(g[0]), (uint32_t)floorf(g[1]), (uint32_t)floorf(g[2])};
     if (b[0] < 1u || b[1] < 1u || b[2] < 1u || b[0] + 2u >= c.dim[0] || b[1] + 2u >= c.dim[1] || b[2] + 2u >= c.dim[2])
-        return brickDistanceGradient(V, c, p);
+        return brickDistanceGradient<HALF>(V, c, p);
     vec3 f = v3(g[0] - (float)b[0], g[1] - (float)b[1], g[2] - (float)b[2]);
     float yr[4], dyr[4], dxyr[4];
 #pragma unroll 1
@@ -165,8 +169,8 @@ __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, co
 #pragma unroll
         for (uint32_t y = 0; y < 4; ++y) {
             uint32_t yy = b[1] + y - 1u, zz = b[2] + z - 1u;
-            float p0 = sdAt(V, c, b[0] - 1u, yy, zz), p1 = sdAt(V, c, b[0], yy, zz);
-            float p2 = sdAt(V, c, b[0] + 1u, yy, zz), p3 = sdAt(V, c, b[0] + 2u, yy, zz);
+            float p0 = sdAt<HALF>(V, c, b[0] - 1u, yy, zz), p1 = sdAt<HALF>(V, c, b[0], yy, zz);
+            float p2 = sdAt<HALF>(V, c, b[0] + 1u, yy, zz), p3 = sdAt<HALF>(V, c, b[0] + 2u, yy, zz);
             xr[y] = catmullRom(p0, p1, p2, p3, f.x);
             dxr[y] = catmullRomD(p0, p1, p2, p3, f.x);
         }
@@ -184,10 +188,11 @@ __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, co
     return r;
 }
 
+template <bool HALF>
 __device__ __forceinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &vol, const DrBrickCtx &c, vec3 p,
                                          bool cubic, bool coarse) {  // :2420-2469
     if (!c.valid) return v3(0, 1, 0);
-    vec3 ln = cubic ? brickDistanceGradientCubic(V, c, p).gradient : brickDistanceGradient(V, c, p).gradient;
+    vec3 ln = cubic ? brickDistanceGradientCubic<HALF>(V, c, p).gradient : brickDistanceGradient<HALF>(V, c, p).gradient;
     if (coarse) {
         vec3 ext = vmax(v3f(vol.localBoundsMax) - v3f(vol.localBoundsMin), v3s(1e-6f));
         vec3 dims = v3((float)vol.dimensions.x, (float)vol.dimensions.y, (float)vol.dimensions.z);
@@ -196,15 +201,15 @@ __device__ __forceinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &v
         vec3 x0 = p, x1 = p;
         x0.x = fmaxf(p.x - sp.x, mn.x);
         x1.x = fminf(p.x + sp.x, mx.x);
-        float dx = (brickDistance(V, c, x1) - brickDistance(V, c, x0)) / fmaxf(x1.x - x0.x, 1e-6f);
+        float dx = (brickDistance<HALF>(V, c, x1) - brickDistance<HALF>(V, c, x0)) / fmaxf(x1.x - x0.x, 1e-6f);
         vec3 y0 = p, y1 = p;
         y0.y = fmaxf(p.y - sp.y, mn.y);
         y1.y = fminf(p.y + sp.y, mx.y);
-        float dy = (brickDistance(V, c, y1) - brickDistance(V, c, y0)) / fmaxf(y1.y - y0.y, 1e-6f);
+        float dy = (brickDistance<HALF>(V, c, y1) - brickDistance<HALF>(V, c, y0)) / fmaxf(y1.y - y0.y, 1e-6f);
         vec3 z0 = p, z1 = p;
         z0.z = fmaxf(p.z - sp.z, mn.z);
         z1.z = fminf(p.z + sp.z, mx.z);
-        float dz = (brickDistance(V, c, z1) - brickDistance(V, c, z0)) / fmaxf(z1.z - z0.z, 1e-6f);
+        float dz = (brickDistance<HALF>(V, c, z1) - brickDistance<HALF>(V, c, z0)) / fmaxf(z1.z - z0.z, 1e-6f);
         vec3 cn = v3(dx, dy, dz);
         if (dot(cn, cn) > 1e-10f) ln = normalize(mixv(normalize(ln), normalize(cn), 0.35f));
     }
@@ -212,6 +217,7 @@ __device__ __forceinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &v
     return normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
 }
 
+template <bool HALF>
 __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd, const DrVolume &vol,
                                             const DrBrick &brick, float closestT, vec3 lo, vec3 ld, float lupw,
                                             float cellSize, DrSdfCounters &sc, DrVolHit &hit) {  // :2608-2807
@@ -231,7 +237,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
     sc.marches++;
     float minimumStep = fmaxf(cellSize * 0.10f / lupw, 0.0005f);
     float prevT = tNear;
-    float prevD = brickDistance(V, ctx, lo + ld * prevT);
+    float prevD = brickDistance<HALF>(V, ctx, lo + ld * prevT);
     if (!isfinite(prevD)) return false;
     float originalEntryT = prevT;
     if (prevD <= 0.0f) {
@@ -240,7 +246,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
         #pragma unroll 1
         for (uint32_t bt = 0; bt < 12u; ++bt) {
             searchT = fmaxf(searchT - backStep, 0.0005f);
-            float sd = brickDistance(V, ctx, lo + ld * searchT);
+            float sd = brickDistance<HALF>(V, ctx, lo + ld * searchT);
             if (!isfinite(sd)) break;
             if (sd > 0.0f) {
                 prevT = searchT;
@@ -254,7 +260,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
     #pragma unroll 1
     for (uint32_t step = 0; step < 192u && t <= tFar && t < closestT; ++step) {
         sc.marchSteps++;
-        float d = brickDistance(V, ctx, lo + ld * t);
+        float d = brickDistance<HALF>(V, ctx, lo + ld * t);
         if (!isfinite(d)) return false;
         bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
         if (crossed) {
@@ -263,7 +269,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
             #pragma unroll 1
             for (uint32_t k = 0; k < 8u; ++k) {
                 float midT = 0.5f * (lowT + highT);
-                float md = brickDistance(V, ctx, lo + ld * midT);
+                float md = brickDistance<HALF>(V, ctx, lo + ld * midT);
                 if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
                     lowT = midT;
                     lowD = md;
@@ -285,7 +291,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
                 vec3 ln;
                 #pragma unroll 1
                 for (uint32_t k = 0;; ++k) {
-                    DrDG pg = brickDistanceGradientCubic(V, ctx, rp);
+                    DrDG pg = brickDistanceGradientCubic<HALF>(V, ctx, rp);
                     ln = pg.gradient;
                     if (k == polishCount) break;
                     float slope = dot(pg.gradient, ld);
@@ -298,7 +304,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
                 if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
                 n = normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
             } else {
-                n = brickNormal(V, vol, ctx, rp, false, coarse);
+                n = brickNormal<HALF>(V, vol, ctx, rp, false, coarse);
             }
             bool ff = dot(n, rd) <= 0.0f;
             if (!ff) n = -n;
@@ -427,6 +433,7 @@ __device__ __forceinline__ bool intersectDense(const DrSdfDev &V, const DrVolume
 
 // closestVolumeHit (PathTrace.metal:3090-3163): dense volumes, then the
 // brick-grid branch for sparse volumes, below closestT.
+template <bool HALF>
 __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
                                               DrVolHit &out) {
     bool any = false;
@@ -547,7 +554,7 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
                 if (bi != 0xffffffffu && bi < V.brickCount) {
                     sc.brickTests++;
                     DrVolHit cand;
-                    if (intersectBrick(V, ro, rd, vol, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cand) &&
+                    if (intersectBrick<HALF>(V, ro, rd, vol, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cand) &&
                         cand.t < closest.t) {
                         closest = cand;
                         closest.volume = vi;
