@@ -11,8 +11,10 @@
 #define DR_MAX_BOUNCES 64
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
 // per-lane global area of (stackMax - DR_LDS_STACK) words, stackMax being
-// the scene's bound of 4 entries per BVH level plus two.
+// the scene's bound of 7 entries per BVH level plus two.
+#ifndef DR_LDS_STACK
 #define DR_LDS_STACK 24
+#endif
 
 // Device counter words: queue lengths per bounce (index b = queue consumed
 // by bounce b), work-stealing cursors of extend and shadow per bounce, and
