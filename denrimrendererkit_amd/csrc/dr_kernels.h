@@ -6,8 +6,12 @@
 
 #include "dr_scene.h"
 
+#ifndef DR_TRACE_BLOCK
 #define DR_TRACE_BLOCK 128
+#endif
+#ifndef DR_SHADE_BLOCK
 #define DR_SHADE_BLOCK 128
+#endif
 #define DR_MAX_BOUNCES 64
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
 // per-lane global area of (stackMax - DR_LDS_STACK) words, stackMax being
