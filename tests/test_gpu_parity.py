@@ -167,6 +167,34 @@ def test_ray_queries_match_oracle(renderer, mode):
         assert misses == 0
 
 
+@pytest.mark.parametrize("max_leaf", [0, 40])
+def test_ray_queries_coincident_triangles_and_big_leaves(renderer, max_leaf):
+    # 600 copies of one triangle (every ray through it has 600 equal-t
+    # candidates: the lowest index must win whatever the tree visits first)
+    # amid 1,500 random triangles; max_leaf 40 forces leaves above the
+    # 15-triangle stack-entry limit, which the builder splits into chunks.
+    # Indices and t bit-exact against the oracle's brute force.
+    from denrimrendererkit_amd import Material, Mesh, RenderScene
+    rng = np.random.default_rng(5)
+    base = np.array([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], np.float32)
+    soup = rng.uniform(-1.5, 1.5, (1500, 1, 3)).astype(np.float32) + \
+        rng.uniform(-0.2, 0.2, (1500, 3, 3)).astype(np.float32)
+    tris = np.concatenate([soup[:700], np.repeat(base[None], 600, axis=0), soup[700:]])
+    s = RenderScene()
+    s.add(Mesh(tris.reshape(-1, 3), np.arange(tris.shape[0] * 3, dtype=np.uint32)), s.add_material(Material()))
+    build = compile_scene(s)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, 0:2] = rng.uniform(-1.2, 1.2, (20000, 2))
+    rays[:, 2] = -3.0
+    rays[:, 4:7] = np.array([0.0, 0.0, 1.0], np.float32) + rng.uniform(-0.1, 0.1, (20000, 3)).astype(np.float32)
+    t, ids = renderer.upload(build, max_leaf).intersect(rays)
+    rt, rids = O.OracleScene(build).intersect(rays, brute_force=True)
+    assert np.array_equal(ids, rids), int((ids != rids).sum())
+    assert np.array_equal(t, rt)
+    stack_hits = int(((ids >= 700) & (ids < 1300)).sum())
+    assert stack_hits > 1000 and set(np.unique(ids[(ids >= 700) & (ids < 1300)])) == {700}
+
+
 def test_watertight_render_parity(renderer):
     from denrimrendererkit_amd.settings import TriangleIntersection
     scene = synthetic.cornell_room_scene()
