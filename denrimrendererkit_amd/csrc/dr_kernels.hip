@@ -36,8 +36,8 @@
 #define DR_FLAG_NEE 2u
 
 // Minimum waves per SIMD the register allocator is held to for the
-// triangle-only tracers (1 = no constraint); the volume variants are left
-// free.
+// triangle-only tracers (1 = no constraint); the volume variants of extend
+// and shadow are held to DR_VOL_WAVES, the other volume variants are free.
 #ifndef DR_EXTEND_WAVES
 #define DR_EXTEND_WAVES 1
 #endif
@@ -45,7 +45,7 @@
 #define DR_SHADOW_WAVES 1
 #endif
 #ifndef DR_SSS_WAVES
-#define DR_SSS_WAVES 4
+#define DR_SSS_WAVES 3
 #endif
 #ifndef DR_VOL_WAVES
 #define DR_VOL_WAVES 3
