@@ -22,8 +22,10 @@ N_COUNTERS = 24
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liboracle.so"], check=True)
+    """Builds liboracle.so and the BSDF pin library liboracle_pins.so."""
+    pins = os.path.join(HERE, "liboracle_pins.so")
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(pins):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "all"], check=True)
     return LIB_PATH
 
 
