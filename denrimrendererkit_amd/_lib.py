@@ -106,9 +106,12 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise LibraryMissing(f"{LIB_PATH} is not built; run `python -m denrimrendererkit_amd._build`")
-    L = C.CDLL(LIB_PATH)
+    # DENRIM_HIP_LIBRARY names another build of the library in this package
+    # folder (libdenrim_hip_lds4.so: the small-LDS-stack build the tests use).
+    path = os.path.join(PKG, os.path.basename(os.environ.get("DENRIM_HIP_LIBRARY", "libdenrim_hip.so")))
+    if not os.path.exists(path):
+        raise LibraryMissing(f"{path} is not built; run `python -m denrimrendererkit_amd._build`")
+    L = C.CDLL(path)
     vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
     L.dr_last_error.restype = C.c_char_p
     L.dr_hip_device_create.argtypes = [i32, C.POINTER(vp)]

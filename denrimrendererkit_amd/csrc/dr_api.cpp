@@ -35,6 +35,11 @@ dr_status fail(dr_status s, const std::string &msg) {
 
 dr_status drFail(dr_status s, const std::string &msg) { return fail(s, msg); }
 
+std::recursive_mutex &drDeviceMutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 namespace {
 
 #define DR_HIP(call, what)                                                                                   \
@@ -107,6 +112,7 @@ static uint32_t autoSamplesPerLaunch(uint64_t pixels) {
 }
 
 static void dropGraphs(dr_session *s) {
+    DR_DEVICE_LOCK;
     for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second);
     s->graphs.clear();
 }
@@ -143,6 +149,7 @@ dr_status dr_hip_device_destroy(dr_device *device) {
 }
 
 static void freeScene(dr_scene *s) {
+    DR_DEVICE_LOCK;
     devFree(s->nodes);
     devFree(s->triPos);
     devFree(s->triangles);
@@ -339,6 +346,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
         for (uint32_t i = 0; i < n; ++i) std::memcpy(&alias[i].x, &prob[i], 4);
     }
 
+    DR_DEVICE_LOCK;
     hipStream_t up = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
     auto chain = [&](hipError_t x) {
@@ -461,6 +469,7 @@ dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t co
     S.watertight = intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
     const size_t lanes = (size_t(count) + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK * DR_TRACE_BLOCK;
     const size_t bytes = size_t(count) * 32 + size_t(count) * 8 + lanes * (scene->stackMax > DR_LDS_STACK ? scene->stackMax - DR_LDS_STACK : 0u) * 4;
+    DR_DEVICE_LOCK;
     char *buf = nullptr;
     DR_HIP(hipMalloc(&buf, bytes), "intersect buffers");
     float4 *dRays = reinterpret_cast<float4 *>(buf);
@@ -489,6 +498,7 @@ dr_status dr_hip_scene_info(const dr_scene *s, uint32_t *nodes, uint32_t *leaves
 }
 
 static void freeSession(dr_session *s) {
+    DR_DEVICE_LOCK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     dropGraphs(s);
     devFree(s->pathBlock);
@@ -606,6 +616,7 @@ dr_status dr_hip_session_create(dr_scene *scene, const dr_settings *st, const dr
     dr_status rc = sessionShape(scene, st, sh);
     if (rc != DR_OK) return rc;
     DR_HIP(hipSetDevice(scene->device->ordinal), "hipSetDevice");
+    DR_DEVICE_LOCK;
     auto *s = new dr_session;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&s->evBegin);
@@ -661,6 +672,23 @@ dr_status dr_hip_session_estimate_bytes(const dr_scene *scene, const dr_settings
     return DR_OK;
 }
 
+}  // extern "C"
+
+dr_status drSessionReserveUnet(dr_session *s) {
+    const size_t need = drUnetWorkBytes(s->settings.width, s->settings.height);
+    if (s->unetWorkBytes >= need) return DR_OK;
+    DR_DEVICE_LOCK;
+    DR_HIP(hipStreamSynchronize(s->stream), "stream synchronize");
+    devFree(s->unetWork);
+    s->unetWork = nullptr;
+    s->unetWorkBytes = 0;
+    DR_HIP(hipMalloc(&s->unetWork, need), "denoiser workspace");
+    s->unetWorkBytes = need;
+    return DR_OK;
+}
+
+extern "C" {
+
 dr_status dr_hip_session_destroy(dr_session *s) {
     if (!s) return DR_OK;
     freeSession(s);
@@ -701,6 +729,7 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
     auto run = [&](DrStage stage, uint32_t b) -> hipError_t {
         size_t ev = s->stageEvents.size();
         if (s->stageTiming) {
+            DR_DEVICE_LOCK;
             hipEvent_t a, z;
             if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&z) != hipSuccess) return hipErrorOutOfMemory;
             s->stageEvents.push_back(a);
@@ -782,6 +811,7 @@ static void collectStageTimes(dr_session *s) {
         }
         add(DR_STAGE_FINALIZE);
     }
+    DR_DEVICE_LOCK;
     for (auto ev : s->stageEvents) (void)hipEventDestroy(ev);
     s->stageEvents.clear();
 }
@@ -796,6 +826,7 @@ static dr_status batchGraph(dr_session *s, uint32_t k, hipGraphExec_t *out) {
         }
     DrLaunch L = makeLaunch(s, 0, k, 0, 0, s->settings.width, s->settings.height);
     L.sampleBase = s->sampleBase;
+    DR_DEVICE_LOCK;  // no other thread allocates or frees while this stream captures
     DR_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     dr_status r = enqueueBatch(s, L);
     hipError_t e = r == DR_OK ? dr::launchAdvanceSampleBase(s->sampleBase, k, s->stream) : hipSuccess;

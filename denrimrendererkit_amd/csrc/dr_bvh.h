@@ -17,17 +17,20 @@ struct BvhBuildStats {
 };
 
 // Builds a binned-SAH binary BVH over the triangles' world-space vertices
-// (reference GPUTriangle records), collapses it to a 4-wide BVH (each node
-// takes up to four children by repeatedly opening its largest-area inner
-// child) and writes the traversal layout:
+// (reference GPUTriangle records), collapses it to a 4-wide BVH (each wide
+// node takes the cut of its binary subtree into at most four children of
+// least SAH cost, a bottom-up dynamic program over (node, slots)) and
+// writes the traversal layout:
 //   nodes  -- DrNode4 per inner node, depth-first pre-order, root at 0;
 //   triPos -- 3 float4 per triangle in leaf order: {v0, bits(orig index)},
 //             {v1, 0}, {v2, 0} (exact vertices: the watertight test needs
 //             them; the Moller-Trumbore test forms v1 - v0, v2 - v0 itself,
 //             the same float subtractions as the reference).
-// Child boxes are widened outward by a few ulps of their magnitude so the
-// fused-multiply-add slab test can never cull a box that the exact test
-// would keep; that keeps the closest hit identical to a brute-force search.
+// Child boxes are widened outward by 1e-6 of their coordinates' magnitude
+// plus extent, which covers the fused-multiply-add slab test's error that
+// scales with the box (the part that scales with the ray origin is covered
+// per ray by the traversal's slab offsets), so no box that the exact test
+// keeps is culled and the closest hit equals a brute-force search.
 // Depth is capped at DR_MAX_DEPTH; stats->maxDepth is the 4-wide tree's
 // depth, and a traversal lane needs at most 7 stack entries per level
 // (three inner children and four leaf children) plus two. Leaves hold at

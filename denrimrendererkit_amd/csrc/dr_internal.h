@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -90,10 +91,23 @@ struct dr_session {
     size_t unetWorkBytes = 0;
 };
 
+// One lock orders graph capture (a batch graph captured on a session stream,
+// possibly on the render server's scheduler thread) against the calls that
+// synchronize the device or change its allocations on other threads:
+// hipMalloc / hipFree, stream, event and graph-exec creation and
+// destruction. It is recursive because the locked calls nest (a session
+// destroy drops its graphs). Held only around those calls and the capture
+// itself, never around kernel execution.
+std::recursive_mutex &drDeviceMutex();
+#define DR_DEVICE_LOCK std::lock_guard<std::recursive_mutex> drDeviceLock_(drDeviceMutex())
+
 // Sets the thread's dr_last_error() message and returns s.
 dr_status drFail(dr_status s, const std::string &msg);
 // Device bytes of the U-Net's activations for one width x height image.
 size_t drUnetWorkBytes(uint32_t width, uint32_t height);
 // Enqueues the U-Net on the session's beauty into session->denoised.
 dr_status drUnetDenoiseSession(dr_unet *net, dr_session *session, void **work, size_t *workBytes);
+// Allocates the session's U-Net workspace for its current size now (no-op
+// when it already holds enough), so that a later denoise allocates nothing.
+dr_status drSessionReserveUnet(dr_session *session);
 

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -28,7 +29,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/denrim_hip.h"
+#include "dr_internal.h"
 
 int drDeviceOrdinal(const dr_device *device);  // dr_api.cpp
 void drSceneRetain(dr_scene *scene);            // dr_api.cpp: scene references
@@ -80,6 +81,10 @@ struct dr_server {
     bool stop = false;
     std::thread worker;
     int ordinal = 0;
+    // Fault injection for tests of the failure path: when set (environment
+    // DENRIM_HIP_SERVER_FAIL_ENCODE = n > 0 at server creation), every n-th
+    // encode call is reported as failed without touching the device.
+    uint32_t failEncodeEvery = 0, encodeCalls = 0;
 
     void run();
     void admit();
@@ -118,9 +123,16 @@ bool dr_server::spillOne() {
 // Gives r a session: a pooled one that fits, else a new one within the
 // budget, making room by dropping pooled sessions and spilling finished
 // requests to the host. Returns false while there is no room. Called with mu.
+// A request with the learned denoiser takes only a session that already
+// holds a large enough U-Net workspace, and a new session for it gets that
+// workspace at once: the budget counts it (the request's bytes include it)
+// and the denoise on the scheduler thread then allocates nothing.
 bool dr_server::bind(Request *r) {
+    const bool learned = r->settings.denoiser == DR_DENOISER_LEARNED;
+    const size_t unetBytes = learned ? drUnetWorkBytes(r->settings.width, r->settings.height) : 0u;
     for (;;) {
         for (size_t i = 0; i < pool.size(); ++i) {
+            if (learned && pool[i].session->unetWorkBytes < unetBytes) continue;
             if (dr_hip_session_retarget(pool[i].session, r->scene, &r->settings, &r->camera, &r->camera) == DR_OK) {
                 r->shell = pool[i];
                 pool.erase(pool.begin() + long(i));
@@ -140,6 +152,12 @@ bool dr_server::bind(Request *r) {
     dr_session *s = nullptr;
     if (dr_hip_session_create(r->scene, &r->settings, &r->camera, &r->camera, &s) != DR_OK) {
         fail(r, dr_last_error());
+        return true;
+    }
+    if (learned && drSessionReserveUnet(s) != DR_OK) {
+        std::string err = dr_last_error();
+        dr_hip_session_destroy(s);
+        fail(r, err);
         return true;
     }
     r->shell = Shell{s, r->bytes};
@@ -183,13 +201,17 @@ void dr_server::run() {
             act = active;
         }
         bool progressed = false;
-        for (Request *r : act) {
+        for (Request *&r : act) {
             if (r->inflight == 0 && r->done < r->samples) {
                 uint32_t k = std::min(r->samples - r->done, cfg.samples_per_batch);
-                if (dr_hip_session_encode(r->shell.session, k) != DR_OK) {
+                const bool injected = failEncodeEvery && ++encodeCalls % failEncodeEvery == 0;
+                if (injected || dr_hip_session_encode(r->shell.session, k) != DR_OK) {
                     std::lock_guard<std::mutex> lk(mu);
                     active.erase(std::find(active.begin(), active.end(), r));
-                    fail(r, dr_last_error());
+                    fail(r, injected ? std::string("injected encode failure") : std::string(dr_last_error()));
+                    // Once FAILED is published a client may release (and
+                    // free) the request: drop it from this snapshot.
+                    r = nullptr;
                     progressed = true;
                     continue;
                 }
@@ -198,7 +220,7 @@ void dr_server::run() {
             }
         }
         for (Request *r : act) {
-            if ((r->inflight == 0 && !r->finishing) || r->state != DR_REQ_RUNNING) continue;
+            if (!r || (r->inflight == 0 && !r->finishing) || r->state != DR_REQ_RUNNING) continue;
             int done = 0;
             dr_status q = dr_hip_session_query(r->shell.session, &done);
             if (q == DR_OK && !done) continue;
@@ -245,6 +267,7 @@ dr_status dr_hip_server_create(dr_device *device, const dr_server_settings *sett
     s->cfg = *settings;
     if (s->cfg.samples_per_batch == 0) s->cfg.samples_per_batch = 0xffffffffu;
     s->ordinal = drDeviceOrdinal(device);
+    if (const char *f = std::getenv("DENRIM_HIP_SERVER_FAIL_ENCODE")) s->failEncodeEvery = uint32_t(std::strtoul(f, nullptr, 10));
     s->worker = std::thread([s] { s->run(); });
     *out = s;
     return DR_OK;

@@ -55,8 +55,24 @@ struct DrStack {
 
 struct DrRayPre {
     vec3 o, d;
-    vec3 inv;   // 1/d with zero components replaced by +-1e-20
-    vec3 oinv;  // o * inv
+    vec3 inv;  // 1/d with zero components replaced by +-1e-20
+    // Slab offsets: a plane at x is entered at fma(x, inv, c). cLo serves the
+    // boxes' low planes, cHi their high planes: -o * inv moved by e =
+    // 2^-21 |o * inv| toward the near side for the plane the ray enters first
+    // and toward the far side for the one it leaves by (per axis, from the
+    // sign of inv). The rounding of o * inv and the part of the fma's and
+    // 1/d's relative error that scales with |o| stay below e, so a ray far
+    // from the scene's origin never culls a box the exact test keeps; the
+    // part that scales with the box's own coordinates is covered by the
+    // builder's outward widening of every child box.
+    vec3 cLo, cHi;
+    // Box tests run against max(best t) * (1 + 2^-16) + slack, slack =
+    // 2^-14 max|o| / max|d|: a triangle's computed hit distance carries the
+    // rounding of o - v0 (about 2^-24 |o|, amplified by the triangle's shape
+    // and the ray's incidence), so a box must not be culled by a competitor
+    // whose computed t lies below the box's exact entry by that much. Keeping
+    // a few more boxes near the current best changes no result.
+    float slack;
 };
 
 __device__ __forceinline__ float safeInv(float d) {
@@ -69,19 +85,30 @@ __device__ __forceinline__ DrRayPre dr_prep(vec3 o, vec3 d) {
     r.o = o;
     r.d = d;
     r.inv = vec3{safeInv(d.x), safeInv(d.y), safeInv(d.z)};
-    r.oinv = vec3{o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z};
+    const vec3 oinv = vec3{o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z};
+    const float k = 4.76837158e-7f;  // 2^-21
+    // (|o * inv| is capped so that an infinite offset never meets its
+    // opposite and makes a NaN.)
+    const vec3 e = vec3{copysignf(k * fminf(fabsf(oinv.x), 1e37f), r.inv.x),
+                        copysignf(k * fminf(fabsf(oinv.y), 1e37f), r.inv.y),
+                        copysignf(k * fminf(fabsf(oinv.z), 1e37f), r.inv.z)};
+    r.cLo = vec3{-oinv.x - e.x, -oinv.y - e.y, -oinv.z - e.z};
+    r.cHi = vec3{-oinv.x + e.x, -oinv.y + e.y, -oinv.z + e.z};
+    const float om = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
+    const float dm = fmaxf(fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fabsf(d.z)), 1e-30f);
+    r.slack = 6.10351562e-5f * om / dm;  // 2^-14
     return r;
 }
 
 // Slab test of one box; returns entry distance or +inf on miss.
 __device__ __forceinline__ float dr_box(const DrRayPre &r, float x0, float y0, float z0, float x1, float y1, float z1,
                                         float tmax) {
-    float tx0 = __builtin_fmaf(x0, r.inv.x, -r.oinv.x);
-    float tx1 = __builtin_fmaf(x1, r.inv.x, -r.oinv.x);
-    float ty0 = __builtin_fmaf(y0, r.inv.y, -r.oinv.y);
-    float ty1 = __builtin_fmaf(y1, r.inv.y, -r.oinv.y);
-    float tz0 = __builtin_fmaf(z0, r.inv.z, -r.oinv.z);
-    float tz1 = __builtin_fmaf(z1, r.inv.z, -r.oinv.z);
+    float tx0 = __builtin_fmaf(x0, r.inv.x, r.cLo.x);
+    float tx1 = __builtin_fmaf(x1, r.inv.x, r.cHi.x);
+    float ty0 = __builtin_fmaf(y0, r.inv.y, r.cLo.y);
+    float ty1 = __builtin_fmaf(y1, r.inv.y, r.cHi.y);
+    float tz0 = __builtin_fmaf(z0, r.inv.z, r.cLo.z);
+    float tz1 = __builtin_fmaf(z1, r.inv.z, r.cHi.z);
     float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     return tn <= tf ? tn : __builtin_inff();
@@ -242,6 +269,7 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
     const DrNode4 n = S.nodes[ni];
     ++nv;
     const float inf = __builtin_inff();
+    tmax = __builtin_fmaf(tmax, 1.0000153f, r.slack);  // 1 + 2^-16
     float t0 = dr_box(r, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmax);
     float t1 = dr_box(r, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmax);
     float t2 = dr_box(r, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmax);

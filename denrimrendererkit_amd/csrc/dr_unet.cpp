@@ -122,6 +122,7 @@ void buildLayers(dr_unet *u) {
 }
 
 void freeWorkspace(Workspace &w) {
+    DR_DEVICE_LOCK;
     if (w.block) (void)hipFree(w.block);
     w = Workspace{};
 }
@@ -159,6 +160,7 @@ dr_status ensureWorkspace(dr_unet *u, Workspace &w, int n, int H, int W, bool gr
     if (w.block && bytes <= w.bytes) {
         // reuse the block
     } else {
+        DR_DEVICE_LOCK;
         (void)hipStreamSynchronize(u->stream);
         if (w.block) (void)hipFree(w.block);
         w.block = nullptr;
@@ -281,6 +283,7 @@ dr_status adam(dr_unet *u, float lr) {
 
 dr_status ensureLoss(dr_unet *u, uint32_t n) {
     if (n <= u->lossCap) return DR_OK;
+    DR_DEVICE_LOCK;
     (void)hipStreamSynchronize(u->stream);
     if (u->loss) (void)hipFree(u->loss);
     u->loss = nullptr;
@@ -317,6 +320,7 @@ dr_status drUnetDenoiseSession(dr_unet *u, dr_session *s, void **work, size_t *w
     std::vector<size_t> off(NT);
     const size_t bytes = inferLayout(imgW, imgH, off.data());
     if (*workBytes < bytes) {
+        DR_DEVICE_LOCK;
         (void)hipStreamSynchronize(s->stream);
         if (*work) (void)hipFree(*work);
         *work = nullptr;
@@ -349,6 +353,7 @@ dr_status dr_hip_unet_create(dr_device *device, uint32_t seed, dr_unet **out) {
     auto *u = new dr_unet;
     u->device = device;
     buildLayers(u);
+    DR_DEVICE_LOCK;
     hipError_t e = hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking);
     const size_t P = u->paramCount;
     if (e == hipSuccess) e = hipMalloc(&u->params, P * 4);
@@ -380,6 +385,7 @@ dr_status dr_hip_unet_create(dr_device *device, uint32_t seed, dr_unet **out) {
 
 dr_status dr_hip_unet_destroy(dr_unet *u) {
     if (!u) return DR_OK;
+    DR_DEVICE_LOCK;
     if (u->stream) (void)hipStreamSynchronize(u->stream);
     for (void *p : {static_cast<void *>(u->params), static_cast<void *>(u->grads), static_cast<void *>(u->m),
                     static_cast<void *>(u->v), static_cast<void *>(u->wf), static_cast<void *>(u->wt),
@@ -426,6 +432,7 @@ dr_status dr_hip_unet_set_batch(dr_unet *u, const float *features, const float *
     dr_status rc = ensureWorkspace(u, u->train, int(n), int(h), int(w), true);
     if (rc != DR_OK) return rc;
     const size_t px = size_t(n) * h * w;
+    DR_DEVICE_LOCK;
     float *tmp = nullptr;
     DR_UHIP(hipMalloc(&tmp, px * 16 * 4), "staging");
     hipError_t e = drCopy(tmp, features, px * 16 * 4, hipMemcpyHostToDevice, u->stream);
@@ -479,6 +486,7 @@ dr_status dr_hip_unet_train_step(dr_unet *u, float lr, float *loss) {
 dr_status dr_hip_unet_dataset_reserve(dr_unet *u, uint32_t capacity, uint32_t h, uint32_t w) {
     if (!u || capacity == 0 || h == 0 || w == 0 || h % 16 || w % 16)
         return ufail(DR_ERR_INVALID_ARGUMENT, "dataset size must be positive, h and w multiples of 16");
+    DR_DEVICE_LOCK;
     DR_UHIP(hipStreamSynchronize(u->stream), "stream synchronize");
     if (u->dsFeat) (void)hipFree(u->dsFeat);
     if (u->dsTarget) (void)hipFree(u->dsTarget);

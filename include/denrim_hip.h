@@ -164,7 +164,7 @@ typedef enum {
     DR_STAT_PRIMARY_QUERIES = 0,
     DR_STAT_BOUNCE_QUERIES = 1,
     DR_STAT_SHADOW_QUERIES = 2,
-    DR_STAT_NODE_VISITS = 3,          /* extension queries: node fetches (2 boxes each) */
+    DR_STAT_NODE_VISITS = 3,          /* extension queries: node fetches (4 child boxes each) */
     DR_STAT_TRIANGLE_TESTS = 4,
     DR_STAT_SHADOW_NODE_VISITS = 5,   /* NEE shadow queries */
     DR_STAT_SHADOW_TRIANGLE_TESTS = 6,

@@ -167,6 +167,40 @@ def test_ray_queries_match_oracle(renderer, mode):
         assert misses == 0
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ray_queries_far_origins(renderer, mode):
+    # Rays from about 1e3 x the scene's size away, aimed at every vertex and
+    # edge midpoint of a closed icosphere (radius 0.5, off the world origin,
+    # and at it): the slab test's rounding grows with the ray origin, which
+    # the per-ray slab offsets cover. Indices and t bit-exact against the
+    # oracle's brute force.
+    from denrimrendererkit_amd import Material, Mesh, RenderScene
+    sphere = synthetic.icosphere(3, 0.5)
+    v = np.asarray(sphere.vertices, np.float32)
+    tri = np.asarray(sphere.indices, np.int64).reshape(-1, 3)
+    edges = sorted({tuple(sorted((int(a), int(c)))) for t in tri for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))})
+    targets = np.concatenate([v, np.array([(v[a] + v[c]) * np.float32(0.5) for a, c in edges], np.float32)])
+    rng = np.random.default_rng(31)
+    for centre, dist in (((3.0, -2.0, 1.0), 1.0e3), ((0.0, 0.0, 0.0), 2.0e3), ((40.0, 10.0, -25.0), 5.0e3)):
+        s = RenderScene()
+        moved = Mesh(v + np.asarray(centre, np.float32), np.asarray(sphere.indices, np.uint32))
+        s.add(moved, s.add_material(Material()))
+        build = compile_scene(s)
+        tgt = targets + np.asarray(centre, np.float32)
+        u = rng.standard_normal((len(tgt), 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        rays = np.zeros((len(tgt), 8), np.float32)
+        rays[:, 0:3] = (tgt + dist * u).astype(np.float32)
+        rays[:, 4:7] = (tgt - rays[:, 0:3]).astype(np.float32)
+        t, ids = renderer.upload(build).intersect(rays, intersection=mode)
+        rt, rids = O.OracleScene(build).intersect(rays, brute_force=True, intersection=mode)
+        bad = np.nonzero(ids != rids)[0]
+        assert len(bad) == 0, (centre, [(int(i), rays[i].tolist(), int(ids[i]), int(rids[i]), float(t[i]), float(rt[i]))
+                                        for i in bad[:4]])
+        assert np.array_equal(t, rt), centre
+        assert (ids != 0xFFFFFFFF).mean() > 0.5
+
+
 @pytest.mark.parametrize("max_leaf", [0, 40])
 def test_ray_queries_coincident_triangles_and_big_leaves(renderer, max_leaf):
     # 600 copies of one triangle (every ray through it has 600 equal-t
@@ -307,3 +341,70 @@ def test_preview_and_interactive_kernels_parity(renderer, quality, scene_name):
     sess.render(2)
     assert np.array_equal(sess.pixels(RenderOutput.BEAUTY), first) and sess.sample_count == 3
     print(f"{scene_name} {quality.name}: channels outside tol {bad:.4%}")
+
+
+_LDS4_CHILD = r"""
+import sys, numpy as np
+sys.path[:0] = [sys.argv[1], sys.argv[1] + '/oracle', sys.argv[1] + '/tests']
+from denrimrendererkit_amd import DenrimRenderer, Material, RenderQuality, RenderScene, RenderSettings
+from denrimrendererkit_amd import compile_scene, synthetic
+from denrimrendererkit_amd.settings import RenderOutput
+import test_gpu_parity as T
+r = DenrimRenderer(0)
+out = {}
+room_scene = synthetic.cornell_room_scene()
+room = compile_scene(room_scene)
+rng = np.random.default_rng(21)
+rays = np.zeros((20000, 8), np.float32)
+rays[:, 0:3] = rng.uniform([-1.9, 0.1, -1.9], [1.9, 2.9, 1.9], (20000, 3))
+rays[:, 4:7] = rng.standard_normal((20000, 3))
+out['room_t'], out['room_ids'] = r.upload(room).intersect(rays)
+sphere, edge = T._edge_rays_icosphere()
+s = RenderScene()
+s.add(sphere, s.add_material(Material()))
+out['ico_t'], out['ico_ids'] = r.upload(compile_scene(s)).intersect(edge)
+st = RenderSettings(width=160, height=90, max_bounces=4, quality=RenderQuality.FINAL, frame_seed=0)
+sess = r.make_session(r.upload(room), st, camera=room_scene.camera)
+sess.render(2)
+out['beauty'] = sess.pixels(RenderOutput.BEAUTY)
+out['tri'] = sess.triangle_ids()
+np.savez(sys.argv[2], **out)
+print('lds4 child done', flush=True)
+"""
+
+
+def test_stack_spill_build_matches(renderer, tmp_path):
+    # The library built with 4 LDS stack entries per lane (libdenrim_hip_lds4.so)
+    # spills the traversal stack to its per-lane global area on every scene.
+    # Run in a child process (one library per process): its ray queries equal
+    # the oracle's brute force bit for bit, and its room render equals the
+    # default build's bit for bit.
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = str(tmp_path / "lds4.npz")
+    env = dict(os.environ, DENRIM_HIP_LIBRARY="libdenrim_hip_lds4.so")
+    r = subprocess.run([sys.executable, "-c", _LDS4_CHILD, root, path], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    got = np.load(path)
+    room_scene = synthetic.cornell_room_scene()
+    room = compile_scene(room_scene)
+    rng = np.random.default_rng(21)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, 0:3] = rng.uniform([-1.9, 0.1, -1.9], [1.9, 2.9, 1.9], (20000, 3))
+    rays[:, 4:7] = rng.standard_normal((20000, 3))
+    rt, rids = O.OracleScene(room).intersect(rays, brute_force=True)
+    assert np.array_equal(got["room_ids"], rids) and np.array_equal(got["room_t"], rt)
+    from denrimrendererkit_amd import Material, RenderScene
+    sphere, edge = _edge_rays_icosphere()
+    s = RenderScene()
+    s.add(sphere, s.add_material(Material()))
+    rt, rids = O.OracleScene(compile_scene(s)).intersect(edge, brute_force=True)
+    assert np.array_equal(got["ico_ids"], rids) and np.array_equal(got["ico_t"], rt)
+    st = RenderSettings(width=160, height=90, max_bounces=4, quality=RenderQuality.FINAL, frame_seed=0)
+    sess = renderer.make_session(renderer.upload(room), st, camera=room_scene.camera)
+    sess.render(2)
+    assert np.array_equal(got["tri"], sess.triangle_ids())
+    assert np.array_equal(got["beauty"], sess.pixels(RenderOutput.BEAUTY))

@@ -10,6 +10,10 @@
   * A retargeted session (another scene, size, settings and camera in the
     same device memory) renders exactly what a new session renders, and
     refuses settings that need more memory than it holds.
+  * Learned-denoiser requests' U-Net workspaces count against the budget
+    (server count and device free memory), mixed with plain requests.
+  * Injected encode failures: requests released the moment their wait
+    returns, failed or done, while the scheduler keeps serving.
 """
 import numpy as np
 import pytest
@@ -173,5 +177,100 @@ def test_denoised_requests_equal_alone_in_mix(renderer):
             alone.render(spp)
             assert np.array_equal(server.result(t), alone.pixels()), (t, d)
             server.release(t)
+    finally:
+        server.close()
+
+
+def _device_free_bytes():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_learned_and_plain_requests_stay_within_budget(renderer):
+    # The U-Net workspace of learned requests counts against the budget: a
+    # pooled session is handed to a learned request only when it already
+    # holds the workspace, a new one gets it at once. Alternating learned and
+    # plain requests never hold more device memory than the budget (the
+    # server's own count and the device's free memory agree), and the
+    # results equal sessions rendering them alone.
+    import ctypes as C
+    from denrimrendererkit_amd import _lib
+    from denrimrendererkit_amd.denoise import UNetDenoiser
+    from denrimrendererkit_amd.renderer import settings_struct
+    from denrimrendererkit_amd.settings import DenoiseSettings
+    net = UNetDenoiser(renderer, seed=5)
+    scene = synthetic.material_preview_scene(12, hdri=(64, 32), segments=16)
+    gs = renderer.upload(compile_scene(scene))
+
+    def st(W, H, den):
+        s = _settings(W, H, bounces=3)
+        if den:
+            s.denoise = DenoiseSettings.learned()
+        return s
+
+    reqs = [(64, 64, False), (96, 96, True), (64, 64, False), (80, 48, True), (96, 96, False), (64, 64, True),
+            (48, 48, False), (96, 96, True)]
+
+    def need(W, H, den):
+        b = C.c_uint64()
+        ss = settings_struct(st(W, H, den), 2)
+        _lib.check(_lib.load().dr_hip_session_estimate_bytes(gs._h, C.byref(ss), C.byref(b)), "estimate")
+        return b.value
+
+    budget = int(max(need(W, H, d) for W, H, d in reqs) * 1.7)
+    free0 = _device_free_bytes()
+    server = RenderServer(renderer, memory_budget_bytes=budget, max_active=4)
+    server.set_denoiser_network(net)
+    try:
+        tickets = [server.submit(gs, st(W, H, d), scene.camera, 2, samples_per_launch=2) for W, H, d in reqs]
+        peak_used, peak_dev = 0, 0
+        for t in tickets:
+            while server.state(t) not in ("done", "failed"):
+                peak_used = max(peak_used, server.usage()["used_bytes"])
+                peak_dev = max(peak_dev, free0 - _device_free_bytes())
+        peak_used = max(peak_used, server.usage()["used_bytes"])
+        peak_dev = max(peak_dev, free0 - _device_free_bytes())
+        assert peak_used <= budget, (peak_used, budget)
+        # stream, event and graph objects take a little device memory beside the sessions
+        assert peak_dev <= budget + (64 << 20), (peak_dev, budget)
+        for t, (W, H, d) in zip(tickets, reqs):
+            assert server.wait(t) == "done"
+            alone = renderer.make_session(gs, st(W, H, d), camera=scene.camera, samples_per_launch=2)
+            if d:
+                alone.set_denoiser_network(net)
+            alone.render(2)
+            assert np.array_equal(server.result(t), alone.pixels()), (t, W, H, d)
+            server.release(t)
+    finally:
+        server.close()
+
+
+def test_failed_encode_released_at_once(renderer, monkeypatch):
+    # Every 7th encode call fails (fault injection read at server creation).
+    # Each request is released as soon as its wait returns, failed or done;
+    # the scheduler must not touch a failed request after publishing it.
+    monkeypatch.setenv("DENRIM_HIP_SERVER_FAIL_ENCODE", "7")
+    scene = synthetic.material_preview_scene(13, hdri=(64, 32), segments=16)
+    gs = renderer.upload(compile_scene(scene))
+    server = RenderServer(renderer, memory_budget_bytes=1 << 30, max_active=4, samples_per_batch=1)
+    monkeypatch.delenv("DENRIM_HIP_SERVER_FAIL_ENCODE")
+    alone = _alone(renderer, gs, scene.camera, _settings(32, 32), 3)[0]
+    try:
+        tickets = [server.submit(gs, _settings(32, 32), scene.camera, 3) for _ in range(14)]
+        states = []
+        for t in tickets:
+            s = server.wait(t)
+            if s == "done":
+                assert np.array_equal(server.result(t), alone)
+            server.release(t)
+            states.append(s)
+        assert states.count("failed") >= 1 and states.count("done") >= 1, states
+        t = server.submit(gs, _settings(32, 32), scene.camera, 3)
+        s = server.wait(t)
+        server.release(t)
+        assert s in ("done", "failed")
     finally:
         server.close()
