@@ -33,6 +33,7 @@ EXPORTS = (
     "dr_hip_unet_get_grads", "dr_hip_unet_get_output", "dr_hip_unet_train_step", "dr_hip_unet_dataset_reserve",
     "dr_hip_unet_dataset_add", "dr_hip_unet_train_dataset", "dr_hip_server_set_denoiser_network",
     "dr_hip_scene_intersect",
+    "dr_hip_scene_intersect_device",
 )
 
 
@@ -172,6 +173,7 @@ def load():
     L.dr_hip_unet_train_dataset.argtypes = [vp, u32, u32, f32, f32, u32, vp]
     L.dr_hip_server_set_denoiser_network.argtypes = [vp, vp]
     L.dr_hip_scene_intersect.argtypes = [vp, vp, u32, u32, vp, vp]
+    L.dr_hip_scene_intersect_device.argtypes = [vp, vp, u32, u32, vp, vp, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name != "dr_last_error":

@@ -135,10 +135,13 @@ dr_status dr_hip_device_create(int ordinal, dr_device **out) {
     std::string arch = prop.gcnArchName;
     if (arch.rfind("gfx950", 0) != 0)
         return fail(DR_ERR_MISSING_SHADER_FUNCTION, "kernels are built for gfx950, device is " + arch);
+    uint32_t *scratch = nullptr;
+    DR_HIP(dr::intersectScratchAddress(&scratch), "traversal scratch address");
     auto *d = new dr_device;
     d->ordinal = ordinal;
     d->cuCount = prop.multiProcessorCount;
     d->arch = arch;
+    d->intersectScratch = scratch;
     *out = d;
     return DR_OK;
 }
@@ -420,6 +423,47 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
 }
 
 }  // extern "C"
+
+namespace {
+// True when p is device (or managed) memory of `ordinal`.
+bool onDevice(const void *p, int ordinal) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unknown host pointer: clear the sticky error
+        return false;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == ordinal;
+}
+}  // namespace
+
+extern "C" dr_status dr_hip_scene_intersect_device(dr_scene *scene, const float *rays, uint32_t count,
+                                                   uint32_t intersection, float *t, uint32_t *ids, void *stream) {
+    if (!scene || (count && (!rays || !t || !ids))) return fail(DR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (intersection > DR_INTERSECTION_WATERTIGHT) return fail(DR_ERR_INVALID_ARGUMENT, "unknown intersection test");
+    if (count == 0) return DR_OK;
+    const dr_device *dev = scene->device;
+    DR_HIP(hipSetDevice(dev->ordinal), "hipSetDevice");
+    if (!onDevice(rays, dev->ordinal) || !onDevice(t, dev->ordinal) || !onDevice(ids, dev->ordinal))
+        return fail(DR_ERR_INVALID_ARGUMENT, "rays, t and ids must be device memory of the scene's device");
+    // Each launch holds as many lanes as the scratch spills for at the
+    // scene's stack depth; a depth whose spill for one block of lanes
+    // exceeds the scratch is refused.
+    const uint32_t spill = scene->stackMax > DR_LDS_STACK ? scene->stackMax - DR_LDS_STACK : 0u;
+    uint32_t lanes = 0;
+    if (spill) {
+        lanes = DR_INTERSECT_SCRATCH_WORDS / spill / DR_TRACE_BLOCK * DR_TRACE_BLOCK;
+        if (lanes == 0)
+            return fail(DR_ERR_INVALID_ARGUMENT,
+                        "scene BVH too deep for the traversal scratch: " + std::to_string(size_t(spill) * 4 * DR_TRACE_BLOCK) +
+                            " B per block of rays, " + std::to_string(size_t(DR_INTERSECT_SCRATCH_WORDS) * 4) + " B available");
+    }
+    DrSceneDev S = scene->dev;
+    S.watertight = intersection == DR_INTERSECTION_WATERTIGHT ? 1u : 0u;
+    DR_HIP(dr::launchIntersect(S, reinterpret_cast<const float4 *>(rays), count, t, ids, dev->intersectScratch,
+                               static_cast<hipStream_t>(stream), lanes),
+           "intersect launch");
+    return DR_OK;
+}
 
 // Scene lifetime: sessions (and requests queued on a render server) hold a
 // reference; a released scene is freed when the last holder lets go. The

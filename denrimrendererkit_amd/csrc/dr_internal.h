@@ -27,6 +27,9 @@ struct dr_device {
     int ordinal = 0;
     int cuCount = 0;
     std::string arch;
+    // Device address of this device's static traversal-stack scratch
+    // (dr_hip_scene_intersect_device), taken once at device creation.
+    uint32_t *intersectScratch = nullptr;
 };
 
 struct dr_scene {

@@ -13,6 +13,8 @@
 #define DR_SHADE_BLOCK 128
 #endif
 #define DR_MAX_BOUNCES 64
+// Size of that scratch: 16 MiB per device.
+#define DR_INTERSECT_SCRATCH_WORDS (4u << 20)
 // Traversal stack entries kept in LDS per lane; deeper entries spill to a
 // per-lane global area of (stackMax - DR_LDS_STACK) words, stackMax being
 // the scene's bound of 7 entries per BVH level plus two.
@@ -94,7 +96,12 @@ hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st);
 // Closest triangle hit of `count` rays (2 float4 each: origin, direction):
 // t (+inf on a miss) and the original triangle index (~0 on a miss);
 // overflow holds the traversal-stack spill of ceil(count / 128) * 128 lanes.
+// maxLanes > 0 splits the rays into launches of at most maxLanes lanes (a
+// multiple of the block size) that share one overflow area of maxLanes lanes.
 hipError_t launchIntersect(const DrSceneDev &S, const float4 *rays, uint32_t count, float *t, uint32_t *ids,
-                           uint32_t *overflow, hipStream_t st);
+                           uint32_t *overflow, hipStream_t st, uint32_t maxLanes = 0);
+// Device address of the static traversal-stack scratch of the allocation-free
+// ray queries (DR_INTERSECT_SCRATCH_WORDS words); call with the device current.
+hipError_t intersectScratchAddress(uint32_t **ptr);
 hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t st);
 }

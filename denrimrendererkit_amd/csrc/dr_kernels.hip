@@ -27,6 +27,8 @@
 // coalesced). The per-path RNG stream is keyed by (frameSeed, pixel,
 // sampleIndex) only, so a path's result does not depend on which slot,
 // queue position or batch it ran in.
+#include <algorithm>
+
 #include "dr_kernels.h"
 #include "dr_shade.h"
 #include "dr_traverse.h"
@@ -1580,12 +1582,16 @@ namespace dr {
 
 // One ray per lane through the same while-while traversal and triangle
 // tests as the path tracer's extension queries (triangle geometry only).
+// The launch covers rays [first, first + lanes); lane k's stack spill is
+// overflow[k + j * lanes].
 __global__ __launch_bounds__(DR_TRACE_BLOCK) void k_intersect(DrSceneDev S, const float4 *__restrict__ rays,
-                                                              uint32_t count, float *__restrict__ tOut,
+                                                              uint32_t first, uint32_t count,
+                                                              float *__restrict__ tOut,
                                                               uint32_t *__restrict__ idOut, uint32_t *overflow) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
-    const uint32_t i = blockIdx.x * DR_TRACE_BLOCK + threadIdx.x;
-    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, overflow + i, gridDim.x * DR_TRACE_BLOCK};
+    const uint32_t lane = blockIdx.x * DR_TRACE_BLOCK + threadIdx.x;
+    const uint32_t i = first + lane;
+    DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, overflow + lane, gridDim.x * DR_TRACE_BLOCK};
     const bool live = i < count;
     // Lanes past the end trace a ray that misses every box.
     const float4 o = live ? rays[2 * i] : make_float4(3e38f, 3e38f, 3e38f, 0.f);
@@ -1601,10 +1607,27 @@ __global__ __launch_bounds__(DR_TRACE_BLOCK) void k_intersect(DrSceneDev S, cons
 }
 
 hipError_t launchIntersect(const DrSceneDev &S, const float4 *rays, uint32_t count, float *t, uint32_t *ids,
-                           uint32_t *overflow, hipStream_t st) {
-    const uint32_t blocks = (count + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK;
-    if (blocks) hipLaunchKernelGGL(k_intersect, dim3(blocks), dim3(DR_TRACE_BLOCK), 0, st, S, rays, count, t, ids, overflow);
-    return hipGetLastError();
+                           uint32_t *overflow, hipStream_t st, uint32_t maxLanes) {
+    const uint32_t chunk = maxLanes ? maxLanes : (count + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK * DR_TRACE_BLOCK;
+    for (uint32_t first = 0; first < count; first += chunk) {
+        const uint32_t n = std::min(chunk, count - first);
+        const uint32_t blocks = (n + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK;
+        hipLaunchKernelGGL(k_intersect, dim3(blocks), dim3(DR_TRACE_BLOCK), 0, st, S, rays, first, count, t, ids,
+                           overflow);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Traversal-stack spill of the allocation-free ray queries
+// (dr_hip_scene_intersect_device): a fixed array each device holds until the
+// process ends, addressed through hipGetSymbolAddress once per device (the
+// array's name in host code is not its device address).
+static __device__ __align__(256) uint32_t scratch[DR_INTERSECT_SCRATCH_WORDS];
+
+hipError_t intersectScratchAddress(uint32_t **ptr) {
+    return hipGetSymbolAddress(reinterpret_cast<void **>(ptr), HIP_SYMBOL(scratch));
 }
 
 hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st) {

@@ -145,6 +145,15 @@ class GpuScene:
                                                           _ptr(ids)), "intersect")
         return t, ids
 
+    def intersect_device(self, rays_ptr: int, count: int, t_ptr: int, ids_ptr: int, stream: int = 0,
+                         intersection: int = 0) -> None:
+        """The same queries on device buffers (addresses of count x 8 floats,
+        count floats, count u32), enqueued on `stream` with no allocation
+        (dr_hip_scene_intersect_device); the caller synchronizes."""
+        _lib.check(_lib.load().dr_hip_scene_intersect_device(self._h, C.c_void_p(rays_ptr), count, int(intersection),
+                                                             C.c_void_p(t_ptr), C.c_void_p(ids_ptr),
+                                                             C.c_void_p(stream or None)), "intersect_device")
+
     def release(self):
         if getattr(self, "_h", None):
             _lib.load().dr_hip_scene_release(self._h)

@@ -316,6 +316,20 @@ dr_status dr_hip_session_reset_stats(dr_session *session);
 dr_status dr_hip_scene_intersect(dr_scene *scene, const float *rays, uint32_t count, uint32_t intersection, float *t,
                                  uint32_t *ids);
 
+/* The same ray queries on caller device buffers, allocating nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). rays (count x 8 floats), t and ids are
+ * device memory of the scene's device (else DR_ERR_INVALID_ARGUMENT); the
+ * queries are enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting. The traversal stack spills into a
+ * static 16 MiB array per device, held until the process ends: the rays are
+ * traced in launches of as many lanes as that array holds at the scene's BVH
+ * depth, and a scene whose spill for one block of 128 rays exceeds it is
+ * refused. Calls that overlap on one device race on that array. */
+dr_status dr_hip_scene_intersect_device(dr_scene *scene, const float *rays, uint32_t count, uint32_t intersection,
+                                        float *t, uint32_t *ids, void *stream);
+
 /* Rebinds an existing session to another scene, settings and camera,
  * reusing its device memory (no allocation): the session then behaves like
  * a newly created one. Fails with DR_ERR_OUT_OF_MEMORY when the new
