@@ -1,16 +1,19 @@
 """Throughput benchmark of the HIP progressive path tracer on one MI355X.
 
-A step is one progressive sample over the whole frame of the workload (for
-the default 1080p layered-terrain workload: 1920x1080 paths, max depth 8).
-The scene, its BVH and the path-state buffers are resident in HBM before the
-timed region; the timed region is K steps enqueued back to back on the
-session stream, bracketed by a device synchronize on both sides and timed
-both by the host clock and by HIP events on that stream.
+A step is one frame of the workload at its configured sample count (for the
+default 1080p layered-terrain workload: 1920x1080 pixels x 64 spp, max depth
+8), rendered one progressive sample at a time on the session stream. The
+scene, its BVH and the path-state buffers are resident in HBM before the
+timed region; the timed region is K steps enqueued back to back, bracketed by
+a device synchronize on both sides and timed both by the host clock and by
+HIP events on that stream.
 
-Printed: one JSON line with Mrays/s (every scene query: extension, NEE
-shadow and subsurface-walk queries) as `value`, pixel-samples/s, the
-roofline of the dominant kernel and a CPU baseline (the oracle on the
-host's cores over a bounded sample of the same frame).
+Printed: one JSON line with Mrays/s as `value` (every scene query of the
+timed samples -- extension, NEE shadow-chain steps, subsurface-walk segments
+-- from the device counter every kernel variant keeps), pixel-samples/s, the
+roofline of the dominant kernel, the sparse-SDF and room workloads as
+sub-objects, the serving stream, the denoiser, and a CPU baseline (the
+oracle on the host's cores over a bounded sample of the same frame).
 """
 import argparse
 import json
@@ -25,16 +28,18 @@ sys.path.insert(0, HERE)
 
 METRIC = "Mrays/s & spp/s at 1080p; preview p50/p99 latency at equal relMSE"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per unit: a node fetch tests four child boxes (128 B), a
-# triangle test reads 36 B of positions, each query reads its 32-B ray and
-# writes a 16-B hit.
-NODE_BYTES, TRI_BYTES, RAY_BYTES = 128, 36, 48
-# Sparse-volume units: a DDA cell reads its 4-B brick index and 4-B macro
-# word; a brick test reads the 64-B brick header; a march step reads the 8
-# FP16 corner samples (16 B); a hit adds 8 bisection lookups (8 x 16 B), two
-# cubic polishes and a cubic normal (3 x 64 FP16 samples) and the 8-B
-# material pair: 128 + 384 + 8 = 520 B.
-CELL_BYTES, BRICK_BYTES, STEP_BYTES, HIT_BYTES = 8, 64, 16, 520
+# Algorithmic bytes per unit of work, per ray query (the measurement plan):
+# 32 B per node visit (a compact node), 36 B per triangle test (positions
+# only), 48 B per query (32-B ray in, 16-B hit out). A visit of this build's
+# four-wide node reads 128 B; that figure is reported beside the headline as
+# `frac_node128`.
+NODE_BYTES, NODE4_BYTES, TRI_BYTES, RAY_BYTES = 32, 128, 36, 48
+# Sparse-SDF queries: 2 B per FP16 distance sample fetched (8 per march step;
+# 8 x 8 bisection + 3 x 64 cubic polish/normal samples per hit), 4 B per
+# brick-grid cell visited, 128 B per brick entered.
+SAMPLE_BYTES, CELL_BYTES, BRICK_BYTES = 2, 4, 128
+SAMPLES_PER_STEP, SAMPLES_PER_HIT = 8, 8 * 8 + 3 * 64
+STAT_RAY_QUERIES = 36
 
 
 def log(msg):
@@ -63,17 +68,17 @@ def workload(name):
         return synthetic.layered_terrain_scene(), 1920, 1080, 8, {
             "workload": "layered-terrain 1920x1080, 6,168,640 tris (1M fBm terrain + 256 x 20k displaced spheres), "
                         "6 layered material kinds, 64 emissive tris, 2048x1024 HDRI, max depth 8",
-            "width": 1920, "height": 1080, "max_bounces": 8, "triangles": 6168640}
+            "width": 1920, "height": 1080, "max_bounces": 8, "triangles": 6168640, "spp": 64}
     if name == "sdf_1024":
         return synthetic.sdf_bricks_scene(resolution=768, primitives=200, hdri=(2048, 1024)), 1024, 1024, 4, {
             "workload": "sparse SDF bricks 1024x1024: virtual 768^3 grid of 8^3 bricks (1-voxel apron, FP16 "
                         "distances), smooth-min union (k=0.05) of 200 spheres and tori, metal/Lambertian, "
                         "2048x1024 HDRI, ground quad, max depth 4",
-            "width": 1024, "height": 1024, "max_bounces": 4}
+            "width": 1024, "height": 1024, "max_bounces": 4, "spp": 16}
     if name == "room_320x180":
         return synthetic.cornell_room_scene(), 320, 180, 4, {
             "workload": "room 320x180, 8,334 tris, Lambertian + GGX metal, 1 area light, max depth 4",
-            "width": 320, "height": 180, "max_bounces": 4, "triangles": 8334}
+            "width": 320, "height": 180, "max_bounces": 4, "triangles": 8334, "spp": 1}
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -300,18 +305,129 @@ def cpu_baseline(build, camera, W, H, bounces, target_seconds):
             "spp_per_s": round(rows * W / dt, 1)}
 
 
+def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, isect_mode=None):
+    """Renders `warmup` then `steps` frames of workload `name` (spp_per_step
+    progressive samples each, default: the workload's configured spp) and
+    returns the measurement dict: timed ray queries from the device counter,
+    the dominant kernel's roofline from a counting sample's node, triangle
+    and SDF counters, and the scene objects for the CPU baseline."""
+    from denrimrendererkit_amd import RenderQuality, RenderSettings, compile_scene
+    from denrimrendererkit_amd.settings import EnvironmentSampling, TriangleIntersection
+    env_mode = env_mode or EnvironmentSampling.CDF
+    isect_mode = isect_mode or TriangleIntersection.MOLLER_TRUMBORE
+    t0 = time.perf_counter()
+    scene, W, H, bounces, config = workload(name)
+    spp = spp_per_step or config["spp"]
+    build = compile_scene(scene)
+    log(f"{name}: scene compiled: {len(build.triangles)} tris, {len(build.volume_bricks)} bricks in "
+        f"{time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    gscene = renderer.upload(build)
+    info = gscene.info()
+    log(f"{name}: uploaded + BVH: {info} in {time.perf_counter() - t1:.1f} s")
+
+    def settings(counting):
+        return RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
+                              collects_sdf_traversal_stats=counting, intersection=isect_mode,
+                              environment_sampling=env_mode)
+
+    # Counting sample: node fetches, triangle tests and SDF work per sample.
+    cs = renderer.make_session(gscene, settings(True), camera=scene.camera, samples_per_launch=1)
+    cs.render(1)
+    c = cs.stats().astype(np.float64)
+    cs.close()
+    assert c[STAT_RAY_QUERIES] == c[0] + c[1] + c[2], "query counter disagrees with the counting kernels"
+    sdf_ext, sdf_sh = c[8:18], c[18:28]
+    counted_queries = c[0] + c[1] + c[2]
+
+    sess = renderer.make_session(gscene, settings(False), camera=scene.camera, samples_per_launch=1)
+    sess.encode_next_sample(warmup * spp)
+    sess.synchronize()
+    sess.reset_stats()
+    sess.set_stage_timing(True)
+    sess.synchronize()
+    w0 = time.perf_counter()
+    sess.encode_next_sample(steps * spp)
+    sess.synchronize()
+    wall = time.perf_counter() - w0
+    tm = sess.timing()
+    queries = float(sess.stats()[STAT_RAY_QUERIES])
+    gpu_s = tm["total_ms"] / 1e3
+    sec = max(wall, gpu_s)
+    samples = steps * spp
+    r0 = time.perf_counter()
+    beauty = sess.pixels()  # one beauty plane over PCIe: what a caller reads after the K frames
+    readback_s = time.perf_counter() - r0
+    assert np.isfinite(beauty).all(), "non-finite beauty"
+    sess.close()
+
+    # Roofline of the dominant kernel (largest share of stage time). On
+    # triangle scenes k_sss also traces the shadow rays queued by k_shade
+    # (counters 30-32) and the next bounce's k_extend those of resumed paths
+    # (33-35); k_shadow traces the last bounce's rest.
+    st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
+    dom = max(("extend", "shadow", "shade", "sss"), key=lambda k: st_ms[k])
+    if dom == "shadow":
+        nodes, tris, rays, sdf = c[5] - c[31] - c[34], c[6] - c[32] - c[35], c[2] - c[30] - c[33], sdf_sh
+    elif dom == "sss":
+        nodes, tris, rays, sdf = c[28] + c[31], c[29] + c[32], c[7] + c[30], sdf_ext * 0.0
+    elif dom == "extend":
+        nodes, tris, rays, sdf = c[3] + c[34], c[4] + c[35], c[0] + c[1] - c[7] + c[33], sdf_ext
+    else:
+        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
+    launches_per_sample = max(1, st_n[dom] // max(1, samples))
+    sdf_bytes = (SAMPLE_BYTES * (SAMPLES_PER_STEP * sdf[8] + SAMPLES_PER_HIT * sdf[9]) + CELL_BYTES * sdf[2] +
+                 BRICK_BYTES * sdf[4])
+    tri_bytes = TRI_BYTES * tris + RAY_BYTES * rays
+    bytes_per_launch = (NODE_BYTES * nodes + tri_bytes + sdf_bytes) / launches_per_sample
+    bytes4_per_launch = (NODE4_BYTES * nodes + tri_bytes + sdf_bytes) / launches_per_sample
+    avg_ms = st_ms[dom] / max(1, st_n[dom])
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    kname = {"extend": "k_extend", "shadow": "k_shadow", "shade": "k_shade", "sss": "k_sss"}[dom]
+    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(name, kname),
+                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
+                "frac_node128": round(bytes4_per_launch / (avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "bytes_per_launch_node128": int(bytes4_per_launch),
+                "bytes_note": "32 B per node visit + 36 B per triangle test + 48 B per query (+ SDF: 2 B per "
+                              "distance sample, 4 B per grid cell, 128 B per brick); frac_node128 charges the "
+                              "128 B a four-wide node visit reads",
+                "stage_ms_per_sample": {k: round(v / samples, 3) for k, v in st_ms.items()}}
+    config.update({"spp_per_step": spp, "frame_seed": 0, "bvh_nodes": info["nodes"],
+                   "ray_queries_per_sample": int(round(queries / samples)),
+                   "counting_sample_queries": int(counted_queries),
+                   "rays_per_path": round(queries / samples / (W * H), 3)})
+    if len(build.volume_bricks):
+        tot = sdf_ext + sdf_sh
+        config.update({"bricks": int(len(build.volume_bricks)), "brick_samples": int(len(build.volume_brick_samples)),
+                       "sdf_per_sample": {"grid_cells": int(tot[2]), "macro_skips": int(tot[3]),
+                                          "brick_tests": int(tot[4]), "marches": int(tot[7]),
+                                          "march_steps": int(tot[8]), "hits": int(tot[9])}})
+    return {"value": round(queries / sec / 1e6, 2), "unit": "Mrays/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(sec * 1e3 / steps, 3), "spp_per_s": round(W * H * samples / sec, 1),
+            "wall_s": round(wall, 4), "gpu_event_s": round(gpu_s, 4), "readback_ms": round(readback_s * 1e3, 3),
+            "pcie_inclusive_mrays_per_s": round(queries / (sec + readback_s) / 1e6, 2),
+            "config": config, "roofline": roofline,
+            "_scene": (build, scene.camera, W, H, bounces)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="terrain_1080p")
+    ap.add_argument("--spp-per-step", type=int, default=0,
+                    help="progressive samples per step (0: the workload's configured spp, 64 for the terrain)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--environment-sampling", choices=("cdf", "alias"), default="cdf",
                     help="HDRI texel selection of the timed render (the reference's is a CDF search)")
     ap.add_argument("--intersection", choices=("moller-trumbore", "watertight"), default="moller-trumbore",
                     help="triangle test of the timed render (the reference's is Moller-Trumbore)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sub-workloads", action="store_true",
+                    help="skip the sparse-SDF and room workloads measured beside the main one")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the material-preview serving run (0: skip)")
     ap.add_argument("--denoiser-frames", type=int, default=10,
@@ -336,119 +452,48 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from denrimrendererkit_amd import DenrimRenderer, RenderQuality, RenderSettings, compile_scene
-
-    t0 = time.perf_counter()
-    scene, W, H, bounces, config = workload(args.workload)
-    build = compile_scene(scene)
-    log(f"scene compiled: {len(build.triangles)} tris, {len(build.volume_bricks)} bricks in "
-        f"{time.perf_counter() - t0:.1f} s")
+    from denrimrendererkit_amd import DenrimRenderer
+    from denrimrendererkit_amd.settings import EnvironmentSampling, TriangleIntersection
     renderer = DenrimRenderer(0)
-    t1 = time.perf_counter()
-    gscene = renderer.upload(build)
-    info = gscene.info()
-    log(f"uploaded + BVH: {info} in {time.perf_counter() - t1:.1f} s")
-    from denrimrendererkit_amd.settings import TriangleIntersection
-    mode = TriangleIntersection.WATERTIGHT if args.intersection == "watertight" else TriangleIntersection.MOLLER_TRUMBORE
-    from denrimrendererkit_amd.settings import EnvironmentSampling
-    env_mode = EnvironmentSampling.ALIAS if args.environment_sampling == "alias" else EnvironmentSampling.CDF
-    settings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
-                              intersection=mode, environment_sampling=env_mode)
+    isect = TriangleIntersection.WATERTIGHT if args.intersection == "watertight" else TriangleIntersection.MOLLER_TRUMBORE
+    env = EnvironmentSampling.ALIAS if args.environment_sampling == "alias" else EnvironmentSampling.CDF
+    main_run = measure(renderer, args.workload, args.steps, args.warmup, args.spp_per_step or None, env, isect)
+    main_run["config"].update({"intersection": args.intersection, "environment_sampling": args.environment_sampling})
+    log(f"{args.workload}: {main_run['value']} Mrays/s, {main_run['ms_per_step']} ms per step")
 
-    # Counting pass: ray queries, node fetches and triangle tests per step.
-    csettings = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=0,
-                               collects_sdf_traversal_stats=True, intersection=mode, environment_sampling=env_mode)
-    cs = renderer.make_session(gscene, csettings, camera=scene.camera, samples_per_launch=1)
-    cs.render(1)
-    c = cs.stats().astype(np.float64)
-    cs.close()
-    # SDF counters in the reference's slot order: dense tests, dense steps, cells, macro skips, brick tests,
-    # invalid, culls, marches, march steps, hits; extension/subsurface queries, then shadow queries.
-    sdf_ext, sdf_sh = c[8:18], c[18:28]
-    queries_per_step = c[0] + c[1] + c[2]
-    log(f"per step: queries {queries_per_step:.0f} (primary {c[0]:.0f}, bounce {c[1]:.0f} incl. subsurface {c[7]:.0f}, "
-        f"shadow {c[2]:.0f}: {c[30]:.0f} beside the walks, {c[33]:.0f} beside the next extension), "
-        f"node fetches {c[3]:.0f} + shadow {c[5]:.0f} + subsurface {c[28]:.0f}, "
-        f"tri tests {c[4]:.0f} + shadow {c[6]:.0f} + subsurface {c[29]:.0f}")
-
-    sess = renderer.make_session(gscene, settings, camera=scene.camera, samples_per_launch=1)
-    sess.encode_next_sample(args.warmup)
-    sess.synchronize()
-    sess.reset_stats()
-    sess.set_stage_timing(True)
-    sess.synchronize()
-    w0 = time.perf_counter()
-    sess.encode_next_sample(args.steps)
-    sess.synchronize()
-    wall = time.perf_counter() - w0
-    tm = sess.timing()
-    gpu_s = tm["total_ms"] / 1e3
-    sec = max(wall, gpu_s)
-    mrays = queries_per_step * args.steps / sec / 1e6
-    spp_s = W * H * args.steps / sec
-    r0 = time.perf_counter()
-    beauty = sess.pixels()  # one beauty plane over PCIe: what a caller reads after the K samples
-    readback_s = time.perf_counter() - r0
-    assert np.isfinite(beauty).all(), "non-finite beauty"
-
-    # Roofline of the dominant kernel (largest share of stage time).
-    st_ms, st_n = tm["stage_ms"], tm["stage_launches"]
-    dom = max(("extend", "shadow", "shade", "sss"), key=lambda k: st_ms[k])
-    # On triangle scenes k_sss also traces the shadow rays queued by k_shade
-    # (counters 30-32) and the next bounce's k_extend those of resumed paths
-    # (33-35); k_shadow traces the last bounce's rest.
-    if dom == "shadow":
-        nodes, tris, rays, sdf = c[5] - c[31] - c[34], c[6] - c[32] - c[35], c[2] - c[30] - c[33], sdf_sh
-    elif dom == "sss":  # subsurface-walk queries + the shadow rays traced beside them
-        nodes, tris, rays, sdf = c[28] + c[31], c[29] + c[32], c[7] + c[30], sdf_ext * 0.0
-    elif dom == "extend":  # extension queries + the previous bounce's remaining shadow rays
-        nodes, tris, rays, sdf = c[3] + c[34], c[4] + c[35], c[0] + c[1] - c[7] + c[33], sdf_ext
-    else:
-        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
-    launches_per_step = max(1, st_n[dom] // max(1, args.steps))
-    sdf_bytes = CELL_BYTES * sdf[2] + BRICK_BYTES * sdf[4] + STEP_BYTES * sdf[8] + HIT_BYTES * sdf[9]
-    bytes_per_launch = (NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays + sdf_bytes) / launches_per_step
-    avg_ms = st_ms[dom] / max(1, st_n[dom])
-    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-    kname = {"extend": "k_extend", "shadow": "k_shadow", "shade": "k_shade", "sss": "k_sss"}[dom]
-    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.workload, kname),
-                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
-                "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in st_ms.items()}}
+    subs = {}
+    if not args.no_sub_workloads:
+        for name, steps in (("sdf_1024", 2), ("room_320x180", 64)):
+            if name == args.workload:
+                continue
+            r = measure(renderer, name, steps, 1)
+            r.pop("_scene")
+            subs[name] = {k: r[k] for k in ("value", "unit", "steps", "ms_per_step", "spp_per_s", "config", "roofline")}
+            log(f"{name}: {r['value']} Mrays/s, {r['ms_per_step']} ms per step")
 
     serving = None
     if args.serving_requests > 0:
-        sess.close()
         serving = serving_bench(renderer, args.serving_requests)
         log(f"serving: {serving}")
 
     denoiser = None
     if args.denoiser_frames > 0:
-        sess.close()
         denoiser = denoiser_bench(renderer, args.denoiser_frames)
         log(f"denoiser: {denoiser}")
 
+    build, camera, W, H, bounces = main_run.pop("_scene")
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(build, scene.camera, W, H, bounces, args.cpu_seconds)
+        cpu = cpu_baseline(build, camera, W, H, bounces, args.cpu_seconds)
 
-    config.update({"spp_per_step": 1, "frame_seed": 0, "bvh_nodes": info["nodes"], "intersection": args.intersection,
-                   "environment_sampling": args.environment_sampling,
-                   "ray_queries_per_step": int(queries_per_step), "rays_per_path": round(queries_per_step / (W * H), 3)})
-    if len(build.volume_bricks):
-        tot = sdf_ext + sdf_sh
-        config.update({"bricks": int(len(build.volume_bricks)), "brick_samples": int(len(build.volume_brick_samples)),
-                       "sdf_per_step": {"grid_cells": int(tot[2]), "macro_skips": int(tot[3]),
-                                        "brick_tests": int(tot[4]), "marches": int(tot[7]),
-                                        "march_steps": int(tot[8]), "hits": int(tot[9])}})
-    out = {"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "spp_per_s": round(spp_s, 1), "wall_s": round(wall, 4), "gpu_event_s": round(gpu_s, 4),
-           "readback_ms": round(readback_s * 1e3, 3),
-           "pcie_inclusive_mrays_per_s": round(queries_per_step * args.steps / (sec + readback_s) / 1e6, 2),
-           "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    out = {"metric": METRIC, "value": main_run["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": main_run["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    out.update({k: main_run[k] for k in ("spp_per_s", "wall_s", "gpu_event_s", "readback_ms",
+                                         "pcie_inclusive_mrays_per_s", "config", "roofline")})
+    out["cpu_baseline"] = cpu
+    if subs:
+        out["workloads"] = subs
     if denoiser is not None:
         out["denoiser"] = denoiser
     if serving is not None:

@@ -81,6 +81,13 @@ __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t *counter) {
     return base + below;
 }
 
+// Adds the wave's total of v to a 64-bit counter with one atomic; every lane
+// of the wave must be active (the tracers' exit, where all 64 lanes meet).
+__device__ __forceinline__ void waveAddTotal(unsigned long long *stats, int slot, uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (laneId() == 0 && v) atomicAdd(&stats[slot], (unsigned long long)v);
+}
+
 __device__ __forceinline__ void waveAddStat(unsigned long long *stats, int slot, uint32_t v) {
     // Counting runs only (statsEnabled); one relaxed 64-bit atomic per lane.
     if (v) atomicAdd(&stats[slot], (unsigned long long)v);
@@ -475,6 +482,11 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= count) return;  // whole waves past the end leave together
     uint32_t cPrimary = 0u, cBounce = 0u;
+    if (!RESUME) {  // each queued path was one extension query (k_extend)
+        const uint64_t live = __ballot(1);
+        if (laneId() == uint32_t(__ffsll((unsigned long long)live) - 1))
+            atomicAdd(&P.stats[DR_CNT_RAY_QUERIES], (unsigned long long)__popcll(live));
+    }
 
     const uint32_t slot = queue[qi];
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
@@ -908,6 +920,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
             }
         }
     }
+    waveAddTotal(P.stats, DR_CNT_RAY_QUERIES, shQ);  // the extension queries are counted by k_shade
     if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_NODES, nv);
         waveAddStat(P.stats, DR_CNT_TRIS, nt);
@@ -1110,6 +1123,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
         if (resume) P.resumeQueue[rpos] = slot;
     }
+    waveAddTotal(P.stats, DR_CNT_RAY_QUERIES, cSss + shQ);
     if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_BOUNCE, cBounce);
         waveAddStat(P.stats, DR_CNT_SSS_NODES, cNodes);
@@ -1230,6 +1244,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrScene
         const bool occluded = shadowChain<VOL>(S, o, d, maxDistance, tr, stk, nq, nv, nt, sc);
         P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
     }
+    waveAddTotal(P.stats, DR_CNT_RAY_QUERIES, nq);
     if constexpr (STATS) {
         waveAddStat(P.stats, DR_CNT_SHADOW, nq);
         waveAddStat(P.stats, DR_CNT_SHADOW_NODES, nv);

@@ -175,5 +175,9 @@ enum {
     DR_CNT_EXT_SHADOW = 33,
     DR_CNT_EXT_SHADOW_NODES = 34,
     DR_CNT_EXT_SHADOW_TRIS = 35,
-    DR_CNT_SLOTS = 36
+    // Every scene query (extension, NEE shadow-chain step, subsurface-walk
+    // segment), counted by every kernel variant, not only in counting runs:
+    // one atomic per wave, so timed runs know their own query count.
+    DR_CNT_RAY_QUERIES = 36,
+    DR_CNT_SLOTS = 37
 };

@@ -32,7 +32,7 @@ from .scene import AccelerationBuild, Camera, DenrimRendererError, RenderScene, 
 from .settings import RenderAccelerationMode, RenderOutput, RenderQuality, RenderSettings
 
 STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "finalize", "sss", "shade_resume", "preview")
-N_STATS = 36
+N_STATS = 37
 
 
 @dataclass(frozen=True)

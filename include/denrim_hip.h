@@ -197,7 +197,10 @@ typedef enum {
     DR_STAT_SHADOW_QUERIES_IN_EXTEND_KERNEL = 33,
     DR_STAT_SHADOW_NODE_VISITS_IN_EXTEND_KERNEL = 34,
     DR_STAT_SHADOW_TRIANGLE_TESTS_IN_EXTEND_KERNEL = 35,
-    DR_STAT_COUNT = 36
+    /* every scene query (extension, shadow-chain step, subsurface-walk
+     * segment), counted in every run, counting or not */
+    DR_STAT_RAY_QUERIES = 36,
+    DR_STAT_COUNT = 37
 } dr_stat;
 
 /* SDF primitive for the volume builders (SDFPrimitive / CompiledPrimitive,

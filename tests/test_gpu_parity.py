@@ -408,3 +408,25 @@ def test_stack_spill_build_matches(renderer, tmp_path):
     sess.render(2)
     assert np.array_equal(got["tri"], sess.triangle_ids())
     assert np.array_equal(got["beauty"], sess.pixels(RenderOutput.BEAUTY))
+
+
+def test_ray_query_counter_in_every_kernel_variant(renderer):
+    # DR_STAT_RAY_QUERIES is kept by the timed (non-counting) kernels too:
+    # on the same samples it equals the counting run's primary + bounce +
+    # shadow queries, on a triangle scene with layered materials (walks,
+    # shadow chains through dielectrics) and on a scene with SDF volumes.
+    scenes = [(synthetic.layered_material_gallery(), 96, 64, 8),
+              (synthetic.sdf_bricks_scene(resolution=96, primitives=40, hdri=(64, 32)), 64, 64, 4)]
+    for scene, W, H, bounces in scenes:
+        gs = renderer.upload(compile_scene(scene))
+        counts = []
+        for counting in (True, False):
+            st = RenderSettings(width=W, height=H, max_bounces=bounces, quality=RenderQuality.FINAL, frame_seed=3,
+                                collects_sdf_traversal_stats=counting)
+            sess = renderer.make_session(gs, st, camera=scene.camera, samples_per_launch=2)
+            sess.render(4)
+            counts.append(sess.stats().astype(np.int64))
+        c, t = counts
+        assert c[36] == c[0] + c[1] + c[2] > 0
+        assert t[36] == c[36]
+        assert t[:36].sum() == 0  # the timed variants keep no other counter
