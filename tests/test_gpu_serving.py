@@ -274,3 +274,39 @@ def test_failed_encode_released_at_once(renderer, monkeypatch):
         assert s in ("done", "failed")
     finally:
         server.close()
+
+
+def test_material_preview_stream_as_configured(renderer):
+    # The material-preview serving workload at its stated sizes: a 20k-
+    # triangle sphere on a 2-triangle plane with the 1024x512 synthetic HDRI
+    # and seeded layered materials; one request each of 96^2, 256^2, 512^2
+    # and 1280x720 at 16 spp submitted together. Each served result equals
+    # the same request rendered alone, bit for bit; a 96^2 request at 1 spp
+    # equals the oracle (IDs exact, beauty within the 1-spp bound).
+    import oracle as O
+    scenes = [synthetic.material_preview_scene(seed) for seed in range(4)]
+    builds = [compile_scene(s) for s in scenes]
+    assert 20_000 <= len(builds[0].triangles) <= 20_300 and builds[0].environment_samples.shape[0] == 1024 * 512
+    uploaded = [renderer.upload(b) for b in builds]
+    reqs = [(0, 96, 96), (1, 256, 256), (2, 512, 512), (3, 1280, 720)]
+    server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=4)
+    try:
+        tickets = [server.submit(uploaded[i], _settings(W, H), scenes[i].camera, 16) for i, W, H in reqs]
+        for t, (i, W, H) in zip(tickets, reqs):
+            assert server.wait(t) == "done"
+            beauty, albedo = _alone(renderer, uploaded[i], scenes[i].camera, _settings(W, H), 16)
+            assert np.array_equal(server.result(t), beauty), (W, H)
+            assert np.array_equal(server.result(t, RenderOutput.ALBEDO), albedo), (W, H)
+            server.release(t)
+    finally:
+        server.close()
+    st = _settings(96, 96)
+    sess = renderer.make_session(uploaded[0], st, camera=scenes[0].camera)
+    sess.render(1)
+    ref = O.render(O.OracleScene(builds[0]), O.make_settings(builds[0], 96, 96, max_bounces=4, frame_seed=7, clamp=64.0),
+                   scenes[0].camera.gpu_camera(96, 96), sample_count=1)
+    assert np.array_equal(sess.triangle_ids(), ref.aovs[6][..., 0].view(np.uint32))
+    assert np.array_equal(sess.pixels(RenderOutput.MATERIAL_ID), ref.aovs[3])
+    g, r = sess.pixels(RenderOutput.BEAUTY), ref.accumulation
+    d = np.abs(g - r)
+    assert (d > 1e-3 * (1 + np.abs(r))).mean() <= 0.01 and d.mean() < 2e-3

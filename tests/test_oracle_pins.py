@@ -268,3 +268,31 @@ def test_fp32_and_fp64_oracles_agree_on_ids():
     same = (r32.aovs[3][..., 0] == r64.aovs[3][..., 0]).mean()
     assert same > 0.99
     assert np.isfinite(r64.accumulation).all()
+
+
+def test_oracle_matches_more_reference_fixtures():
+    # ReferenceMetrics/ScriptedTexture{Albedo,Normal}.json (the albedo and
+    # normal AOV PNGs of a checker-textured, normal-mapped quad, preview
+    # quality) and DistanceVolumeReference.json (the dense distance-volume
+    # scene, final quality, 4 spp), through the reference's output encodings.
+    from denrimrendererkit_amd import scenescript
+    from denrimrendererkit_amd.image import visualized_rgba8
+    from denrimrendererkit_amd.settings import RenderOutput
+    script = ("camera 0 0 3 0 0 0 34\ntexture checker checker 1 0 0 1 0 0 1 1\n"
+              "texture tangentRight solid 1 0.5 0.5 1\n"
+              "material textured 1 1 1 roughness 0.7 baseColorTexture checker normalMap tangentRight\n"
+              "quad textured -1 -1 0 1 -1 0 1 1 0 -1 1 0\n")
+    scene = scenescript.parse(script)
+    b = compile_scene(scene)
+    st = O.make_settings(b, 48, 48, max_bounces=1, frame_seed=0x1234ABCD, clamp=10.0, quality=0)
+    r = O.render(O.OracleScene(b), st, scene.camera.gpu_camera(48, 48), sample_count=1)
+    for k, out, name in ((2, RenderOutput.ALBEDO, "ScriptedTextureAlbedo"), (1, RenderOutput.NORMAL, "ScriptedTextureNormal")):
+        base = json.load(open(os.path.join(GOLDEN, "reference_metrics", f"{name}.json")))
+        _assert_metrics(_image_metrics(visualized_rgba8(r.aovs[k], out)), base)
+    base = json.load(open(os.path.join(GOLDEN, "reference_metrics", "DistanceVolumeReference.json")))
+    scene = RenderScene.distance_volume_reference()
+    b = compile_scene(scene)
+    W, H = base["metrics"]["width"], base["metrics"]["height"]
+    st = O.make_settings(b, W, H, max_bounces=base["maxBounces"], frame_seed=0x1234ABCD, clamp=64.0)
+    r = O.render(O.OracleScene(b), st, scene.camera.gpu_camera(W, H), sample_count=base["samples"])
+    _assert_metrics(_image_metrics(visualized_rgba8(r.accumulation, RenderOutput.BEAUTY)), base)
