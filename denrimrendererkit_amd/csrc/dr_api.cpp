@@ -557,6 +557,10 @@ static void freeSession(dr_session *s) {
 // Shape of a session for settings: samples per batch, path slots, lanes of
 // the persistent kernels (at most 16 x 128 per CU, no more than the slots
 // need), stack-spill stride, and the bytes of both blocks.
+// Workgroups per CU of the persistent tracer launches.
+#ifndef DR_TRACE_BLOCKS_PER_CU
+#define DR_TRACE_BLOCKS_PER_CU 12u
+#endif
 struct SessionShape {
     uint32_t k, nSlots, traceGrid, stride;
     PathLayout lay;
@@ -572,7 +576,7 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
     sh.k = st->samples_per_launch ? st->samples_per_launch : autoSamplesPerLaunch(pixels);
     if (pixels * sh.k > 0xFFFFFFFFull / 3) return fail(DR_ERR_INVALID_ARGUMENT, "image too large");
     sh.nSlots = uint32_t(pixels * sh.k);
-    const uint32_t full = uint32_t(std::max(1, scene->device->cuCount)) * 16u;
+    const uint32_t full = uint32_t(std::max(1, scene->device->cuCount)) * DR_TRACE_BLOCKS_PER_CU;
     sh.traceGrid = std::max(1u, std::min(full, (sh.nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
     sh.stride = std::max(sh.nSlots, sh.traceGrid * DR_TRACE_BLOCK);
     sh.lay = pathLayout(sh.nSlots, sh.stride, scene->stackMax);

@@ -426,7 +426,7 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
 #define DR_REFILL_MIN 16
 #endif
 #ifndef DR_RETIRE_MIN
-#define DR_RETIRE_MIN 8
+#define DR_RETIRE_MIN 16
 #endif
 // STEPWISE selects the loop shape: false = while-while (node phase until
 // every lane holds a leaf, then the leaf phase; best for the extension
