@@ -94,7 +94,7 @@ SERVE_SPP = (16, 64, 256)
 SERVE_DENOISED_SPP = {
     (96, 96): {16: 0, 64: 0, 256: 0},
     (256, 256): {16: 1, 64: 2, 256: 0},
-    (512, 512): {16: 1, 64: 2, 256: 8},
+    (512, 512): {16: 1, 64: 1, 256: 4},
     (1280, 720): {16: 1, 64: 1, 256: 2},
 }
 
