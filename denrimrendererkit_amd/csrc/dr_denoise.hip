@@ -188,6 +188,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs c) {
 // channels + 8 of padding), so the 16 lanes of a channel group read 16
 // disjoint 4-bank spans. Accumulation runs chunk-major (tap-major inside),
 // which only reorders the FP32 sums of inputs wider than 32 channels.
+// Whether the output tile of k_conv3x3_lds (4 * MB rows x 16 pixels x
+// 16 * NB channels of `bytes` each, 8 elements of padding per pixel) fits
+// the LDS space of its input tile.
+template <int NB, int MB>
+constexpr bool outTileFits(int bytes) {
+    return 4 * MB * 16 * (NB * 16 + 8) * bytes <= (4 * MB + 2) * 18 * 40 * 2;
+}
+
 template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     constexpr int TH = 4 * MB, LR = TH + 2, LC = 18, PIX = 40;
@@ -247,6 +255,44 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     }
     // D: lane holds rows (pixels) 4 * (lane / 16) + j, column (channel) lane % 16.
     const int co0 = lane & 15;
+    float bias[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bias[nb] = c.bias ? c.bias[nb * 16 + co0] : 0.f;
+    if (c.f32out ? outTileFits<NB, MB>(4) : outTileFits<NB, MB>(2)) {
+        // Through LDS: the tile's outputs (bias, ReLU, BF16 or FP32) go to
+        // the input tile's space, pixel-major, then each thread stores whole
+        // 16-byte vectors of one pixel's channels.
+        constexpr int OST = NB * 16 + 8;  // BF16 elements per pixel (8 of padding)
+        __syncthreads();                  // every wave is done with the input tile
+        const bool f32 = c.f32out != 0;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = (wave * MB + mb) * 16 + (lane >> 4) * 4 + j;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const int co = nb * 16 + co0;
+                    float v = acc[mb][nb][j] + bias[nb];
+                    if (c.relu) v = fmaxf(v, 0.f);
+                    if (f32) reinterpret_cast<float *>(tile)[p * OST + co] = v;
+                    else tile[p * OST + co] = f2bf(v);
+                }
+            }
+        __syncthreads();
+        const int esize = f32 ? 4 : 2;
+        const int vecs = NB * 16 * esize / 16;  // 16-byte vectors per pixel
+        for (int v = threadIdx.x; v < TH * 16 * vecs; v += 256) {
+            const int p = v / vecs, q = v - p * vecs;
+            const int oy = ty * TH + (p >> 4), ox = tx * 16 + (p & 15);
+            if (oy >= c.H || ox >= c.W) continue;
+            const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+            const uint4 val = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(tile) +
+                                                               size_t(p) * OST * esize + q * 16);
+            *reinterpret_cast<uint4 *>(static_cast<char *>(c.out) + o * esize + q * 16) = val;
+        }
+        return;
+    }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
         const int oy = row0 + mb;
@@ -259,7 +305,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = nb * 16 + co0;
-                float v = acc[mb][nb][j] + (c.bias ? c.bias[co] : 0.f);
+                float v = acc[mb][nb][j] + bias[nb];
                 if (c.relu) v = fmaxf(v, 0.f);
                 if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
                 else static_cast<uint16_t *>(c.out)[o + co] = f2bf(v);
