@@ -199,7 +199,14 @@ constexpr bool outTileFits(int bytes) {
 template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     constexpr int TH = 4 * MB, LR = TH + 2, LC = 18, PIX = 40;
+    // Up to 64 output channels the chunk's weights (9 taps x cout x 32
+    // channels, zero past cin, rows padded to 40 elements like the pixels)
+    // are staged in LDS with the input tile: one round trip per chunk
+    // instead of one per tap, and one copy per workgroup instead of per wave.
+    constexpr bool WLDS = NB <= 4;
+    constexpr int WROW = 40;
     __shared__ __attribute__((aligned(16))) uint16_t tile[LR * LC * PIX];
+    __shared__ __attribute__((aligned(16))) uint16_t wlds[WLDS ? 9 * NB * 16 * WROW : 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tilesX = (c.W + 15) >> 4, tilesY = (c.H + TH - 1) / TH;
     int t = blockIdx.x;
@@ -232,16 +239,31 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
             }
             *reinterpret_cast<bf8 *>(&tile[pix * PIX + q * 8]) = val;
         }
+        if constexpr (WLDS) {
+            for (int v = threadIdx.x; v < 9 * NB * 16 * 4; v += 256) {
+                const int row = v >> 2, q = v & 3;  // row = tap * cout + output channel
+                const int k = kc + q * 8;
+                bf8 val = zero8;
+                if (k < cin) val = *reinterpret_cast<const bf8 *>(c.wgt + size_t(row) * cin + k);
+                *reinterpret_cast<bf8 *>(&wlds[row * WROW + q * 8]) = val;
+            }
+        }
         __syncthreads();
         const int k0 = kc + kg;
         const bool kin = k0 < cin;
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-            const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin;
             bf8 bv[NB];
+            if constexpr (WLDS) {
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb)
-                bv[nb] = kin ? *reinterpret_cast<const bf8 *>(wt + size_t(nb) * 16 * cin + k0) : zero8;
+                for (int nb = 0; nb < NB; ++nb)
+                    bv[nb] = *reinterpret_cast<const bf8 *>(&wlds[(tap * NB * 16 + nb * 16 + (lane & 15)) * WROW + kg]);
+            } else {
+                const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    bv[nb] = kin ? *reinterpret_cast<const bf8 *>(wt + size_t(nb) * 16 * cin + k0) : zero8;
+            }
             const int lx = (lane & 15) + dx + 1;
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) {
