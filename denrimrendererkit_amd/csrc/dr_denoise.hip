@@ -215,6 +215,9 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     const int ty = t / tilesX, tx = t - ty * tilesX;
     const int row0 = ty * TH + wave * MB;
     const int kg = (lane >> 4) * 8;
+    // Output channels [coBase, coBase + 16 * NB) of c.cout: layers with few
+    // tiles spread their channel groups over blockIdx.y.
+    const int coBase = blockIdx.y * NB * 16;
     const int cin = c.ca + c.cb;
     const int Hb = c.H >> 1, Wb = c.W >> 1;
     f4 acc[MB][NB];
@@ -241,10 +244,12 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
         }
         if constexpr (WLDS) {
             for (int v = threadIdx.x; v < 9 * NB * 16 * 4; v += 256) {
-                const int row = v >> 2, q = v & 3;  // row = tap * cout + output channel
+                const int row = v >> 2, q = v & 3;  // row = tap * 16 * NB + output channel - coBase
+                const int tap = row / (NB * 16), col = row - tap * (NB * 16);
                 const int k = kc + q * 8;
                 bf8 val = zero8;
-                if (k < cin) val = *reinterpret_cast<const bf8 *>(c.wgt + size_t(row) * cin + k);
+                if (k < cin)
+                    val = *reinterpret_cast<const bf8 *>(c.wgt + (size_t(tap) * c.cout + coBase + col) * cin + k);
                 *reinterpret_cast<bf8 *>(&wlds[row * WROW + q * 8]) = val;
             }
         }
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
                 for (int nb = 0; nb < NB; ++nb)
                     bv[nb] = *reinterpret_cast<const bf8 *>(&wlds[(tap * NB * 16 + nb * 16 + (lane & 15)) * WROW + kg]);
             } else {
-                const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(lane & 15) * cin;
+                const uint16_t *wt = c.wgt + size_t(tap) * c.cout * cin + size_t(coBase + (lane & 15)) * cin;
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     bv[nb] = kin ? *reinterpret_cast<const bf8 *>(wt + size_t(nb) * 16 * cin + k0) : zero8;
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     const int co0 = lane & 15;
     float bias[NB];
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) bias[nb] = c.bias ? c.bias[nb * 16 + co0] : 0.f;
+    for (int nb = 0; nb < NB; ++nb) bias[nb] = c.bias ? c.bias[coBase + nb * 16 + co0] : 0.f;
     if (c.f32out ? outTileFits<NB, MB>(4) : outTileFits<NB, MB>(2)) {
         // Through LDS: the tile's outputs (bias, ReLU, BF16 or FP32) go to
         // the input tile's space, pixel-major, then each thread stores whole
@@ -308,7 +313,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
             const int p = v / vecs, q = v - p * vecs;
             const int oy = ty * TH + (p >> 4), ox = tx * 16 + (p & 15);
             if (oy >= c.H || ox >= c.W) continue;
-            const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+            const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout + coBase;
             const uint4 val = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(tile) +
                                                                size_t(p) * OST * esize + q * 16);
             *reinterpret_cast<uint4 *>(static_cast<char *>(c.out) + o * esize + q * 16) = val;
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
             const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
-                const int co = nb * 16 + co0;
+                const int co = coBase + nb * 16 + co0;
                 float v = acc[mb][nb][j] + bias[nb];
                 if (c.relu) v = fmaxf(v, 0.f);
                 if (c.f32out) static_cast<float *>(c.out)[o + co] = v;
@@ -622,15 +627,22 @@ hipError_t launchAtrous(const float4 *src, const float4 *depth, const float4 *no
 #define DR_CONV_LDS 1
 #endif
 template <int NB, int MB>
-hipError_t convLaunch(const ConvArgs &c, hipStream_t stream) {
+hipError_t convLaunch(const ConvArgs &c, hipStream_t stream, unsigned groups = 1) {
     const unsigned grid = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 4 * MB - 1) / (4 * MB)));
-    if (grid && DR_CONV_LDS) hipLaunchKernelGGL((k_conv3x3_lds<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
+    if (grid && DR_CONV_LDS) hipLaunchKernelGGL((k_conv3x3_lds<NB, MB>), dim3(grid, groups), dim3(256), 0, stream, c);
     else if (grid) hipLaunchKernelGGL((k_conv3x3<NB, MB>), dim3(grid), dim3(256), 0, stream, c);
     return hipGetLastError();
 }
 
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
     if (c.ca % 16 || c.cb % 16 || c.cout % 16 || (c.cb && (c.H % 2 || c.W % 2))) return hipErrorInvalidValue;
+    // Layers with fewer 16x16 tiles than CUs (the deep levels) give each
+    // workgroup one 16-channel group of the output, with 4-row tiles when
+    // even that leaves fewer workgroups than CUs.
+    const unsigned tiles16 = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 15) / 16));
+    const unsigned groups = unsigned(c.cout / 16);
+    if (DR_CONV_LDS && groups > 1 && tiles16 < 256u)
+        return tiles16 * groups >= 256u ? convLaunch<1, 4>(c, stream, groups) : convLaunch<1, 1>(c, stream, groups);
     // Row blocks per wave: 4 for narrow outputs, fewer as the accumulators grow.
     switch (c.cout / 16) {
     case 1: return convLaunch<1, 4>(c, stream);
