@@ -10,6 +10,8 @@
 // each MFMA operand is one 8-byte LDS read, reduced over pixels in registers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dr_denoise.h"
 
 namespace dr {
@@ -341,6 +343,143 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     }
 }
 
+// The LDS convolution as a persistent, pipelined kernel for full-resolution
+// layers (16 * NB output channels, NB <= 2, 16-row tiles): a workgroup
+// stages every channel chunk's weights in LDS once, then walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ...; while the matrix cores work on
+// one (tile, chunk) item from LDS, the next item's input tile is already
+// being fetched into registers. Accumulation order and epilogue are those of
+// k_conv3x3_lds, so the outputs are the same bits.
+template <int NB, int CHUNKS>
+__global__ __launch_bounds__(256) void k_conv3x3_pipe(ConvArgs c) {
+    constexpr int MB = 4, TH = 16, LR = TH + 2, LC = 18, PIX = 40, WROW = 40;
+    constexpr int PIECES = LR * LC * 4;             // 16-byte pieces of one input tile chunk
+    constexpr int PER = (PIECES + 255) / 256;       // pieces per thread
+    __shared__ __attribute__((aligned(16))) uint16_t tile[LR * LC * PIX];
+    __shared__ __attribute__((aligned(16))) uint16_t wlds[CHUNKS * 9 * NB * 16 * WROW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tilesX = (c.W + 15) >> 4, tilesY = (c.H + TH - 1) / TH;
+    const int tiles = c.n * tilesX * tilesY;
+    const int kg = (lane >> 4) * 8;
+    const int cin = c.ca + c.cb;
+    const int Hb = c.H >> 1, Wb = c.W >> 1;
+    const bf8 zero8 = {};
+    for (int v = threadIdx.x; v < CHUNKS * 9 * NB * 16 * 4; v += 256) {
+        const int row = v >> 2, q = v & 3;  // row = (chunk * 9 + tap) * 16 * NB + output channel
+        const int chunk = row / (9 * NB * 16), r = row - chunk * (9 * NB * 16);
+        const int tap = r / (NB * 16), col = r - tap * (NB * 16);
+        const int k = chunk * 32 + q * 8;
+        bf8 val = zero8;
+        if (k < cin) val = *reinterpret_cast<const bf8 *>(c.wgt + (size_t(tap) * c.cout + col) * cin + k);
+        *reinterpret_cast<bf8 *>(&wlds[row * WROW + q * 8]) = val;
+    }
+    // Fetches item (t, chunk)'s input tile into registers.
+    bf8 pre[PER];
+    auto fetch = [&](int t, int chunk) {
+        const int img = t / (tilesX * tilesY);
+        const int tt = t - img * tilesX * tilesY;
+        const int ty = tt / tilesX, tx = tt - ty * tilesX;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int v = threadIdx.x + i * 256;
+            const int pix = v >> 2, q = v & 3;
+            const int ly = pix / LC, lx = pix - ly * LC;
+            const int sy = ty * TH - 1 + ly, sx = tx * 16 - 1 + lx;
+            const int k0 = chunk * 32 + q * 8;
+            bf8 val = zero8;
+            if (v < PIECES && k0 < cin && sy >= 0 && sy < c.H && sx >= 0 && sx < c.W) {
+                const uint16_t *src =
+                    k0 < c.ca ? c.a + ((size_t(img) * c.H + sy) * c.W + sx) * c.ca + k0
+                              : c.b + ((size_t(img) * Hb + (sy >> 1)) * Wb + (sx >> 1)) * c.cb + (k0 - c.ca);
+                val = *reinterpret_cast<const bf8 *>(src);
+            }
+            pre[i] = val;
+        }
+    };
+    int t = blockIdx.x, chunk = 0;
+    if (t < tiles) fetch(t, 0);
+    const int co0 = lane & 15;
+    float bias[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bias[nb] = c.bias ? c.bias[nb * 16 + co0] : 0.f;
+    f4 acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
+    while (t < tiles) {
+        __syncthreads();  // the previous item's reads of the tile (and the epilogue's) are done
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int v = threadIdx.x + i * 256;
+            if (v < PIECES) *reinterpret_cast<bf8 *>(&tile[(v >> 2) * PIX + (v & 3) * 8]) = pre[i];
+        }
+        __syncthreads();
+        // Next item: the following chunk of this tile, else the next tile.
+        const int nt = chunk + 1 < CHUNKS ? t : t + int(gridDim.x);
+        const int nc = chunk + 1 < CHUNKS ? chunk + 1 : 0;
+        if (nt < tiles) fetch(nt, nc);
+        const uint16_t *w = wlds + chunk * (9 * NB * 16 * WROW);
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            bf8 bv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                bv[nb] = *reinterpret_cast<const bf8 *>(&w[(tap * NB * 16 + nb * 16 + co0) * WROW + kg]);
+            const int lx = co0 + dx + 1;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const int ly = wave * MB + mb + dy + 1;
+                const bf8 av = *reinterpret_cast<const bf8 *>(&tile[(ly * LC + lx) * PIX + kg]);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
+        if (chunk + 1 == CHUNKS) {
+            // Epilogue of tile t through the tile's LDS, as in k_conv3x3_lds.
+            const int img = t / (tilesX * tilesY);
+            const int tt = t - img * tilesX * tilesY;
+            const int ty = tt / tilesX, tx = tt - ty * tilesX;
+            constexpr int OST = NB * 16 + 8;
+            const bool f32 = c.f32out != 0;
+            __syncthreads();
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int p = (wave * MB + mb) * 16 + (lane >> 4) * 4 + j;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) {
+                        const int co = nb * 16 + co0;
+                        float v = acc[mb][nb][j] + bias[nb];
+                        if (c.relu) v = fmaxf(v, 0.f);
+                        if (f32) reinterpret_cast<float *>(tile)[p * OST + co] = v;
+                        else tile[p * OST + co] = f2bf(v);
+                    }
+                }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
+            __syncthreads();
+            const int esize = f32 ? 4 : 2;
+            const int vecs = NB * 16 * esize / 16;
+            for (int v = threadIdx.x; v < TH * 16 * vecs; v += 256) {
+                const int p = v / vecs, q = v - p * vecs;
+                const int oy = ty * TH + (p >> 4), ox = tx * 16 + (p & 15);
+                if (oy >= c.H || ox >= c.W) continue;
+                const size_t o = ((size_t(img) * c.H + oy) * c.W + ox) * c.cout;
+                const uint4 val = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(tile) +
+                                                                   size_t(p) * OST * esize + q * 16);
+                *reinterpret_cast<uint4 *>(static_cast<char *>(c.out) + o * esize + q * 16) = val;
+            }
+        }
+        t = nt;
+        chunk = nc;
+    }
+}
+
 constexpr int kWgRow = 64 + 4;  // LDS row: 64 pixels + 4 elements of padding
 constexpr int kWgMaxQ = 16;     // weight tiles per wave (cout/16 * cin/16 <= 64)
 
@@ -626,6 +765,30 @@ hipError_t launchAtrous(const float4 *src, const float4 *depth, const float4 *no
 #ifndef DR_CONV_LDS
 #define DR_CONV_LDS 1
 #endif
+// Multiprocessors of the current device (cached per device ordinal).
+int deviceCUs() {
+    static int cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int n = 0;
+        cache[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+    }
+    return cache[dev];
+}
+
+template <int NB, int CHUNKS>
+hipError_t convPipeLaunch(const ConvArgs &c, hipStream_t stream, unsigned tiles) {
+    // One resident wave of workgroups (registers and LDS decide how many per CU).
+    static int perCU = 0;
+    if (!perCU && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_conv3x3_pipe<NB, CHUNKS>, 256, 0) !=
+                       hipSuccess || perCU < 1))
+        perCU = 1;
+    const unsigned grid = std::min<unsigned>(tiles, unsigned(deviceCUs()) * unsigned(perCU));
+    hipLaunchKernelGGL((k_conv3x3_pipe<NB, CHUNKS>), dim3(grid), dim3(256), 0, stream, c);
+    return hipGetLastError();
+}
+
 template <int NB, int MB>
 hipError_t convLaunch(const ConvArgs &c, hipStream_t stream, unsigned groups = 1) {
     const unsigned grid = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 4 * MB - 1) / (4 * MB)));
@@ -643,6 +806,14 @@ hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
     const unsigned groups = unsigned(c.cout / 16);
     if (DR_CONV_LDS && groups > 1 && tiles16 < 256u)
         return tiles16 * groups >= 256u ? convLaunch<1, 4>(c, stream, groups) : convLaunch<1, 1>(c, stream, groups);
+    // Full-resolution layers (many tiles, at most 32 output channels and 64
+    // input channels): the persistent pipelined kernel, when its output tile
+    // fits the tile's LDS (BF16 outputs, or FP32 with 16 channels).
+    const int cin = c.ca + c.cb;
+    if (DR_CONV_LDS && tiles16 >= 4096u && cin <= 64 && (!c.f32out || groups == 1)) {
+        if (groups == 1) return cin > 32 ? convPipeLaunch<1, 2>(c, stream, tiles16) : convPipeLaunch<1, 1>(c, stream, tiles16);
+        if (groups == 2) return cin > 32 ? convPipeLaunch<2, 2>(c, stream, tiles16) : convPipeLaunch<2, 1>(c, stream, tiles16);
+    }
     // Row blocks per wave: 4 for narrow outputs, fewer as the accumulators grow.
     switch (c.cout / 16) {
     case 1: return convLaunch<1, 4>(c, stream);
