@@ -432,6 +432,9 @@ def main():
                     help="requests in the material-preview serving run (0: skip)")
     ap.add_argument("--denoiser-frames", type=int, default=10,
                     help="1080p learned-denoiser inferences to time (0: skip)")
+    ap.add_argument("--denoiser-only", action="store_true",
+                    help="run only the 1080p learned-denoiser inference leg (--denoiser-frames times) and print "
+                         "its JSON object: the command for kernel profiles of the U-Net")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -455,6 +458,9 @@ def main():
     from denrimrendererkit_amd import DenrimRenderer
     from denrimrendererkit_amd.settings import EnvironmentSampling, TriangleIntersection
     renderer = DenrimRenderer(0)
+    if args.denoiser_only:
+        print(json.dumps(denoiser_bench(renderer, max(1, args.denoiser_frames))), flush=True)
+        return
     isect = TriangleIntersection.WATERTIGHT if args.intersection == "watertight" else TriangleIntersection.MOLLER_TRUMBORE
     env = EnvironmentSampling.ALIAS if args.environment_sampling == "alias" else EnvironmentSampling.CDF
     main_run = measure(renderer, args.workload, args.steps, args.warmup, args.spp_per_step or None, env, isect)
