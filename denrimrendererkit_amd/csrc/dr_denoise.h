@@ -36,6 +36,10 @@ struct ConvArgs {
     int n = 0, H = 0, W = 0;
     int relu = 0;
     int f32out = 0;
+    // 1: always the per-layer LDS kernel (no persistent pipelined kernel, no
+    // output-channel split); the other paths give the same bits, and a test
+    // checks that against this one.
+    int plain = 0;
 };
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream);
 

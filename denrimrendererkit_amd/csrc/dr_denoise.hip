@@ -804,13 +804,13 @@ hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
     // even that leaves fewer workgroups than CUs.
     const unsigned tiles16 = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 15) / 16));
     const unsigned groups = unsigned(c.cout / 16);
-    if (DR_CONV_LDS && groups > 1 && tiles16 < 256u)
+    if (DR_CONV_LDS && !c.plain && groups > 1 && tiles16 < 256u)
         return tiles16 * groups >= 256u ? convLaunch<1, 4>(c, stream, groups) : convLaunch<1, 1>(c, stream, groups);
     // Full-resolution layers (many tiles, at most 32 output channels and 64
     // input channels): the persistent pipelined kernel, when its output tile
     // fits the tile's LDS (BF16 outputs, or FP32 with 16 channels).
     const int cin = c.ca + c.cb;
-    if (DR_CONV_LDS && tiles16 >= 4096u && cin <= 64 && (!c.f32out || groups == 1)) {
+    if (DR_CONV_LDS && !c.plain && tiles16 >= 4096u && cin <= 64 && (!c.f32out || groups == 1)) {
         if (groups == 1) return cin > 32 ? convPipeLaunch<1, 2>(c, stream, tiles16) : convPipeLaunch<1, 1>(c, stream, tiles16);
         if (groups == 2) return cin > 32 ? convPipeLaunch<2, 2>(c, stream, tiles16) : convPipeLaunch<2, 1>(c, stream, tiles16);
     }

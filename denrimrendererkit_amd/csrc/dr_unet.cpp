@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -80,6 +81,9 @@ struct dr_unet {
     float *loss = nullptr;  // per-step loss sums
     uint32_t lossCap = 0;
     uint32_t step = 0;
+    // Test hook (environment DENRIM_HIP_UNET_PLAIN_CONV=1 when the network
+    // is created): every convolution on the per-layer LDS kernel.
+    bool plainConv = false;
     Workspace train, infer;
     // Device-resident training set.
     uint16_t *dsFeat = nullptr;
@@ -204,6 +208,7 @@ dr::ConvArgs convArgs(const dr_unet *u, const Workspace &w, const Layer &L) {
     c.n = w.n, c.H = w.H >> lv, c.W = w.W >> lv;
     c.relu = op.out != Y;
     c.f32out = op.out == Y;
+    c.plain = u->plainConv ? 1 : 0;
     return c;
 }
 
@@ -262,6 +267,7 @@ dr_status forwardBackward(dr_unet *u, uint32_t slot) {
         c.cout = L.ca + L.cb;
         c.n = w.n, c.H = H, c.W = W;
         c.relu = 0, c.f32out = 1;
+        c.plain = u->plainConv ? 1 : 0;
         DR_UHIP(dr::launchConv(c, st), "data gradient");
         DR_UHIP(dr::launchGradSlice(w.grad[op.a], w.din, L.ca + L.cb, 0, L.ca, w.n, H, W, false, st), "grad slice");
         if (op.b >= 0)
@@ -352,6 +358,7 @@ dr_status dr_hip_unet_create(dr_device *device, uint32_t seed, dr_unet **out) {
     DR_UHIP(hipSetDevice(device->ordinal), "hipSetDevice");
     auto *u = new dr_unet;
     u->device = device;
+    if (const char *v = std::getenv("DENRIM_HIP_UNET_PLAIN_CONV")) u->plainConv = std::atoi(v) != 0;
     buildLayers(u);
     DR_DEVICE_LOCK;
     hipError_t e = hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking);

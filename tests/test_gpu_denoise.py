@@ -161,3 +161,27 @@ def test_learned_denoiser_1080p_beats_plain_64(renderer):
     p, q = relmse(plain.pixels(), target), relmse(learned.pixels(), target)
     print(f"1920x1080 room: plain 64 spp {p:.5f}, learned 4 spp {q:.5f} (relMSE vs 1024 spp)")
     assert q < p, (q, p)
+
+
+def test_convolution_paths_give_identical_bits(renderer, monkeypatch):
+    # The persistent pipelined kernel (full-resolution layers of a 1080p
+    # frame) and the output-channel split (layers with fewer 16x16 tiles
+    # than CUs: the deep levels at 1080p, every level at 96x96) must give
+    # the bits of the per-layer LDS kernel: DENRIM_HIP_UNET_PLAIN_CONV=1
+    # when a network is created sends all its convolutions there.
+    fast = UNetDenoiser.pretrained(renderer)
+    monkeypatch.setenv("DENRIM_HIP_UNET_PLAIN_CONV", "1")
+    plain = UNetDenoiser.pretrained(renderer)
+    monkeypatch.delenv("DENRIM_HIP_UNET_PLAIN_CONV")
+    scene = synthetic.room_variant(2002)
+    gs = renderer.upload(compile_scene(scene))
+    for W, H in ((1920, 1080), (96, 96)):
+        out = []
+        for net in (fast, plain):
+            s = renderer.make_session(gs, _st(W, H, spp_seed=3, denoise=DenoiseSettings.learned()), camera=scene.camera)
+            s.set_denoiser_network(net)
+            s.render(2)
+            out.append(s.pixels(RenderOutput.BEAUTY))
+            s.close()
+        assert np.isfinite(out[0]).all()
+        assert np.array_equal(out[0], out[1]), (W, H, float(np.abs(out[0] - out[1]).max()))
