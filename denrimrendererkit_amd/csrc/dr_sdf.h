@@ -217,10 +217,22 @@ __device__ __forceinline__ vec3 brickNormal(const DrSdfDev &V, const DrVolume &v
     return normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
 }
 
+// A sign change found by the march: the bracket [lowT, highT] that the
+// bisection starts from (lowD: the distance at lowT).
+struct DrCrossing {
+    float lowT, highT, lowD;
+};
+
+// intersectVolumeBrickPrepared (:2608-2807) up to the first sign change of
+// the sphere-traced march: AABB clip, validity and range culls, the
+// back-tracking entry and the march. The bisection, polish, normal and
+// material of a crossing are brickFinish, which the caller runs once the
+// whole wave has left its grid walk (same operations per lane, so the same
+// bits; the lanes that hit then refine together instead of stalling the
+// others' walks once per hit).
 template <bool HALF>
-__device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 rd, const DrVolume &vol,
-                                            const DrBrick &brick, float closestT, vec3 lo, vec3 ld, float lupw,
-                                            float cellSize, DrSdfCounters &sc, DrVolHit &hit) {  // :2608-2807
+__device__ __forceinline__ bool brickMarch(const DrSdfDev &V, const DrBrick &brick, float closestT, vec3 lo, vec3 ld,
+                                           float lupw, float cellSize, DrSdfCounters &sc, DrCrossing &cr) {
     float tNear, tFar;
     if (!sdfAABB(lo, ld, v3(brick.cmin.x, brick.cmin.y, brick.cmin.z), v3(brick.cmax.x, brick.cmax.y, brick.cmax.z),
                  closestT, tNear, tFar))
@@ -264,61 +276,7 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
         if (!isfinite(d)) return false;
         bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
         if (crossed) {
-            float lowT = prevT, highT = t, lowD = prevD;
-            float refinedT = t;
-            #pragma unroll 1
-            for (uint32_t k = 0; k < 8u; ++k) {
-                float midT = 0.5f * (lowT + highT);
-                float md = brickDistance<HALF>(V, ctx, lo + ld * midT);
-                if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
-                    lowT = midT;
-                    lowD = md;
-                } else {
-                    highT = midT;
-                }
-                refinedT = 0.5f * (lowT + highT);
-            }
-            bool cubicPolish = cellSize <= 0.03f;
-            bool coarse = V.renderQuality > 0u && cellSize > 0.03f;
-            vec3 rp = lo + ld * refinedT;
-            vec3 n;
-            if (cubicPolish) {
-                // Newton polish steps and the cubic normal share one
-                // evaluation site: the last pass (or the pass that stops the
-                // polish) evaluates the gradient at the final point, which is
-                // what volumeBrickNormal computes there.
-                uint32_t polishCount = V.renderQuality > 1u ? 2u : 1u;
-                vec3 ln;
-                #pragma unroll 1
-                for (uint32_t k = 0;; ++k) {
-                    DrDG pg = brickDistanceGradientCubic<HALF>(V, ctx, rp);
-                    ln = pg.gradient;
-                    if (k == polishCount) break;
-                    float slope = dot(pg.gradient, ld);
-                    if (!isfinite(pg.distance) || fabsf(slope) <= 1e-6f) break;
-                    float projected = refinedT - pg.distance / slope;
-                    if (projected <= lowT || projected >= highT) break;
-                    refinedT = projected;
-                    rp = lo + ld * refinedT;
-                }
-                if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
-                n = normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
-            } else {
-                n = brickNormal<HALF>(V, vol, ctx, rp, false, coarse);
-            }
-            bool ff = dot(n, rd) <= 0.0f;
-            if (!ff) n = -n;
-            // Nearest sample's material (sampleVolumeBrickMaterial, :2350-2356).
-            vec3 uvw = vclamp((rp - ctx.smin) / ctx.extent, v3s(0.0f), v3s(1.0f));
-            uint32_t nx = min((uint32_t)roundf(uvw.x * (float)(ctx.dim[0] - 1u)), ctx.dim[0] - 1u);
-            uint32_t ny = min((uint32_t)roundf(uvw.y * (float)(ctx.dim[1] - 1u)), ctx.dim[1] - 1u);
-            uint32_t nz = min((uint32_t)roundf(uvw.z * (float)(ctx.dim[2] - 1u)), ctx.dim[2] - 1u);
-            hit.t = refinedT;
-            hit.normal = n;
-            hit.material = V.material[ctx.offset + nx + ny * ctx.strideY + nz * ctx.strideZ];
-            hit.frontFacing = ff;
-            hit.dense = false;
-            sc.hits++;
+            cr = DrCrossing{prevT, t, prevD};
             return true;
         }
         prevT = t;
@@ -326,6 +284,71 @@ __device__ __forceinline__ bool intersectBrick(const DrSdfDev &V, vec3 ro, vec3 
         t += fmaxf(fabsf(d) / lupw * 0.8f, minimumStep);
     }
     return false;
+}
+
+// The rest of intersectVolumeBrickPrepared for a crossing (:2700-2807):
+// 8 bisections, the cubic Newton polish (fine grids) or the trilinear
+// normal, front-facing flip and the nearest sample's material.
+template <bool HALF>
+__device__ __forceinline__ void brickFinish(const DrSdfDev &V, const DrVolume &vol, const DrBrick &brick, vec3 rd,
+                                            vec3 lo, vec3 ld, float cellSize, DrCrossing cr, DrSdfCounters &sc,
+                                            DrVolHit &hit) {
+    const DrBrickCtx ctx = brickCtx(V, brick);
+    float lowT = cr.lowT, highT = cr.highT, lowD = cr.lowD;
+    float refinedT = cr.highT;
+    #pragma unroll 1
+    for (uint32_t k = 0; k < 8u; ++k) {
+        float midT = 0.5f * (lowT + highT);
+        float md = brickDistance<HALF>(V, ctx, lo + ld * midT);
+        if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
+            lowT = midT;
+            lowD = md;
+        } else {
+            highT = midT;
+        }
+        refinedT = 0.5f * (lowT + highT);
+    }
+    bool cubicPolish = cellSize <= 0.03f;
+    bool coarse = V.renderQuality > 0u && cellSize > 0.03f;
+    vec3 rp = lo + ld * refinedT;
+    vec3 n;
+    if (cubicPolish) {
+        // Newton polish steps and the cubic normal share one evaluation
+        // site: the last pass (or the pass that stops the polish) evaluates
+        // the gradient at the final point, which is what volumeBrickNormal
+        // computes there.
+        uint32_t polishCount = V.renderQuality > 1u ? 2u : 1u;
+        vec3 ln;
+        #pragma unroll 1
+        for (uint32_t k = 0;; ++k) {
+            DrDG pg = brickDistanceGradientCubic<HALF>(V, ctx, rp);
+            ln = pg.gradient;
+            if (k == polishCount) break;
+            float slope = dot(pg.gradient, ld);
+            if (!isfinite(pg.distance) || fabsf(slope) <= 1e-6f) break;
+            float projected = refinedT - pg.distance / slope;
+            if (projected <= lowT || projected >= highT) break;
+            refinedT = projected;
+            rp = lo + ld * refinedT;
+        }
+        if (dot(ln, ln) <= 1e-10f) ln = v3(0, 1, 0);
+        n = normalize(mulCols(vol.nrm, normalize(ln), 0.0f));
+    } else {
+        n = brickNormal<HALF>(V, vol, ctx, rp, false, coarse);
+    }
+    bool ff = dot(n, rd) <= 0.0f;
+    if (!ff) n = -n;
+    // Nearest sample's material (sampleVolumeBrickMaterial, :2350-2356).
+    vec3 uvw = vclamp((rp - ctx.smin) / ctx.extent, v3s(0.0f), v3s(1.0f));
+    uint32_t nx = min((uint32_t)roundf(uvw.x * (float)(ctx.dim[0] - 1u)), ctx.dim[0] - 1u);
+    uint32_t ny = min((uint32_t)roundf(uvw.y * (float)(ctx.dim[1] - 1u)), ctx.dim[1] - 1u);
+    uint32_t nz = min((uint32_t)roundf(uvw.z * (float)(ctx.dim[2] - 1u)), ctx.dim[2] - 1u);
+    hit.t = refinedT;
+    hit.normal = n;
+    hit.material = V.material[ctx.offset + nx + ny * ctx.strideY + nz * ctx.strideZ];
+    hit.frontFacing = ff;
+    hit.dense = false;
+    sc.hits++;
 }
 
 // ------------------------------------------------------- dense volumes --
@@ -508,63 +531,11 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
         bool volHit = false;
         DrVolHit closest;
         closest.t = best;
-        #pragma unroll 1
-        for (uint32_t step = 0; step < maxSteps && t <= tFar && t < closest.t; ++step) {
-            sc.gridCells++;
-            if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] >= (int32_t)gd[0] || cell[1] >= (int32_t)gd[1] ||
-                cell[2] >= (int32_t)gd[2])
-                break;
-            if (hasMacro) {
-                uint32_t mc[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] / ms[a], md[a] - 1u);
-                uint32_t slot = mo + mc[0] + mc[1] * md[0] + mc[2] * md[0] * md[1];
-                if (slot < V.gridIndexCount && V.gridIndices[slot] == 0u) {
-                    vec3 cur = brickPos + vel * (t - tNear);
-                    float cv[3] = {cur.x, cur.y, cur.z};
-                    float mt[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        float boundary = stepDir[a] > 0 ? (float)min((mc[a] + 1u) * ms[a], gd[a]) : (float)(mc[a] * ms[a]);
-                        mt[a] = stepDir[a] == 0 ? __builtin_inff() : t + (boundary - cv[a]) / velv[a];
-                    }
-                    float macroNextT = fminf(mt[0], fminf(mt[1], mt[2]));
-                    if (macroNextT > t + 1e-6f) {
-                        sc.macroSkips++;
-                        if (macroNextT > tFar || macroNextT >= closest.t) break;
-                        t = macroNextT + 1e-5f;
-                        cur = brickPos + vel * (t - tNear);
-                        vec3 fc2 = vclamp(v3(floorf(cur.x), floorf(cur.y), floorf(cur.z)), v3s(0.0f), gdm1);
-                        cell[0] = (int32_t)fc2.x;
-                        cell[1] = (int32_t)fc2.y;
-                        cell[2] = (int32_t)fc2.z;
-                        float cv2[3] = {cur.x, cur.y, cur.z};
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            float nb = stepDir[a] > 0 ? (float)cell[a] + 1.0f : (float)cell[a];
-                            tMax[a] = stepDir[a] == 0 ? __builtin_inff() : t + (nb - cv2[a]) / velv[a];
-                        }
-                        continue;
-                    }
-                }
-            }
-            uint32_t slot = gio + (uint32_t)cell[0] + (uint32_t)cell[1] * gd[0] + (uint32_t)cell[2] * gd[0] * gd[1];
-            if (slot < V.gridIndexCount) {
-                uint32_t bi = V.gridIndices[slot];
-                if (bi != 0xffffffffu && bi < V.brickCount) {
-                    sc.brickTests++;
-                    DrVolHit cand;
-                    if (intersectBrick<HALF>(V, ro, rd, vol, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cand) &&
-                        cand.t < closest.t) {
-                        closest = cand;
-                        closest.volume = vi;
-                        volHit = true;
-                    }
-                }
-            }
+        // The end of a grid step (:3064-3086): false when the walk stops.
+        auto advance = [&]() -> bool {
             float nextT = fminf(tMax[0], fminf(tMax[1], tMax[2]));
-            if (volHit && closest.t <= nextT + 0.0005f) break;
-            if (nextT > tFar || nextT >= closest.t) break;
+            if (volHit && closest.t <= nextT + 0.0005f) return false;
+            if (nextT > tFar || nextT >= closest.t) return false;
             if (tMax[0] <= tMax[1] && tMax[0] <= tMax[2]) {
                 cell[0] += stepDir[0];
                 t = tMax[0];
@@ -578,6 +549,85 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
                 t = tMax[2];
                 tMax[2] += tDelta[2];
             }
+            return true;
+        };
+        // The grid walk leaves its loop at a brick whose march crossed the
+        // surface; that hit is finished outside the loop (so the wave's
+        // hitting lanes refine together) and the walk resumes where it
+        // stopped, exactly as the reference's single loop continues.
+        uint32_t step = 0;
+        #pragma unroll 1
+        while (true) {
+            bool found = false;
+            uint32_t foundBrick = 0u;
+            DrCrossing cr;
+            #pragma unroll 1
+            for (; step < maxSteps && t <= tFar && t < closest.t; ++step) {
+                sc.gridCells++;
+                if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] >= (int32_t)gd[0] ||
+                    cell[1] >= (int32_t)gd[1] || cell[2] >= (int32_t)gd[2])
+                    break;
+                if (hasMacro) {
+                    uint32_t mc[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] / ms[a], md[a] - 1u);
+                    uint32_t slot = mo + mc[0] + mc[1] * md[0] + mc[2] * md[0] * md[1];
+                    if (slot < V.gridIndexCount && V.gridIndices[slot] == 0u) {
+                        vec3 cur = brickPos + vel * (t - tNear);
+                        float cv[3] = {cur.x, cur.y, cur.z};
+                        float mt[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            float boundary =
+                                stepDir[a] > 0 ? (float)min((mc[a] + 1u) * ms[a], gd[a]) : (float)(mc[a] * ms[a]);
+                            mt[a] = stepDir[a] == 0 ? __builtin_inff() : t + (boundary - cv[a]) / velv[a];
+                        }
+                        float macroNextT = fminf(mt[0], fminf(mt[1], mt[2]));
+                        if (macroNextT > t + 1e-6f) {
+                            sc.macroSkips++;
+                            if (macroNextT > tFar || macroNextT >= closest.t) break;
+                            t = macroNextT + 1e-5f;
+                            cur = brickPos + vel * (t - tNear);
+                            vec3 fc2 = vclamp(v3(floorf(cur.x), floorf(cur.y), floorf(cur.z)), v3s(0.0f), gdm1);
+                            cell[0] = (int32_t)fc2.x;
+                            cell[1] = (int32_t)fc2.y;
+                            cell[2] = (int32_t)fc2.z;
+                            float cv2[3] = {cur.x, cur.y, cur.z};
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                float nb = stepDir[a] > 0 ? (float)cell[a] + 1.0f : (float)cell[a];
+                                tMax[a] = stepDir[a] == 0 ? __builtin_inff() : t + (nb - cv2[a]) / velv[a];
+                            }
+                            continue;
+                        }
+                    }
+                }
+                uint32_t slot = gio + (uint32_t)cell[0] + (uint32_t)cell[1] * gd[0] + (uint32_t)cell[2] * gd[0] * gd[1];
+                if (slot < V.gridIndexCount) {
+                    uint32_t bi = V.gridIndices[slot];
+                    if (bi != 0xffffffffu && bi < V.brickCount) {
+                        sc.brickTests++;
+                        if (brickMarch<HALF>(V, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cr)) {
+                            found = true;
+                            foundBrick = bi;
+                            break;
+                        }
+                    }
+                }
+                if (!advance()) break;
+            }
+            if (!found) break;
+            // A crossing's final t lies in its bracket, below closest.t, so
+            // it always replaces the current closest hit.
+            DrVolHit cand;
+            brickFinish<HALF>(V, vol, V.bricks[foundBrick], rd, lo, ld, cellSize, cr, sc, cand);
+            if (cand.t < closest.t) {
+                closest = cand;
+                closest.volume = vi;
+                volHit = true;
+            }
+            if (!advance()) break;
+            ++step;
         }
         if (volHit && closest.t < best) {
             out = closest;
