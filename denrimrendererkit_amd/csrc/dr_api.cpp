@@ -168,6 +168,7 @@ static void freeScene(dr_scene *s) {
     devFree(s->bricks);
     devFree(s->dist16);
     devFree(s->dist32);
+    devFree(s->corner16);
     devFree(s->sampleMaterial);
     devFree(s->grids);
     devFree(s->gridIndices);
@@ -250,6 +251,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     std::vector<uint16_t> dist16;
     std::vector<float> dist32;
     std::vector<uint2> sampleMat;
+    uint32_t sampleCount = 0;
     bool distHalf = true;
     if (d->volume_count) {
         const auto *rb = static_cast<const RefBrick *>(d->volume_bricks);
@@ -270,21 +272,79 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
             o.cmin = make_float4(b.localBoundsMin[0], b.localBoundsMin[1], b.localBoundsMin[2], pf);
             o.cmax = make_float4(b.localBoundsMax[0], b.localBoundsMax[1], b.localBoundsMax[2], of);
         }
+        // The FP16 corner records are indexed by sample, so the valid bricks'
+        // sample ranges must not overlap; if the caller's do, each valid brick
+        // gets a private copy of its samples (distances and materials alike,
+        // so every lookup reads the same values) and invalid bricks keep
+        // failing the range check through zero dimensions.
         const auto *rs = static_cast<const RefBrickSample *>(d->volume_brick_samples);
-        const uint32_t n = d->volume_brick_sample_count;
+        std::vector<uint32_t> src;  // new sample index -> caller's; empty: identity
+        {
+            const uint32_t n0 = d->volume_brick_sample_count;
+            std::vector<std::pair<uint32_t, uint32_t>> ranges;  // (offset, brick)
+            auto cellsOf = [](const DrBrick &o) {
+                uint32_t packed;
+                std::memcpy(&packed, &o.cmin.w, 4);
+                return uint64_t(packed & 0xffu) * ((packed >> 8) & 0xffu) * ((packed >> 16) & 0xffu);
+            };
+            auto validOf = [&](const DrBrick &o) {
+                uint32_t packed, offset;
+                std::memcpy(&packed, &o.cmin.w, 4);
+                std::memcpy(&offset, &o.cmax.w, 4);
+                return (packed & 0xffu) >= 2u && ((packed >> 8) & 0xffu) >= 2u && ((packed >> 16) & 0xffu) >= 2u &&
+                       offset < n0 && cellsOf(o) <= uint64_t(n0 - offset);
+            };
+            for (uint32_t i = 0; i < d->volume_brick_count; ++i)
+                if (validOf(bricks[i])) {
+                    uint32_t offset;
+                    std::memcpy(&offset, &bricks[i].cmax.w, 4);
+                    ranges.emplace_back(offset, i);
+                }
+            std::sort(ranges.begin(), ranges.end());
+            bool overlap = false;
+            for (size_t k = 1; k < ranges.size() && !overlap; ++k)
+                overlap = uint64_t(ranges[k].first) < uint64_t(ranges[k - 1].first) + cellsOf(bricks[ranges[k - 1].second]);
+            if (overlap) {
+                std::vector<bool> valid(d->volume_brick_count);
+                for (auto &r : ranges) valid[r.second] = true;
+                for (uint32_t i = 0; i < d->volume_brick_count; ++i) {
+                    DrBrick &o = bricks[i];
+                    if (!valid[i]) {
+                        const uint32_t zero = 0u;
+                        std::memcpy(&o.cmin.w, &zero, 4);
+                        continue;
+                    }
+                    uint32_t offset;
+                    std::memcpy(&offset, &o.cmax.w, 4);
+                    const uint64_t cells = cellsOf(o);
+                    if (src.size() + cells > 0xffffffffull) {
+                        freeScene(s);
+                        delete s;
+                        return fail(DR_ERR_INVALID_SCENE, "brick samples exceed 2^32 once overlapping bricks are split");
+                    }
+                    const uint32_t base = uint32_t(src.size());
+                    for (uint64_t k = 0; k < cells; ++k) src.push_back(offset + uint32_t(k));
+                    std::memcpy(&o.cmax.w, &base, 4);
+                }
+            }
+        }
+        const uint32_t n = src.empty() ? d->volume_brick_sample_count : uint32_t(src.size());
+        auto at = [&](uint32_t i) -> const RefBrickSample & { return rs[src.empty() ? i : src[i]]; };
+        sampleCount = n;
         dist16.resize(n);
-        for (uint32_t i = 0; i < n && distHalf; ++i) distHalf = halfExact(rs[i].distance, dist16[i]);
+        for (uint32_t i = 0; i < n && distHalf; ++i) distHalf = halfExact(at(i).distance, dist16[i]);
         if (!distHalf) {
             dist16.clear();
             dist32.resize(n);
-            for (uint32_t i = 0; i < n; ++i) dist32[i] = rs[i].distance;
+            for (uint32_t i = 0; i < n; ++i) dist32[i] = at(i).distance;
         }
         sampleMat.resize(n);
         const uint32_t last = d->material_count - 1u;
         for (uint32_t i = 0; i < n; ++i) {
-            uint32_t a = std::min(rs[i].materialA, last), b = std::min(rs[i].materialB, last);
+            const RefBrickSample &r = at(i);
+            uint32_t a = std::min(r.materialA, last), b = std::min(r.materialB, last);
             uint32_t blend;
-            std::memcpy(&blend, &rs[i].materialBlend, 4);
+            std::memcpy(&blend, &r.materialBlend, 4);
             sampleMat[i] = make_uint2(a | b << 16, blend);
         }
     }
@@ -370,6 +430,10 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     chain(upload(&s->bricks, bricks.data(), bricks.size(), up));
     chain(upload(&s->dist16, dist16.data(), dist16.size(), up));
     chain(upload(&s->dist32, dist32.data(), dist32.size(), up));
+    chain(devAlloc(&s->corner16, distHalf ? size_t(sampleCount) : 0u));
+    if (e == hipSuccess && distHalf && sampleCount)
+        chain(dr::launchBrickCorners(s->bricks, d->volume_brick_count, s->dist16, sampleCount, s->corner16, up));
+    if (e == hipSuccess) chain(hipStreamSynchronize(up));
     chain(upload(&s->sampleMaterial, sampleMat.data(), sampleMat.size(), up));
     chain(upload(&s->grids, d->volume_brick_grids, d->volume_brick_grid_count, up));
     chain(upload(&s->gridIndices, d->volume_brick_grid_indices, d->volume_brick_grid_index_count, up));
@@ -409,12 +473,13 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     v.bricks = s->bricks;
     v.dist16 = s->dist16;
     v.dist32 = s->dist32;
+    v.corner16 = s->corner16;
     v.material = s->sampleMaterial;
     v.grids = s->grids;
     v.gridIndices = s->gridIndices;
     v.volumeCount = d->volume_count;
     v.brickCount = d->volume_brick_count;
-    v.sampleCount = d->volume_brick_sample_count;
+    v.sampleCount = sampleCount;
     v.gridCount = d->volume_brick_grid_count;
     v.gridIndexCount = d->volume_brick_grid_index_count;
     v.distHalf = distHalf ? 1u : 0u;

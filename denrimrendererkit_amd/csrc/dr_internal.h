@@ -51,6 +51,7 @@ struct dr_scene {
     DrBrick *bricks = nullptr;
     uint16_t *dist16 = nullptr;
     float *dist32 = nullptr;
+    uint4 *corner16 = nullptr;
     uint2 *sampleMaterial = nullptr;
     DrBrickGrid *grids = nullptr;
     uint32_t *gridIndices = nullptr;

@@ -104,4 +104,7 @@ hipError_t launchIntersect(const DrSceneDev &S, const float4 *rays, uint32_t cou
 // ray queries (DR_INTERSECT_SCRATCH_WORDS words); call with the device current.
 hipError_t intersectScratchAddress(uint32_t **ptr);
 hipError_t launchAdvanceSampleBase(uint32_t *base, uint32_t samples, hipStream_t st);
+// Fills the FP16 corner records (DrSdfDev::corner16) of every valid brick.
+hipError_t launchBrickCorners(const DrBrick *bricks, uint32_t brickCount, const uint16_t *dist, uint32_t sampleCount,
+                              uint4 *corner, hipStream_t st);
 }

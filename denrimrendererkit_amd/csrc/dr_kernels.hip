@@ -1645,6 +1645,37 @@ hipError_t intersectScratchAddress(uint32_t **ptr) {
     return hipGetSymbolAddress(reinterpret_cast<void **>(ptr), HIP_SYMBOL(scratch));
 }
 
+// Corner records of the FP16 brick distances (DrSdfDev::corner16), one
+// workgroup per brick, written at the brick's own sample indices; bricks
+// that brickCtx rejects get none.
+__global__ __launch_bounds__(256) void k_brick_corners(const DrBrick *bricks, uint32_t brickCount,
+                                                       const uint16_t *dist, uint32_t sampleCount, uint4 *corner) {
+    const uint32_t bi = blockIdx.x;
+    if (bi >= brickCount) return;
+    const DrBrick b = bricks[bi];
+    const uint32_t packed = __float_as_uint(b.cmin.w), offset = __float_as_uint(b.cmax.w);
+    const uint32_t dx = packed & 0xffu, dy = (packed >> 8) & 0xffu, dz = (packed >> 16) & 0xffu;
+    const uint32_t n = dx * dy * dz;
+    if (dx < 2u || dy < 2u || dz < 2u || offset >= sampleCount || n > sampleCount - offset) return;
+    const uint16_t *src = dist + offset;
+    uint4 *dst = corner + offset;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t x = i % dx, y = (i / dx) % dy, z = i / (dx * dy);
+        const uint32_t x1 = min(x + 1u, dx - 1u), y1 = min(y + 1u, dy - 1u), z1 = min(z + 1u, dz - 1u);
+        auto at = [&](uint32_t xx, uint32_t yy, uint32_t zz) -> uint32_t { return src[xx + yy * dx + zz * dx * dy]; };
+        dst[i] = make_uint4(at(x, y, z) | at(x1, y, z) << 16, at(x, y1, z) | at(x1, y1, z) << 16,
+                            at(x, y, z1) | at(x1, y, z1) << 16, at(x, y1, z1) | at(x1, y1, z1) << 16);
+    }
+}
+
+hipError_t launchBrickCorners(const DrBrick *bricks, uint32_t brickCount, const uint16_t *dist, uint32_t sampleCount,
+                              uint4 *corner, hipStream_t st) {
+    if (brickCount == 0u) return hipSuccess;
+    hipLaunchKernelGGL(k_brick_corners, dim3(brickCount), dim3(256), 0, st, bricks, brickCount, dist, sampleCount,
+                       corner);
+    return hipGetLastError();
+}
+
 hipError_t launchSetSampleBase(uint32_t *base, uint32_t value, hipStream_t st) {
     hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, st, base, value);
     return hipGetLastError();

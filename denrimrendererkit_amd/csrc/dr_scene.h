@@ -95,6 +95,12 @@ struct DrSdfDev {
     const DrBrick *bricks;
     const uint16_t *dist16;     // FP16 bits (distHalf != 0)
     const float *dist32;        // FP32 (distHalf == 0)
+    // FP16 only: for sample i = (x, y, z) of a valid brick (valid bricks'
+    // sample ranges are disjoint), the 8 distances a trilinear lookup of the
+    // cell at i reads, (x|x+1, y, z), (x|x+1, y+1, z), (x|x+1, y, z+1),
+    // (x|x+1, y+1, z+1) with the upper index clamped to the brick: one 16-B
+    // record per sample.
+    const uint4 *corner16;
     const uint2 *material;      // x: A | B << 16, y: bits(blend)
     const DrBrickGrid *grids;
     const uint32_t *gridIndices;

@@ -97,12 +97,35 @@ __device__ __forceinline__ void brickGrid(const DrBrickCtx &c, vec3 p, uint32_t 
     f = v3(fr[0], fr[1], fr[2]);
 }
 
+// The 8 FP16 distances of the lookup cell b (n = b + 1 clamped) in one
+// 16-B load: d[i] = (x: i & 1 ? n : b, y: i & 2 ? n : b, z: i & 4 ? n : b).
+__device__ __forceinline__ void cornerLoad(const DrSdfDev &V, const DrBrickCtx &c, const uint32_t b[3], float d[8]) {
+    const uint4 q = V.corner16[c.offset + b[0] + b[1] * c.strideY + b[2] * c.strideZ];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        d[2 * k] = dr_half((uint16_t)(w[k] & 0xffffu));
+        d[2 * k + 1] = dr_half((uint16_t)(w[k] >> 16));
+    }
+}
+
 template <bool HALF>
 __device__ __forceinline__ float brickDistance(const DrSdfDev &V, const DrBrickCtx &c, vec3 p) {  // :2096-2127
     if (!c.valid) return __builtin_inff();
     uint32_t b[3], n[3];
     vec3 f;
     brickGrid(c, p, b, n, f);
+    if constexpr (HALF) {
+        float d[8];
+        cornerLoad(V, c, b, d);
+        float c00 = mixf(d[0], d[1], f.x);
+        float c10 = mixf(d[2], d[3], f.x);
+        float c01 = mixf(d[4], d[5], f.x);
+        float c11 = mixf(d[6], d[7], f.x);
+        float c0 = mixf(c00, c10, f.y);
+        float c1 = mixf(c01, c11, f.y);
+        return mixf(c0, c1, f.z);
+    }
     float c00 = mixf(sdAt<HALF>(V, c, b[0], b[1], b[2]), sdAt<HALF>(V, c, n[0], b[1], b[2]), f.x);
     float c10 = mixf(sdAt<HALF>(V, c, b[0], n[1], b[2]), sdAt<HALF>(V, c, n[0], n[1], b[2]), f.x);
     float c01 = mixf(sdAt<HALF>(V, c, b[0], b[1], n[2]), sdAt<HALF>(V, c, n[0], b[1], n[2]), f.x);
@@ -124,10 +147,17 @@ __device__ __forceinline__ DrDG brickDistanceGradient(const DrSdfDev &V, const D
     uint32_t b[3], n[3];
     vec3 f;
     brickGrid(c, p, b, n, f);
-    float d000 = sdAt<HALF>(V, c, b[0], b[1], b[2]), d100 = sdAt<HALF>(V, c, n[0], b[1], b[2]);
-    float d010 = sdAt<HALF>(V, c, b[0], n[1], b[2]), d110 = sdAt<HALF>(V, c, n[0], n[1], b[2]);
-    float d001 = sdAt<HALF>(V, c, b[0], b[1], n[2]), d101 = sdAt<HALF>(V, c, n[0], b[1], n[2]);
-    float d011 = sdAt<HALF>(V, c, b[0], n[1], n[2]), d111 = sdAt<HALF>(V, c, n[0], n[1], n[2]);
+    float d000, d100, d010, d110, d001, d101, d011, d111;
+    if constexpr (HALF) {
+        float d[8];
+        cornerLoad(V, c, b, d);
+        d000 = d[0], d100 = d[1], d010 = d[2], d110 = d[3], d001 = d[4], d101 = d[5], d011 = d[6], d111 = d[7];
+    } else {
+        d000 = sdAt<HALF>(V, c, b[0], b[1], b[2]), d100 = sdAt<HALF>(V, c, n[0], b[1], b[2]);
+        d010 = sdAt<HALF>(V, c, b[0], n[1], b[2]), d110 = sdAt<HALF>(V, c, n[0], n[1], b[2]);
+        d001 = sdAt<HALF>(V, c, b[0], b[1], n[2]), d101 = sdAt<HALF>(V, c, n[0], b[1], n[2]);
+        d011 = sdAt<HALF>(V, c, b[0], n[1], n[2]), d111 = sdAt<HALF>(V, c, n[0], n[1], n[2]);
+    }
     float c00 = mixf(d000, d100, f.x), c10 = mixf(d010, d110, f.x);
     float c01 = mixf(d001, d101, f.x), c11 = mixf(d011, d111, f.x);
     float c0 = mixf(c00, c10, f.y), c1 = mixf(c01, c11, f.y);
@@ -231,21 +261,24 @@ struct DrCrossing {
 // bits; the lanes that hit then refine together instead of stalling the
 // others' walks once per hit).
 template <bool HALF>
-__device__ __forceinline__ bool brickMarch(const DrSdfDev &V, const DrBrick &brick, float closestT, vec3 lo, vec3 ld,
+__device__ __forceinline__ bool brickMarch(const DrSdfDev &V, uint32_t bi, float closestT, vec3 lo, vec3 ld,
                                            float lupw, float cellSize, DrSdfCounters &sc, DrCrossing &cr) {
+    // The whole header in one round of loads: every field feeds the one
+    // branch below, so no load waits behind another.
+    const float4 *hp = reinterpret_cast<const float4 *>(V.bricks + bi);
+    DrBrick brick;
+    brick.smin = hp[0];
+    brick.smax = hp[1];
+    brick.cmin = hp[2];
+    brick.cmax = hp[3];
     float tNear, tFar;
-    if (!sdfAABB(lo, ld, v3(brick.cmin.x, brick.cmin.y, brick.cmin.z), v3(brick.cmax.x, brick.cmax.y, brick.cmax.z),
-                 closestT, tNear, tFar))
-        return false;
+    const bool inBox = sdfAABB(lo, ld, v3(brick.cmin.x, brick.cmin.y, brick.cmin.z),
+                               v3(brick.cmax.x, brick.cmax.y, brick.cmax.z), closestT, tNear, tFar);
     DrBrickCtx ctx = brickCtx(V, brick);
-    if (!ctx.valid) {
-        sc.invalid++;
-        return false;
-    }
-    if (brick.smin.w > 0.0f || brick.smax.w < 0.0f) {
-        sc.rangeCulls++;
-        return false;
-    }
+    const bool cull = brick.smin.w > 0.0f || brick.smax.w < 0.0f;
+    sc.invalid += (inBox && !ctx.valid) ? 1u : 0u;
+    sc.rangeCulls += (inBox && ctx.valid && cull) ? 1u : 0u;
+    if (!inBox || !ctx.valid || cull) return false;
     sc.marches++;
     float minimumStep = fmaxf(cellSize * 0.10f / lupw, 0.0005f);
     float prevT = tNear;
@@ -290,10 +323,9 @@ __device__ __forceinline__ bool brickMarch(const DrSdfDev &V, const DrBrick &bri
 // 8 bisections, the cubic Newton polish (fine grids) or the trilinear
 // normal, front-facing flip and the nearest sample's material.
 template <bool HALF>
-__device__ __forceinline__ void brickFinish(const DrSdfDev &V, const DrVolume &vol, const DrBrick &brick, vec3 rd,
-                                            vec3 lo, vec3 ld, float cellSize, DrCrossing cr, DrSdfCounters &sc,
-                                            DrVolHit &hit) {
-    const DrBrickCtx ctx = brickCtx(V, brick);
+__device__ __forceinline__ void brickFinish(const DrSdfDev &V, const DrVolume &vol, uint32_t bi, vec3 rd, vec3 lo,
+                                            vec3 ld, float cellSize, DrCrossing cr, DrSdfCounters &sc, DrVolHit &hit) {
+    const DrBrickCtx ctx = brickCtx(V, V.bricks[bi]);
     float lowT = cr.lowT, highT = cr.highT, lowD = cr.lowD;
     float refinedT = cr.highT;
     #pragma unroll 1
@@ -527,6 +559,10 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
         uint32_t ms[3] = {max(grid->macroSizeAndReserved.x, 1u), max(grid->macroSizeAndReserved.y, 1u),
                           max(grid->macroSizeAndReserved.z, 1u)};
         bool hasMacro = md[0] > 0u && md[1] > 0u && md[2] > 0u && mo < V.gridIndexCount;
+        // Macro cells of a power-of-two size (4 in the builder) divide by shifting.
+        const bool msPow2 = (ms[0] & (ms[0] - 1u)) == 0u && (ms[1] & (ms[1] - 1u)) == 0u && (ms[2] & (ms[2] - 1u)) == 0u;
+        const uint32_t msh[3] = {(uint32_t)__builtin_ctz(ms[0]), (uint32_t)__builtin_ctz(ms[1]),
+                                 (uint32_t)__builtin_ctz(ms[2])};
         uint32_t maxSteps = min(gd[0] + gd[1] + gd[2] + 8u, 1024u);
         bool volHit = false;
         DrVolHit closest;
@@ -567,12 +603,22 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
                 if (cell[0] < 0 || cell[1] < 0 || cell[2] < 0 || cell[0] >= (int32_t)gd[0] ||
                     cell[1] >= (int32_t)gd[1] || cell[2] >= (int32_t)gd[2])
                     break;
+                // The cell's brick index is fetched beside its macro word
+                // (one round trip; it is unused when the macro cell is empty).
+                const uint32_t slot =
+                    gio + (uint32_t)cell[0] + (uint32_t)cell[1] * gd[0] + (uint32_t)cell[2] * gd[0] * gd[1];
+                const uint32_t bi = slot < V.gridIndexCount ? V.gridIndices[slot] : 0xffffffffu;
                 if (hasMacro) {
                     uint32_t mc[3];
+                    if (msPow2) {
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] / ms[a], md[a] - 1u);
-                    uint32_t slot = mo + mc[0] + mc[1] * md[0] + mc[2] * md[0] * md[1];
-                    if (slot < V.gridIndexCount && V.gridIndices[slot] == 0u) {
+                        for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] >> msh[a], md[a] - 1u);
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) mc[a] = min((uint32_t)cell[a] / ms[a], md[a] - 1u);
+                    }
+                    uint32_t mslot = mo + mc[0] + mc[1] * md[0] + mc[2] * md[0] * md[1];
+                    if (mslot < V.gridIndexCount && V.gridIndices[mslot] == 0u) {
                         vec3 cur = brickPos + vel * (t - tNear);
                         float cv[3] = {cur.x, cur.y, cur.z};
                         float mt[3];
@@ -602,12 +648,10 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
                         }
                     }
                 }
-                uint32_t slot = gio + (uint32_t)cell[0] + (uint32_t)cell[1] * gd[0] + (uint32_t)cell[2] * gd[0] * gd[1];
-                if (slot < V.gridIndexCount) {
-                    uint32_t bi = V.gridIndices[slot];
+                {
                     if (bi != 0xffffffffu && bi < V.brickCount) {
                         sc.brickTests++;
-                        if (brickMarch<HALF>(V, V.bricks[bi], closest.t, lo, ld, lupw, cellSize, sc, cr)) {
+                        if (brickMarch<HALF>(V, bi, closest.t, lo, ld, lupw, cellSize, sc, cr)) {
                             found = true;
                             foundBrick = bi;
                             break;
@@ -620,7 +664,7 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
             // A crossing's final t lies in its bracket, below closest.t, so
             // it always replaces the current closest hit.
             DrVolHit cand;
-            brickFinish<HALF>(V, vol, V.bricks[foundBrick], rd, lo, ld, cellSize, cr, sc, cand);
+            brickFinish<HALF>(V, vol, foundBrick, rd, lo, ld, cellSize, cr, sc, cand);
             if (cand.t < closest.t) {
                 closest = cand;
                 closest.volume = vi;
