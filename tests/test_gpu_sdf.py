@@ -97,6 +97,42 @@ def test_sdf_primary_counters_exact(renderer, quality):
     assert sess.stats()[0] == ref.counters[O.CNT_PRIMARY]
 
 
+def test_overlapping_brick_sample_ranges(renderer):
+    # Bricks whose sample ranges overlap (the layout allows it: each header
+    # names its own offset) and a brick whose range lies past the samples.
+    # The upload gives overlapping bricks private copies of their samples and
+    # keeps the out-of-range brick invalid; hits, IDs and every counter must
+    # still equal the oracle's, which reads the caller's arrays as they are.
+    from denrimrendererkit_amd import Camera, Material, RenderScene
+    from denrimrendererkit_amd.sdf import build_sparse_volume
+    scene = RenderScene(camera=Camera(origin=(0.5, 0.9, 2.0), target=(0.5, 0.45, 0.5)))
+    metal = scene.add_material(Material(base_color=(0.9, 0.75, 0.55), metallic=1.0, roughness=0.3))
+    matte = scene.add_material(Material(base_color=(0.7, 0.7, 0.7), specular=0.0, roughness=0.5))
+    prims = synthetic.sdf_primitives(synthetic.content_rng(3), 40, metal, matte)
+    scene.add_sparse_volume(build_sparse_volume(prims, dims=(64, 64, 64), narrow_band=2 / 63), metal)
+    build = compile_scene(scene)
+    bricks = build.volume_bricks
+    dso = bricks["dimensionsAndSampleOffset"]
+    assert len(bricks) > 40
+    for k in range(1, 40, 3):  # brick k reads from brick k - 1's first sample
+        dso[k, 3] = dso[k - 1, 3]
+    dso[2, 3] = dso[1, 3] + 7  # a range straddling two bricks
+    dso[5, 3] = len(build.volume_brick_samples) + 5  # past the end: invalid
+    W, H = 80, 64
+    settings = RenderSettings(width=W, height=H, max_bounces=1, quality=RenderQuality.FINAL, frame_seed=0,
+                              collects_sdf_traversal_stats=True)
+    sess = renderer.make_session(renderer.upload(build), settings, camera=scene.camera)
+    sess.render(1)
+    ost = O.make_settings(build, W, H, max_bounces=1, frame_seed=0, clamp=settings.resolved_sample_radiance_clamp,
+                          quality=RenderQuality.FINAL.value)
+    ref = O.render(O.OracleScene(build), ost, scene.camera.gpu_camera(W, H), sample_count=1)
+    _check_ids_and_aovs(sess, ref)
+    g, o = _gpu_sdf(sess), sdf_stats(ref.counters)
+    assert g["hits"] > 0 and g["invalid"] > 0
+    for k in SDF_NAMES:
+        assert g[k] == o[k], (k, g[k], o[k])
+
+
 def test_sparse_brick_without_zero_crossing_is_culled_before_march(renderer):
     # RenderSessionTests.swift:862-919; the samples (0.05) are not FP16-exact,
     # so this also covers the FP32 distance storage.
