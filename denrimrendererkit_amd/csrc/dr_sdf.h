@@ -193,14 +193,34 @@ __device__ __forceinline__ DrDG brickDistanceGradientCubic(const DrSdfDev &V, co
         return brickDistanceGradient<HALF>(V, c, p);
     vec3 f = v3(g[0] - (float)b[0], g[1] - (float)b[1], g[2] - (float)b[2]);
     float yr[4], dyr[4], dxyr[4];
-#pragma unroll 1
+    // FP16: the 4^3 neighbourhood is the 2^3 corner records of cells
+    // b - 1 + {0, 2} per axis (no clamping inside this range), 8 loads at once.
+    uint32_t q[8][4];
+    if constexpr (HALF) {
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint4 r = V.corner16[c.offset + (b[0] - 1u + 2u * (k & 1u)) + (b[1] - 1u + (k & 2u)) * c.strideY +
+                                       (b[2] - 1u + ((k >> 1) & 2u)) * c.strideZ];
+            q[k][0] = r.x, q[k][1] = r.y, q[k][2] = r.z, q[k][3] = r.w;
+        }
+    }
+    // Sample (x, y, z) of the neighbourhood, x, y, z in 0..3.
+    auto nb = [&](uint32_t x, uint32_t y, uint32_t z) -> float {
+        if constexpr (HALF) {
+            const uint32_t k = (x >> 1) | (y & 2u) | ((z & 2u) << 1);
+            const uint32_t w = q[k][((y & 1u) | ((z & 1u) << 1))];
+            return dr_half((uint16_t)((x & 1u) ? (w >> 16) : (w & 0xffffu)));
+        } else {
+            return sdAt<HALF>(V, c, b[0] - 1u + x, b[1] - 1u + y, b[2] - 1u + z);
+        }
+    };
+#pragma unroll
     for (uint32_t z = 0; z < 4; ++z) {
         float xr[4], dxr[4];
 #pragma unroll
         for (uint32_t y = 0; y < 4; ++y) {
-            uint32_t yy = b[1] + y - 1u, zz = b[2] + z - 1u;
-            float p0 = sdAt<HALF>(V, c, b[0] - 1u, yy, zz), p1 = sdAt<HALF>(V, c, b[0], yy, zz);
-            float p2 = sdAt<HALF>(V, c, b[0] + 1u, yy, zz), p3 = sdAt<HALF>(V, c, b[0] + 2u, yy, zz);
+            float p0 = nb(0u, y, z), p1 = nb(1u, y, z);
+            float p2 = nb(2u, y, z), p3 = nb(3u, y, z);
             xr[y] = catmullRom(p0, p1, p2, p3, f.x);
             dxr[y] = catmullRomD(p0, p1, p2, p3, f.x);
         }
