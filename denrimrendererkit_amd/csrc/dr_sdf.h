@@ -135,6 +135,41 @@ __device__ __forceinline__ float brickDistance(const DrSdfDev &V, const DrBrickC
     return mixf(c0, c1, f.z);
 }
 
+// The last corner record a lane loaded, keyed by its index (a bisection's
+// midpoints close in on one cell, so most of them reuse it).
+struct DrCornerCache {
+    uint32_t idx;
+    uint4 rec;
+};
+
+// brickDistance<true> through a one-record cache (FP16 corner records).
+__device__ __forceinline__ float brickDistanceCached(const DrSdfDev &V, const DrBrickCtx &c, vec3 p,
+                                                     DrCornerCache &cache) {
+    if (!c.valid) return __builtin_inff();
+    uint32_t b[3], n[3];
+    vec3 f;
+    brickGrid(c, p, b, n, f);
+    const uint32_t idx = c.offset + b[0] + b[1] * c.strideY + b[2] * c.strideZ;
+    if (idx != cache.idx) {
+        cache.rec = V.corner16[idx];
+        cache.idx = idx;
+    }
+    const uint32_t w[4] = {cache.rec.x, cache.rec.y, cache.rec.z, cache.rec.w};
+    float d[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        d[2 * k] = dr_half((uint16_t)(w[k] & 0xffffu));
+        d[2 * k + 1] = dr_half((uint16_t)(w[k] >> 16));
+    }
+    float c00 = mixf(d[0], d[1], f.x);
+    float c10 = mixf(d[2], d[3], f.x);
+    float c01 = mixf(d[4], d[5], f.x);
+    float c11 = mixf(d[6], d[7], f.x);
+    float c0 = mixf(c00, c10, f.y);
+    float c1 = mixf(c01, c11, f.y);
+    return mixf(c0, c1, f.z);
+}
+
 struct DrDG {
     float distance;
     vec3 gradient;
@@ -301,8 +336,14 @@ __device__ __forceinline__ bool brickMarch(const DrSdfDev &V, uint32_t bi, float
     if (!inBox || !ctx.valid || cull) return false;
     sc.marches++;
     float minimumStep = fmaxf(cellSize * 0.10f / lupw, 0.0005f);
+    // Successive samples often fall in one cell: FP16 lookups keep the last
+    // corner record.
+    DrCornerCache cache{0xffffffffu, make_uint4(0u, 0u, 0u, 0u)};
+    auto dist = [&](float at) {
+        return HALF ? brickDistanceCached(V, ctx, lo + ld * at, cache) : brickDistance<HALF>(V, ctx, lo + ld * at);
+    };
     float prevT = tNear;
-    float prevD = brickDistance<HALF>(V, ctx, lo + ld * prevT);
+    float prevD = dist(prevT);
     if (!isfinite(prevD)) return false;
     float originalEntryT = prevT;
     if (prevD <= 0.0f) {
@@ -311,7 +352,7 @@ __device__ __forceinline__ bool brickMarch(const DrSdfDev &V, uint32_t bi, float
         #pragma unroll 1
         for (uint32_t bt = 0; bt < 12u; ++bt) {
             searchT = fmaxf(searchT - backStep, 0.0005f);
-            float sd = brickDistance<HALF>(V, ctx, lo + ld * searchT);
+            float sd = dist(searchT);
             if (!isfinite(sd)) break;
             if (sd > 0.0f) {
                 prevT = searchT;
@@ -325,7 +366,7 @@ __device__ __forceinline__ bool brickMarch(const DrSdfDev &V, uint32_t bi, float
     #pragma unroll 1
     for (uint32_t step = 0; step < 192u && t <= tFar && t < closestT; ++step) {
         sc.marchSteps++;
-        float d = brickDistance<HALF>(V, ctx, lo + ld * t);
+        float d = dist(t);
         if (!isfinite(d)) return false;
         bool crossed = (prevD > 0.0f && d <= 0.0f) || (prevD < 0.0f && d >= 0.0f);
         if (crossed) {
@@ -348,10 +389,11 @@ __device__ __forceinline__ void brickFinish(const DrSdfDev &V, const DrVolume &v
     const DrBrickCtx ctx = brickCtx(V, V.bricks[bi]);
     float lowT = cr.lowT, highT = cr.highT, lowD = cr.lowD;
     float refinedT = cr.highT;
+    DrCornerCache cache{0xffffffffu, make_uint4(0u, 0u, 0u, 0u)};
     #pragma unroll 1
     for (uint32_t k = 0; k < 8u; ++k) {
         float midT = 0.5f * (lowT + highT);
-        float md = brickDistance<HALF>(V, ctx, lo + ld * midT);
+        float md = HALF ? brickDistanceCached(V, ctx, lo + ld * midT, cache) : brickDistance<HALF>(V, ctx, lo + ld * midT);
         if ((lowD > 0.0f && md > 0.0f) || (lowD < 0.0f && md < 0.0f)) {
             lowT = midT;
             lowD = md;
