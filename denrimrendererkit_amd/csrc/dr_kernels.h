@@ -24,7 +24,7 @@
 
 // Device counter words: queue lengths per bounce (index b = queue consumed
 // by bounce b), work-stealing cursors of extend and shadow per bounce, and
-// shadow-queue lengths per bounce. Zeroed by k_raygen at the start of each
+// shadow-queue lengths per bounce. Zeroed by k_batch_begin at the start of each
 // sample batch, so no counter is reset while a kernel reads it.
 #define DR_CTR_QUEUE 0
 #define DR_CTR_EXTEND (DR_CTR_QUEUE + DR_MAX_BOUNCES + 1)
@@ -71,7 +71,7 @@ struct DrOutputs {
 };
 
 enum DrStage {
-    DR_STAGE_RAYGEN = 0,
+    DR_STAGE_RAYGEN = 0,  // batch start (counters); the rays come from EXTEND of bounce 0
     DR_STAGE_EXTEND = 1,
     DR_STAGE_SHADE = 2,
     DR_STAGE_SHADOW = 3,

@@ -148,13 +148,13 @@ __device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d
 }
 
 // ---------------------------------------------------------------- raygen --
-__global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        for (int k = 0; k < DR_CTR_WORDS; ++k) P.counters[k] = 0u;
-        P.counters[DR_CTR_QUEUE + 0] = nSlots;
-    }
-    if (i >= nSlots) return;
+// Path slot i of the batch: pixel, sample index, RNG seed and the camera
+// ray (pathTraceKernel's set-up, PathTrace.metal:5204-5230), written to the
+// path state for the stages that follow. The extension kernel of bounce 0
+// runs this for each slot it claims, so rays are generated by the
+// persistent tracing lanes that trace them.
+__device__ __forceinline__ void generatePath(const DrLaunch &L, const DrCamera &cam, const DrPaths &P, uint32_t i,
+                                             vec3 &o, vec3 &d) {
     uint32_t tilePixels = L.tileWidth * L.tileHeight;
     uint32_t sLocal = i / tilePixels;
     uint32_t p = i - sLocal * tilePixels;
@@ -166,7 +166,6 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
     float jy = randomFloat(state);
     float lx = randomFloat(state);
     float ly = randomFloat(state);
-    vec3 o, d;
     makeCameraRay(cam, L.width, L.height, px, py, jx, jy, lx, ly, o, d);
     P.rayO[i] = make_float4(o.x, o.y, o.z, 0.0f);  // w: previousBSDFPDF
     P.rayD[i] = make_float4(d.x, d.y, d.z, 0.0f);
@@ -174,6 +173,13 @@ __global__ void k_raygen(DrLaunch L, DrCamera cam, DrPaths P, uint32_t nSlots) {
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.meta[i] = make_uint4(px + py * L.width, sampleIndex, state, 0u);
     P.queue[0][i] = i;
+}
+
+// Start of a sample batch: every counter word zeroed, bounce 0's queue
+// holds all nSlots paths (generated by k_extend(0)).
+__global__ void k_batch_begin(DrPaths P, uint32_t nSlots) {
+    for (uint32_t k = threadIdx.x; k < DR_CTR_WORDS; k += blockDim.x)
+        P.counters[k] = k == DR_CTR_QUEUE + 0 ? nSlots : 0u;
 }
 
 // Fills the surface record of a hit (attribute half of intersectTriangle,
@@ -853,7 +859,8 @@ __device__ __forceinline__ bool shadowStep(const DrSceneDev &S, const DrPaths &P
 // kernel and before the next k_shade, so radiance is summed in the same
 // order as before.
 template <bool VOL, bool STATS>
-__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrSceneDev S, DrPaths P, uint32_t bounce) {
+__global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrSceneDev S, DrLaunch L, DrCamera cam, DrPaths P,
+                                                                         uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t *queue = P.queue[bounce & 1u];
     const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
@@ -876,9 +883,16 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
                 shadow = qi >= count;
                 if (!shadow) {
-                    slot = queue[qi];
-                    float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
-                    r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                    if (bounce == 0u) {
+                        slot = qi;
+                        vec3 o, d;
+                        generatePath(L, cam, P, slot, o, d);
+                        r = dr_prep(o, d);
+                    } else {
+                        slot = queue[qi];
+                        float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                        r = dr_prep(v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z));
+                    }
                     best = DrHitRec{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
                     laneAny = false;
                 } else {
@@ -911,10 +925,19 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
             if (base >= count) break;
             uint32_t qi = base + laneId();
             if (qi < count) {
-                uint32_t slot = queue[qi];
-                float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                uint32_t slot;
+                vec3 o, d;
+                if (bounce == 0u) {
+                    slot = qi;
+                    generatePath(L, cam, P, slot, o, d);
+                } else {
+                    slot = queue[qi];
+                    float4 o4 = P.rayO[slot], d4 = P.rayD[slot];
+                    o = v3(o4.x, o4.y, o4.z);
+                    d = v3(d4.x, d4.y, d4.z);
+                }
                 float4 h, hn;
-                sceneClosest<true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, nv, nt, sc, h, hn);
+                sceneClosest<true>(S, o, d, stk, nv, nt, sc, h, hn);
                 P.hit[slot] = h;
                 if (__float_as_uint(h.w) != DR_MISS && (__float_as_uint(h.w) & DR_VOL_HIT)) P.hitN[slot] = hn;
             }
@@ -1692,17 +1715,17 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
     const bool vol = S.sdf.volumeCount != 0u;
     switch (stage) {
     case DR_STAGE_RAYGEN:
-        hipLaunchKernelGGL(k_raygen, dim3((nSlots + bThreads - 1) / bThreads), dim3(bThreads), 0, st, L, cam, P, nSlots);
+        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(bThreads), 0, st, P, nSlots);
         break;
     case DR_STAGE_EXTEND:
         if (vol && L.statsEnabled)
-            hipLaunchKernelGGL((k_extend<true, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+            hipLaunchKernelGGL((k_extend<true, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, cam, P, bounce);
         else if (vol)
-            hipLaunchKernelGGL((k_extend<true, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+            hipLaunchKernelGGL((k_extend<true, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, cam, P, bounce);
         else if (L.statsEnabled)
-            hipLaunchKernelGGL((k_extend<false, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+            hipLaunchKernelGGL((k_extend<false, true>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, cam, P, bounce);
         else
-            hipLaunchKernelGGL((k_extend<false, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, P, bounce);
+            hipLaunchKernelGGL((k_extend<false, false>), dim3(traceGrid), dim3(DR_TRACE_BLOCK), 0, st, S, L, cam, P, bounce);
         break;
     case DR_STAGE_SHADE:
     case DR_STAGE_RESUME: {
