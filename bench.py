@@ -108,8 +108,9 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
     Scenes are resident in HBM before timing. The same arrival schedule is
     served (a) one request at a time on a plain session, (b) by the render
     server (continuous batching, graph replays) at the same spp (identical
-    images), (c) by the server with the learned denoiser at the spp of
-    SERVE_DENOISED_SPP. relMSE against 4096-spp renders is measured for up to
+    images), admitting in arrival order with 16 requests on the GPU and (b')
+    least work first with 4, (c) by the server (least work first, 4) with the
+    learned denoiser at the spp of SERVE_DENOISED_SPP. relMSE against 4096-spp renders is measured for up to
     `verify` requests of at most 256^2 in (a) and (c)."""
     import threading
 
@@ -164,8 +165,8 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
     span = time.perf_counter() - t0
     out = {"plain": {"images_per_s": round(n_requests / span, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}}
 
-    def served(denoise):
-        server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=16)
+    def served(denoise, admission="arrival", max_active=16):
+        server = RenderServer(renderer, memory_budget_bytes=24 << 30, max_active=max_active, admission=admission)
         server.set_denoiser_network(net)
         outstanding, lat, errors, got = {}, [], [], {}
         lock = threading.Lock()
@@ -212,8 +213,15 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
             raise RuntimeError("served requests failed: %s" % errors)
         return {"images_per_s": round(n_requests / span, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}, got
 
+    # 16 requests on the GPU at once in arrival order; 4 at once, least work
+    # first (short previews are not held behind long renders, and four
+    # streams overlap as well as sixteen).
     out["served"], _ = served(False)
-    out["served_denoised"], den_images = served(True)
+    out["served"]["admission"] = "arrival, 16 active"
+    out["served_least_work"], _ = served(False, "least_work", 4)
+    out["served_least_work"]["admission"] = "least work first, 4 active"
+    out["served_denoised"], den_images = served(True, "least_work", 4)
+    out["served_denoised"]["admission"] = "least work first, 4 active"
     # relMSE of the verified requests against 4096-spp renders (another frame seed).
     rel_plain, rel_den = [], []
     for k in check:
@@ -230,6 +238,8 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
                               "served_denoised_mean": round(float(np.mean(rel_den)), 5)}
     out.update({"requests": n_requests, "arrival_rate_per_s": round(rate, 3), "load_vs_plain": load,
                 "speedup_served": round(out["served"]["images_per_s"] / out["plain"]["images_per_s"], 3),
+                "speedup_served_least_work": round(out["served_least_work"]["images_per_s"] /
+                                                   out["plain"]["images_per_s"], 3),
                 "speedup_served_denoised": round(out["served_denoised"]["images_per_s"] /
                                                  out["plain"]["images_per_s"], 3),
                 "mix": "res {96^2:.5, 256^2:.25, 512^2:.15, 1280x720:.1}, spp {16,64,256}, max depth %d" %

@@ -86,7 +86,7 @@ class Settings(C.Structure):
 
 class ServerSettings(C.Structure):
     _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("samples_per_batch", C.c_uint32),
-                ("use_graphs", C.c_uint32)]
+                ("use_graphs", C.c_uint32), ("admission", C.c_uint32)]
 
 
 class GpuCamera(C.Structure):

@@ -165,18 +165,27 @@ bool dr_server::bind(Request *r) {
     return true;
 }
 
-// Admits queued requests in arrival order while the budget and the active
-// limit allow; a request larger than the whole budget fails. Called with mu.
+// Admits queued requests (in arrival order, or least work first) while the
+// budget and the active limit allow; a request larger than the whole budget
+// fails. Called with mu.
 void dr_server::admit() {
     while (!pending.empty() && active.size() < cfg.max_active) {
-        Request *r = pending.front();
+        auto next = pending.begin();
+        if (cfg.admission == DR_ADMIT_LEAST_WORK) {
+            auto work = [](const Request *q) {
+                return uint64_t(q->settings.width) * q->settings.height * std::max(1u, q->samples);
+            };
+            for (auto it = pending.begin(); it != pending.end(); ++it)
+                if (work(*it) < work(*next)) next = it;
+        }
+        Request *r = *next;
         if (r->bytes > cfg.memory_budget_bytes) {
-            pending.pop_front();
+            pending.erase(next);
             fail(r, "request needs more memory than the server budget");
             continue;
         }
         if (!bind(r)) break;
-        pending.pop_front();
+        pending.erase(next);
         if (r->state == DR_REQ_FAILED) continue;
         const uint32_t k = std::max(1u, r->settings.samples_per_launch);
         const bool graphs = cfg.use_graphs && (r->samples + k - 1) / k >= 4u;
@@ -261,7 +270,8 @@ extern "C" {
 dr_status dr_hip_server_create(dr_device *device, const dr_server_settings *settings, dr_server **out) {
     if (!device || !settings || !out) return DR_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (settings->max_active == 0 || settings->memory_budget_bytes == 0) return DR_ERR_INVALID_ARGUMENT;
+    if (settings->max_active == 0 || settings->memory_budget_bytes == 0 || settings->admission > DR_ADMIT_LEAST_WORK)
+        return DR_ERR_INVALID_ARGUMENT;
     auto *s = new dr_server;
     s->device = device;
     s->cfg = *settings;

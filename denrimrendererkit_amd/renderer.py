@@ -341,9 +341,16 @@ class RenderServer:
     its own stream. A request's pixels equal those of a session rendering it
     alone."""
 
+    ADMISSION = {"arrival": 0, "least_work": 1}
+
     def __init__(self, renderer: DenrimRenderer, memory_budget_bytes: int = 8 << 30, max_active: int = 16,
-                 samples_per_batch: int = 0, use_graphs: bool = True):
-        cfg = _lib.ServerSettings(memory_budget_bytes, max_active, samples_per_batch, int(use_graphs))
+                 samples_per_batch: int = 0, use_graphs: bool = True, admission: str = "arrival"):
+        """admission: "arrival" admits queued requests in submission order,
+        "least_work" the one with the fewest pixel samples first."""
+        if admission not in self.ADMISSION:
+            raise ValueError(f"admission must be one of {sorted(self.ADMISSION)}")
+        cfg = _lib.ServerSettings(memory_budget_bytes, max_active, samples_per_batch, int(use_graphs),
+                                  self.ADMISSION[admission])
         h = C.c_void_p()
         _lib.check(_lib.load().dr_hip_server_create(renderer._h, C.byref(cfg), C.byref(h)), "server")
         self.renderer = renderer

@@ -249,7 +249,12 @@ typedef struct {
     uint32_t max_active;           /* requests on the GPU at once */
     uint32_t samples_per_batch;    /* samples a request encodes per scheduling step; 0 = all */
     uint32_t use_graphs;           /* replay batch graphs (dr_hip_session_set_graphs) */
+    uint32_t admission;            /* DR_ADMIT_ARRIVAL or DR_ADMIT_LEAST_WORK */
 } dr_server_settings;
+/* Order in which queued requests are admitted: arrival order, or least work
+ * first (width x height x samples, ties in arrival order), which lets short
+ * previews pass long renders in the queue. */
+enum { DR_ADMIT_ARRIVAL = 0, DR_ADMIT_LEAST_WORK = 1 };
 typedef enum { DR_REQ_QUEUED = 0, DR_REQ_RUNNING = 1, DR_REQ_DONE = 2, DR_REQ_FAILED = 3 } dr_request_state;
 
 /* Thread-local message of the last failing call ("" if none). */

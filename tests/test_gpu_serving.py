@@ -76,11 +76,12 @@ def _alone(renderer, gs, camera, st, spp):
     return s.pixels(RenderOutput.BEAUTY), s.pixels(RenderOutput.ALBEDO)
 
 
-def test_served_requests_equal_alone_in_mix(renderer):
+@pytest.mark.parametrize("admission", ["arrival", "least_work"])
+def test_served_requests_equal_alone_in_mix(renderer, admission):
     scenes = [synthetic.material_preview_scene(seed, hdri=(64, 32), segments=24) for seed in range(3)]
     uploaded = [renderer.upload(compile_scene(s)) for s in scenes]
     reqs = [(0, 48, 48, 8), (1, 64, 32, 4), (2, 24, 40, 16), (0, 32, 32, 2), (1, 48, 48, 8), (2, 64, 64, 3)]
-    server = RenderServer(renderer, memory_budget_bytes=4 << 30, max_active=3)
+    server = RenderServer(renderer, memory_budget_bytes=4 << 30, max_active=3, admission=admission)
     try:
         tickets = [server.submit(uploaded[i], _settings(W, H), scenes[i].camera, spp) for i, W, H, spp in reqs]
         for t in tickets:
@@ -103,6 +104,33 @@ def test_served_requests_equal_alone_in_mix(renderer):
         assert 0 < use["used_bytes"] <= 4 << 30  # pooled sessions stay within the budget
     finally:
         server.close()
+
+
+@pytest.mark.parametrize("admission", ["arrival", "least_work"])
+def test_admission_order(renderer, admission):
+    # One request on the GPU at a time: a long request A runs, a long B and
+    # a short C queue behind it. Arrival order runs B before C; least work
+    # first runs C before B, so B is still unfinished when C is done.
+    scene = synthetic.material_preview_scene(2, hdri=(64, 32), segments=24)
+    gs = renderer.upload(compile_scene(scene))
+    server = RenderServer(renderer, memory_budget_bytes=2 << 30, max_active=1, admission=admission)
+    try:
+        a = server.submit(gs, _settings(256, 256), scene.camera, 64)
+        b = server.submit(gs, _settings(256, 256), scene.camera, 64)
+        c = server.submit(gs, _settings(16, 16), scene.camera, 1)
+        assert server.wait(c) == "done"
+        state_b = server.state(b)
+        assert state_b in (("done",) if admission == "arrival" else ("queued", "running")), state_b
+        for t in (a, b):
+            assert server.wait(t) == "done"
+        beauty, _ = _alone(renderer, gs, scene.camera, _settings(16, 16), 1)
+        assert np.array_equal(server.result(c), beauty)
+        for t in (a, b, c):
+            server.release(t)
+    finally:
+        server.close()
+    with pytest.raises(ValueError):
+        RenderServer(renderer, admission="shortest")
 
 
 def test_server_budget_limits_concurrency(renderer):
