@@ -40,8 +40,21 @@ struct ConvArgs {
     // output-channel split); the other paths give the same bits, and a test
     // checks that against this one.
     int plain = 0;
+    // Output layer of an inference (f32out, n = 1): when unpackOut is set,
+    // the denoised beauty of the imgW x imgH image is written to unpackOut
+    // as launchUnpack computes it from this layer's output and the planes
+    // below; `out` is written only where the layer cannot write the beauty
+    // itself (launchConvUnpack then runs the unpack kernel over it).
+    const float4 *unpackAlbedo = nullptr;
+    const float4 *unpackBeauty = nullptr;
+    float4 *unpackOut = nullptr;
+    int imgW = 0, imgH = 0;
 };
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream);
+// launchConv of an output layer with unpackOut set: the denoised beauty
+// directly from the pipelined kernel's epilogue, else the convolution
+// followed by launchUnpack (the same values either way).
+hipError_t launchConvUnpack(const ConvArgs &c, hipStream_t stream);
 
 // Weight and bias gradients of a ConvArgs convolution: dw[9][cout][cin] +=
 // sum over pixels of in(p + tap) * dpre(p), db[cout] += sum dpre (FP32

@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
 // one (tile, chunk) item from LDS, the next item's input tile is already
 // being fetched into registers. Accumulation order and epilogue are those of
 // k_conv3x3_lds, so the outputs are the same bits.
-template <int NB, int CHUNKS>
+// UNPACK: the epilogue writes the denoised beauty as k_unpack does.
+template <int NB, int CHUNKS, bool UNPACK = false>
 __global__ __launch_bounds__(256) void k_conv3x3_pipe(ConvArgs c) {
     constexpr int MB = 4, TH = 16, LR = TH + 2, LC = 18, PIX = 40, WROW = 40;
     constexpr int PIECES = LR * LC * 4;             // 16-byte pieces of one input tile chunk
@@ -463,6 +464,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_pipe(ConvArgs c) {
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
             __syncthreads();
+            if (UNPACK) {
+                // The denoised beauty of this tile's image pixels (k_unpack's
+                // arithmetic on the same FP32 outputs).
+                const float *yt = reinterpret_cast<const float *>(tile);
+                for (int p = threadIdx.x; p < TH * 16; p += 256) {
+                    const int oy = ty * TH + (p >> 4), ox = tx * 16 + (p & 15);
+                    if (oy >= c.imgH || ox >= c.imgW) continue;
+                    const size_t i = size_t(oy) * c.imgW + ox;
+                    const float *yp = yt + p * OST;
+                    const float4 a = c.unpackAlbedo[i];
+                    c.unpackOut[i] = make_float4(expm1f(fminf(fmaxf(yp[0], 0.f), 30.f)) * (a.x + 0.01f),
+                                                 expm1f(fminf(fmaxf(yp[1], 0.f), 30.f)) * (a.y + 0.01f),
+                                                 expm1f(fminf(fmaxf(yp[2], 0.f), 30.f)) * (a.z + 0.01f),
+                                                 c.unpackBeauty[i].w);
+                }
+                t = nt;
+                chunk = nc;
+                continue;
+            }
             const int esize = f32 ? 4 : 2;
             const int vecs = NB * 16 * esize / 16;
             for (int v = threadIdx.x; v < TH * 16 * vecs; v += 256) {
@@ -777,15 +797,16 @@ int deviceCUs() {
     return cache[dev];
 }
 
-template <int NB, int CHUNKS>
+template <int NB, int CHUNKS, bool UNPACK = false>
 hipError_t convPipeLaunch(const ConvArgs &c, hipStream_t stream, unsigned tiles) {
     // One resident wave of workgroups (registers and LDS decide how many per CU).
     static int perCU = 0;
-    if (!perCU && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_conv3x3_pipe<NB, CHUNKS>, 256, 0) !=
-                       hipSuccess || perCU < 1))
+    if (!perCU &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_conv3x3_pipe<NB, CHUNKS, UNPACK>, 256, 0) !=
+             hipSuccess || perCU < 1))
         perCU = 1;
     const unsigned grid = std::min<unsigned>(tiles, unsigned(deviceCUs()) * unsigned(perCU));
-    hipLaunchKernelGGL((k_conv3x3_pipe<NB, CHUNKS>), dim3(grid), dim3(256), 0, stream, c);
+    hipLaunchKernelGGL((k_conv3x3_pipe<NB, CHUNKS, UNPACK>), dim3(grid), dim3(256), 0, stream, c);
     return hipGetLastError();
 }
 
@@ -826,6 +847,21 @@ hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
     case 12: return convLaunch<12, 1>(c, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launchConvUnpack(const ConvArgs &c, hipStream_t stream) {
+    if (!c.unpackOut) return launchConv(c, stream);
+    if (!c.f32out || c.relu || c.n != 1 || c.cout != 16 || c.cb || c.imgW > c.W || c.imgH > c.H)
+        return hipErrorInvalidValue;
+    // The same choice launchConv makes for this layer: the pipelined kernel
+    // takes full-resolution 16-channel outputs.
+    const unsigned tiles16 = unsigned(((c.W + 15) / 16) * ((c.H + 15) / 16));
+    if (DR_CONV_LDS && !c.plain && tiles16 >= 4096u && c.ca <= 64)
+        return c.ca > 32 ? convPipeLaunch<1, 2, true>(c, stream, tiles16) : convPipeLaunch<1, 1, true>(c, stream, tiles16);
+    hipError_t e = launchConv(c, stream);
+    if (e != hipSuccess) return e;
+    return launchUnpack(static_cast<const float *>(c.out), c.unpackAlbedo, c.unpackBeauty, c.imgW, c.imgH, c.W,
+                        c.unpackOut, stream);
 }
 
 hipError_t launchWgrad(const WgradArgs &g, hipStream_t stream) {
