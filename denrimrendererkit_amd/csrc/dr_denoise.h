@@ -49,6 +49,9 @@ struct ConvArgs {
     const float4 *unpackBeauty = nullptr;
     float4 *unpackOut = nullptr;
     int imgW = 0, imgH = 0;
+    // When set (BF16 outputs, H and W even), the 2x2 average pool of the
+    // output (launchPool's values, n x H/2 x W/2 x cout) is written here too.
+    uint16_t *poolOut = nullptr;
 };
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream);
 // launchConv of an output layer with unpackOut set: the denoised beauty

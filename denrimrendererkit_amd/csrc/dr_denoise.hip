@@ -198,6 +198,41 @@ constexpr bool outTileFits(int bytes) {
     return 4 * MB * 16 * (NB * 16 + 8) * bytes <= (4 * MB + 2) * 18 * 40 * 2;
 }
 
+// 2x2 average pool of a TH x 16-pixel output tile held in LDS (BF16,
+// `ost` elements per pixel, channels [coBase, coBase + chans) of c.cout) into
+// c.poolOut, with k_pool's arithmetic ((a + b) + (c + d)) * 0.25.
+__device__ __forceinline__ void poolTile(const ConvArgs &c, const uint16_t *tile, int ost, int TH, int chans, int img,
+                                         int ty, int tx, int coBase) {
+    const int Ho = c.H >> 1, Wo = c.W >> 1, vc = chans / 8;
+    for (int v = threadIdx.x; v < (TH / 2) * 8 * vc; v += blockDim.x) {
+        const int pp = v / vc, cv = v - pp * vc;
+        const int py = pp >> 3, px = pp & 7;
+        const int oy = ty * (TH / 2) + py, ox = tx * 8 + px;
+        if (oy >= Ho || ox >= Wo) continue;
+        auto at = [&](int r, int x) { return *reinterpret_cast<const uint4 *>(&tile[(r * 16 + x) * ost + cv * 8]); };
+        const uint4 a = at(2 * py, 2 * px), b = at(2 * py, 2 * px + 1), cc = at(2 * py + 1, 2 * px),
+                    d = at(2 * py + 1, 2 * px + 1);
+        const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w};
+        const uint32_t wc[4] = {cc.x, cc.y, cc.z, cc.w}, wd[4] = {d.x, d.y, d.z, d.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t lohi[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int sh = h * 16;
+                const float val = ((bf2f(uint16_t(wa[k] >> sh)) + bf2f(uint16_t(wb[k] >> sh))) +
+                                   (bf2f(uint16_t(wc[k] >> sh)) + bf2f(uint16_t(wd[k] >> sh)))) *
+                                  0.25f;
+                lohi[h] = f2bf(val);
+            }
+            o[k] = lohi[0] | (lohi[1] << 16);
+        }
+        *reinterpret_cast<uint4 *>(c.poolOut + ((size_t(img) * Ho + oy) * Wo + ox) * c.cout + coBase + cv * 8) =
+            make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
 template <int NB, int MB>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
     constexpr int TH = 4 * MB, LR = TH + 2, LC = 18, PIX = 40;
@@ -320,6 +355,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs c) {
                                                                size_t(p) * OST * esize + q * 16);
             *reinterpret_cast<uint4 *>(static_cast<char *>(c.out) + o * esize + q * 16) = val;
         }
+        if (c.poolOut && !f32) poolTile(c, tile, OST, TH, NB * 16, img, ty, tx, coBase);
         return;
     }
 #pragma unroll
@@ -494,6 +530,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_pipe(ConvArgs c) {
                                                                    size_t(p) * OST * esize + q * 16);
                 *reinterpret_cast<uint4 *>(static_cast<char *>(c.out) + o * esize + q * 16) = val;
             }
+            if (c.poolOut && !f32) poolTile(c, tile, OST, TH, NB * 16, img, ty, tx, 0);
         }
         t = nt;
         chunk = nc;
@@ -818,7 +855,47 @@ hipError_t convLaunch(const ConvArgs &c, hipStream_t stream, unsigned groups = 1
     return hipGetLastError();
 }
 
+hipError_t launchConvOnly(const ConvArgs &c, hipStream_t stream);
+
+// Whether the kernel launchConvOnly picks for c writes its output tile
+// through LDS (k_conv3x3_pipe, or k_conv3x3_lds with a fitting BF16 tile),
+// where poolTile can read it.
+template <int NB, int MB>
+bool poolsInLds() {
+    return DR_CONV_LDS && outTileFits<NB, MB>(2);
+}
+bool convPoolsItself(const ConvArgs &c) {
+    if (!DR_CONV_LDS || c.f32out) return false;
+    const unsigned tiles16 = unsigned(c.n * ((c.W + 15) / 16) * ((c.H + 15) / 16));
+    const unsigned groups = unsigned(c.cout / 16);
+    const int cin = c.ca + c.cb;
+    if (!c.plain && groups > 1 && tiles16 < 256u)
+        return tiles16 * groups >= 256u ? poolsInLds<1, 4>() : poolsInLds<1, 1>();
+    if (!c.plain && tiles16 >= 4096u && cin <= 64 && (groups == 1 || groups == 2)) return true;
+    switch (c.cout / 16) {
+    case 1: return poolsInLds<1, 4>();
+    case 2: return poolsInLds<2, 4>();
+    case 3: return poolsInLds<3, 4>();
+    case 4: return poolsInLds<4, 4>();
+    case 5: return poolsInLds<5, 2>();
+    case 7: return poolsInLds<7, 2>();
+    case 8: return poolsInLds<8, 2>();
+    case 12: return poolsInLds<12, 1>();
+    default: return false;
+    }
+}
+
 hipError_t launchConv(const ConvArgs &c, hipStream_t stream) {
+    if (c.poolOut && (c.f32out || c.H % 2 || c.W % 2)) return hipErrorInvalidValue;
+    if (!c.poolOut || convPoolsItself(c)) return launchConvOnly(c, stream);
+    ConvArgs plainOut = c;
+    plainOut.poolOut = nullptr;
+    hipError_t e = launchConvOnly(plainOut, stream);
+    if (e != hipSuccess) return e;
+    return launchPool(static_cast<const uint16_t *>(c.out), c.poolOut, c.n, c.H, c.W, c.cout, stream);
+}
+
+hipError_t launchConvOnly(const ConvArgs &c, hipStream_t stream) {
     if (c.ca % 16 || c.cb % 16 || c.cout % 16 || (c.cb && (c.H % 2 || c.W % 2))) return hipErrorInvalidValue;
     // Layers with fewer 16x16 tiles than CUs (the deep levels) give each
     // workgroup one 16-channel group of the output, with 4-row tiles when

@@ -220,6 +220,12 @@ dr_status forward(dr_unet *u, Workspace &w, hipStream_t st, const dr::ConvArgs *
         const Op &op = kOps[i];
         if (op.conv) {
             dr::ConvArgs c = convArgs(u, w, u->layers[li++]);
+            // A pool of this output right after it is written by the
+            // convolution's epilogue (launchConv falls back to launchPool).
+            if (i + 1 < kNumOps && !kOps[i + 1].conv && kOps[i + 1].a == op.out) {
+                c.poolOut = w.act[kOps[i + 1].out];
+                ++i;
+            }
             if (unpack && op.out == Y) {
                 c.unpackAlbedo = unpack->unpackAlbedo;
                 c.unpackBeauty = unpack->unpackBeauty;
