@@ -466,6 +466,20 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     g.envRotationY = d->environment_rotation_y;
     g.envMaxRadiance = d->environment_max_radiance;
     g.opaqueOnly = opaqueOnly ? 1u : 0u;
+    // Volume hits blend two materials (mix of opacity and transmission), so
+    // the occlusion test of volume scenes asks for a margin on both sides of
+    // the 0.001 thresholds, which a blend's rounding cannot cross.
+    bool opaqueMargin = opaqueOnly;
+    for (uint32_t i = 0; i < d->material_count && opaqueMargin; ++i) {
+        const DrMaterial &m = mats[i];
+        opaqueMargin = m.s[M_BASE].w >= 0.002f && std::min(std::max(m.s[M_EMISSION].w, 0.f), 1.f) <= 0.0005f;
+    }
+    bool denseFields = false;
+    if (d->volume_count && d->volume_sample_count) {
+        const auto *vs = static_cast<const DrVolumeSample *>(d->volume_samples);
+        for (uint32_t i = 0; i < d->volume_sample_count && !denseFields; ++i)
+            denseFields = vs[i].materialFieldFlags.x != 0u;
+    }
     DrSdfDev &v = g.sdf;
     v.volumes = s->volumes;
     v.dense = s->denseSamples;
@@ -483,6 +497,7 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     v.gridCount = d->volume_brick_grid_count;
     v.gridIndexCount = d->volume_brick_grid_index_count;
     v.distHalf = distHalf ? 1u : 0u;
+    v.opaqueShadows = opaqueMargin && !denseFields ? 1u : 0u;
     *out = s;
     return DR_OK;
 }

@@ -120,9 +120,14 @@ __device__ __forceinline__ void addSdfStats(unsigned long long *stats, const DrS
 // intersectScene (PathTrace.metal:3294-3421): closest triangle (lowest index
 // on equal t), then the closest sparse-volume hit in front of it (:3397-3418).
 // VOL = false compiles the triangle-only form (scenes without volumes).
+// occludeBelow / certain (volumes): an occlusion test; *certain is set
+// when some hit is known to lie below occludeBelow (a triangle, or a sparse
+// crossing whose bracket does), and h.x is then an upper bound of the
+// closest hit's t rather than the hit.
 template <bool VOL>
 __device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d, DrStack &stk, uint32_t &nv,
-                                             uint32_t &nt, DrSdfCounters &sc, float4 &h, float4 &hn) {
+                                             uint32_t &nt, DrSdfCounters &sc, float4 &h, float4 &hn,
+                                             float occludeBelow = -__builtin_inff(), bool *certain = nullptr) {
     DrHitRec hr{__builtin_inff(), 0.0f, 0.0f, DR_MISS};
     bool hit = false;
     if (S.triangleCount) {
@@ -131,9 +136,17 @@ __device__ __forceinline__ bool sceneClosest(const DrSceneDev &S, vec3 o, vec3 d
     }
     h = make_float4(hr.t, hr.u, hr.v, __uint_as_float(hr.id));
     if constexpr (VOL) {
+        if (hit && hr.t < occludeBelow) {
+            *certain = true;
+            return true;
+        }
         DrVolHit vh;
-        if (S.sdf.distHalf ? dr_trace_volumes<true>(S.sdf, o, d, hr.t, sc, vh)
-                          : dr_trace_volumes<false>(S.sdf, o, d, hr.t, sc, vh)) {
+        if (S.sdf.distHalf ? dr_trace_volumes<true>(S.sdf, o, d, hr.t, sc, vh, occludeBelow, certain)
+                          : dr_trace_volumes<false>(S.sdf, o, d, hr.t, sc, vh, occludeBelow, certain)) {
+            if (certain && *certain) {
+                h.x = vh.t;
+                return true;
+            }
             uint32_t id = DR_VOL_HIT | (vh.frontFacing ? DR_VOL_FRONT : 0u) | vh.volume;
             if (vh.dense)
                 h = make_float4(vh.t, __uint_as_float(vh.denseSample), 0.0f, __uint_as_float(id | DR_VOL_DENSE));
@@ -1166,7 +1179,12 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
 // hits, stepping through nearly transparent surfaces and attenuating
 // through transmissive ones; tr returns the transmittance. The triangle-only
 // form limits each query to the remaining distance.
-template <bool VOL>
+// OCCLUDE (volume scenes whose every hit blocks light, S.sdf.opaqueShadows):
+// the first query only has to decide whether some hit lies closer than
+// maxDistance, which a triangle below it or a sparse crossing bracketed
+// below it settles without refining the hit; the counting kernels keep the
+// full queries so their counters stay the reference's.
+template <bool VOL, bool OCCLUDE = false>
 __device__ __forceinline__ bool shadowChain(const DrSceneDev &S, vec3 o, vec3 d, float maxDistance, vec3 &tr,
                                             DrStack &stk, uint32_t &nq, uint32_t &nv, uint32_t &nt,
                                             DrSdfCounters &sc) {
@@ -1174,6 +1192,15 @@ __device__ __forceinline__ bool shadowChain(const DrSceneDev &S, vec3 o, vec3 d,
     bool occluded;
     occluded = true;
     float traveled = 0.0f;
+    if constexpr (VOL && OCCLUDE) {
+        if (S.sdf.opaqueShadows) {
+            ++nq;
+            float4 h4, hn4;
+            bool certain = false;
+            const bool hit = sceneClosest<true>(S, o, d, stk, nv, nt, sc, h4, hn4, maxDistance, &certain);
+            return certain || (hit && h4.x < maxDistance);
+        }
+    }
     for (uint32_t step = 0; step < 8u; ++step) {
         ++nq;
         float4 h4, hn4;
@@ -1264,7 +1291,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrScene
         vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
         float maxDistance = o4.w;
         vec3 tr;
-        const bool occluded = shadowChain<VOL>(S, o, d, maxDistance, tr, stk, nq, nv, nt, sc);
+        const bool occluded = shadowChain<VOL, !STATS>(S, o, d, maxDistance, tr, stk, nq, nv, nt, sc);
         P.shR[k] = occluded ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : make_float4(tr.x, tr.y, tr.z, 1.0f);
     }
     waveAddTotal(P.stats, DR_CNT_RAY_QUERIES, nq);

@@ -107,6 +107,11 @@ struct DrSdfDev {
     uint32_t volumeCount, brickCount, sampleCount, gridCount, gridIndexCount;
     uint32_t renderQuality;     // 2 = final
     uint32_t distHalf;
+    // 1: every material is opaque and lets no light through (as
+    // DrSceneDev::opaqueOnly, with a margin that blends of two materials
+    // keep) and no dense-volume sample overrides material fields, so any hit
+    // in front of a light blocks it.
+    uint32_t opaqueShadows;
 };
 
 struct DrSceneDev {

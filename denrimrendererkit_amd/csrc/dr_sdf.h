@@ -550,9 +550,15 @@ __device__ __forceinline__ bool intersectDense(const DrSdfDev &V, const DrVolume
 
 // closestVolumeHit (PathTrace.metal:3090-3163): dense volumes, then the
 // brick-grid branch for sparse volumes, below closestT.
+// occludeBelow / certain: an occlusion test. When a sparse brick's march
+// brackets a crossing entirely below occludeBelow, the walk stops there with
+// *certain = true and out.t = the bracket's upper end (the refined hit, not
+// computed, lies at or below it); otherwise the result is the closest hit
+// as usual.
 template <bool HALF>
 __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec3 rd, float closestT, DrSdfCounters &sc,
-                                              DrVolHit &out) {
+                                              DrVolHit &out, float occludeBelow = -__builtin_inff(),
+                                              bool *certain = nullptr) {
     bool any = false;
     float best = closestT;
     for (uint32_t vi = 0; vi < V.volumeCount; ++vi) {
@@ -723,6 +729,15 @@ __device__ __forceinline__ bool dr_trace_volumes(const DrSdfDev &V, vec3 ro, vec
                 if (!advance()) break;
             }
             if (!found) break;
+            if (cr.highT < occludeBelow) {
+                // The caller only asks whether some hit lies below
+                // occludeBelow: this crossing's final t is at most highT.
+                out.t = cr.highT;
+                out.volume = vi;
+                out.dense = false;
+                *certain = true;
+                return true;
+            }
             // A crossing's final t lies in its bracket, below closest.t, so
             // it always replaces the current closest hit.
             DrVolHit cand;

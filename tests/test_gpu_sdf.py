@@ -97,6 +97,29 @@ def test_sdf_primary_counters_exact(renderer, quality):
     assert sess.stats()[0] == ref.counters[O.CNT_PRIMARY]
 
 
+def test_opaque_volume_shadow_test_equals_full_query(renderer):
+    # In a volume scene where every hit blocks light, the timed shadow kernel
+    # stops at the first triangle or sparse crossing known to lie in front of
+    # the light, without refining it; the counting kernels run the full
+    # closest-hit query. Both must give the same image, bit for bit, and the
+    # oracle's within the 1-spp bar.
+    scene = synthetic.sdf_bricks_scene(resolution=96, primitives=60, hdri=(256, 128))
+    build = compile_scene(scene)
+    gs = renderer.upload(build)
+    out = []
+    for stats in (False, True):
+        st = RenderSettings(width=96, height=96, max_bounces=4, quality=RenderQuality.FINAL, frame_seed=0,
+                            collects_sdf_traversal_stats=stats)
+        sess = renderer.make_session(gs, st, camera=scene.camera)
+        sess.render(2)
+        out.append((sess, sess.pixels(RenderOutput.BEAUTY)))
+    assert np.array_equal(out[0][1], out[1][1])
+    ost = O.make_settings(build, 96, 96, max_bounces=4, frame_seed=0, clamp=64.0, quality=RenderQuality.FINAL.value)
+    ref = O.render(O.OracleScene(build), ost, scene.camera.gpu_camera(96, 96), sample_count=2)
+    _check_ids_and_aovs(out[0][0], ref)
+    _check_beauty(out[0][0], ref)
+
+
 def test_overlapping_brick_sample_ranges(renderer):
     # Bricks whose sample ranges overlap (the layout allows it: each header
     # names its own offset) and a brick whose range lies past the samples.
