@@ -350,7 +350,9 @@ def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, ise
     sdf_ext, sdf_sh = c[8:18], c[18:28]
     counted_queries = c[0] + c[1] + c[2]
 
-    sess = renderer.make_session(gscene, settings(False), camera=scene.camera, samples_per_launch=1)
+    # samples_per_launch left to the library (0): about 32 M path slots per
+    # batch, so a 64-spp 1080p frame runs as 4 batches of 16 samples.
+    sess = renderer.make_session(gscene, settings(False), camera=scene.camera, samples_per_launch=0)
     sess.encode_next_sample(warmup * spp)
     sess.synchronize()
     sess.reset_stats()
@@ -385,12 +387,14 @@ def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, ise
         nodes, tris, rays, sdf = c[3] + c[34], c[4] + c[35], c[0] + c[1] - c[7] + c[33], sdf_ext
     else:
         nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
-    launches_per_sample = max(1, st_n[dom] // max(1, samples))
+    # Counters are per sample (the counting sample); the timed launches of
+    # the dominant stage carry samples x those bytes between them.
+    launch_share = samples / max(1, st_n[dom])
     sdf_bytes = (SAMPLE_BYTES * (SAMPLES_PER_STEP * sdf[8] + SAMPLES_PER_HIT * sdf[9]) + CELL_BYTES * sdf[2] +
                  BRICK_BYTES * sdf[4])
     tri_bytes = TRI_BYTES * tris + RAY_BYTES * rays
-    bytes_per_launch = (NODE_BYTES * nodes + tri_bytes + sdf_bytes) / launches_per_sample
-    bytes4_per_launch = (NODE4_BYTES * nodes + tri_bytes + sdf_bytes) / launches_per_sample
+    bytes_per_launch = (NODE_BYTES * nodes + tri_bytes + sdf_bytes) * launch_share
+    bytes4_per_launch = (NODE4_BYTES * nodes + tri_bytes + sdf_bytes) * launch_share
     avg_ms = st_ms[dom] / max(1, st_n[dom])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     kname = {"extend": "k_extend", "shadow": "k_shadow", "shade": "k_shade", "sss": "k_sss"}[dom]
@@ -404,7 +408,9 @@ def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, ise
                               "distance sample, 4 B per grid cell, 128 B per brick); frac_node128 charges the "
                               "128 B a four-wide node visit reads",
                 "stage_ms_per_sample": {k: round(v / samples, 3) for k, v in st_ms.items()}}
+    batches = max(1, st_n["raygen"])
     config.update({"spp_per_step": spp, "frame_seed": 0, "bvh_nodes": info["nodes"],
+                   "samples_per_launch": round(samples / batches, 2),
                    "ray_queries_per_sample": int(round(queries / samples)),
                    "counting_sample_queries": int(counted_queries),
                    "rays_per_path": round(queries / samples / (W * H), 3)})

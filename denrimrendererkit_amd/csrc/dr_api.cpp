@@ -106,9 +106,15 @@ static PathLayout pathLayout(size_t N, size_t stride, uint32_t stackMax) {
 }
 
 // Samples of one pixel in flight together when the caller leaves it open:
-// about 2 M path slots per batch, at most 64 samples.
-static uint32_t autoSamplesPerLaunch(uint64_t pixels) {
-    return uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (2u << 20) / std::max<uint64_t>(pixels, 1))));
+// about 32 M path slots per batch (348 B each, 11.7 GB of path state), at
+// most 64 samples. The paths still alive after a few bounces are too few to
+// fill the GPU on their own, so a batch of one sample spends most of its
+// time in thin late-bounce launches whose length is set by their slowest
+// chain; with k samples in flight each launch carries k times the paths
+// (1080p layered terrain, 64 spp: 562 Mrays/s at k = 1, 1118 at 8, 1217 at
+// 16).
+uint32_t drAutoSamplesPerLaunch(uint64_t pixels) {
+    return uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (32u << 20) / std::max<uint64_t>(pixels, 1))));
 }
 
 static void dropGraphs(dr_session *s) {
@@ -653,12 +659,14 @@ static dr_status sessionShape(const dr_scene *scene, const dr_settings *st, Sess
     if (st->quality > 2) return fail(DR_ERR_INVALID_ARGUMENT, "quality must be 0 (preview), 1 (interactive) or 2 (final)");
     if (st->max_bounces > DR_MAX_BOUNCES) return fail(DR_ERR_INVALID_ARGUMENT, "max_bounces exceeds 64");
     const uint64_t pixels = uint64_t(st->width) * st->height;
-    sh.k = st->samples_per_launch ? st->samples_per_launch : autoSamplesPerLaunch(pixels);
+    sh.k = st->samples_per_launch ? st->samples_per_launch : drAutoSamplesPerLaunch(pixels);
     if (pixels * sh.k > 0xFFFFFFFFull / 3) return fail(DR_ERR_INVALID_ARGUMENT, "image too large");
     sh.nSlots = uint32_t(pixels * sh.k);
     const uint32_t full = uint32_t(std::max(1, scene->device->cuCount)) * DR_TRACE_BLOCKS_PER_CU;
     sh.traceGrid = std::max(1u, std::min(full, (sh.nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
-    sh.stride = std::max(sh.nSlots, sh.traceGrid * DR_TRACE_BLOCK);
+    // Traversal-stack spill lanes: the tracers' persistent lanes, and one per
+    // pixel for the per-pixel kernels (preview, first-hit AOVs at finalize).
+    sh.stride = uint32_t(std::max<uint64_t>(pixels, uint64_t(sh.traceGrid) * DR_TRACE_BLOCK));
     sh.lay = pathLayout(sh.nSlots, sh.stride, scene->stackMax);
     if (st->denoiser == DR_DENOISER_APPLE_SVGF)  // RenderSession.swift:575-576
         return fail(DR_ERR_INVALID_SCENE, "Apple SVGF denoising is not available on this platform.");
@@ -854,6 +862,9 @@ static DrLaunch makeLaunch(const dr_session *s, uint32_t sampleBegin, uint32_t s
 
 static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
     const uint32_t nSlots = L.tileWidth * L.tileHeight * L.samplesInBatch;
+    // The persistent tracers get as many workgroups as this batch's paths
+    // fill (a batch may hold fewer samples or pixels than the session).
+    const uint32_t traceGrid = std::max(1u, std::min(s->traceGrid, (nSlots + DR_TRACE_BLOCK - 1) / DR_TRACE_BLOCK));
     auto run = [&](DrStage stage, uint32_t b) -> hipError_t {
         size_t ev = s->stageEvents.size();
         if (s->stageTiming) {
@@ -865,7 +876,7 @@ static dr_status enqueueBatch(dr_session *s, const DrLaunch &L) {
             (void)hipEventRecord(a, s->stream);
         }
         hipError_t e = dr::launchStage(stage, b, s->sceneDev, L, s->camera, s->previousCamera, s->paths, s->out,
-                                       nSlots, s->traceGrid, s->stream);
+                                       nSlots, traceGrid, s->stream);
         if (s->stageTiming) (void)hipEventRecord(s->stageEvents[ev + 1], s->stream);
         s->stageLaunches[stage]++;
         return e;
