@@ -309,11 +309,15 @@ dr_status dr_hip_server_submit(dr_server *s, dr_scene *scene, const dr_settings 
     r->settings = *settings;
     r->camera = *camera;
     r->samples = samples;
-    // Up to about 1 M paths per batch: small requests run in one batch.
+    // Up to about 4 M paths per batch (1.4 GB of path state): small requests
+    // run in one batch, and a 1280x720 request in batches of 4 samples. The
+    // requests on the GPU share it, so a batch needs fewer paths than a
+    // session rendering alone (on the 48-request material-preview stream, 4
+    // active, least work first: 27 images/s at 1 M, 37 at 4 M, 35 at 16 M).
     if (r->settings.samples_per_launch == 0) {
         uint64_t px = uint64_t(settings->width) * settings->height;
         r->settings.samples_per_launch = uint32_t(
-            std::max<uint64_t>(1, std::min<uint64_t>(samples ? samples : 1, (1u << 20) / std::max<uint64_t>(px, 1))));
+            std::max<uint64_t>(1, std::min<uint64_t>(samples ? samples : 1, (4u << 20) / std::max<uint64_t>(px, 1))));
     }
     dr_status rc = dr_hip_session_estimate_bytes(scene, &r->settings, &r->bytes);
     if (rc != DR_OK) return rc;
