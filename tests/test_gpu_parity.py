@@ -84,6 +84,32 @@ def test_cornell_box_multi_sample_batched(renderer):
     assert sess.sample_count == 4
 
 
+def test_batch_size_does_not_change_bits(renderer):
+    # A batch's slots hold pixels in groups of four with their samples
+    # together; 37 x 23 = 851 pixels leaves a group of three at the end.
+    # Every batching of 7 samples (1 at a time, 3 + 3 + 1, all 7 at once by
+    # default) must give the same running mean and AOVs bit for bit, and the
+    # oracle's IDs.
+    from denrimrendererkit_amd import RenderScene
+    scene = RenderScene.cornell_box()
+    build = compile_scene(scene)
+    gs = renderer.upload(build)
+    settings = RenderSettings(width=37, height=23, max_bounces=3, quality=RenderQuality.FINAL, frame_seed=0x5EED)
+    outs = []
+    for spl in (1, 3, 0):
+        sess = renderer.make_session(gs, settings, camera=scene.camera, samples_per_launch=spl)
+        sess.render(7)
+        outs.append((sess.pixels(), sess.triangle_ids(), sess.pixels(RenderOutput.DEPTH)))
+        sess.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    osc = O.OracleScene(build)
+    ost = O.make_settings(build, 37, 23, max_bounces=3, frame_seed=0x5EED, clamp=64.0)
+    ref = O.render(osc, ost, scene.camera.gpu_camera(37, 23), sample_count=7)
+    assert np.array_equal(outs[0][1], ref.aovs[6][..., 0].view(np.uint32))
+
+
 @pytest.mark.parametrize("seed", [0, 0x1234ABCD])
 def test_layered_materials_parity(renderer, seed):
     # Two spheres of each layered kind (rough dielectric, metal, clearcoat,
