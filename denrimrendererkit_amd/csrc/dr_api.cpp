@@ -248,6 +248,10 @@ dr_status dr_hip_scene_upload(dr_device *device, const dr_scene_desc *d, dr_scen
     if (d->triangle_count)
         dr::buildBvh(tris, d->triangle_count, d->max_leaf_triangles ? d->max_leaf_triangles : 4u, nodes, triPos,
                      &s->stats);
+    if (nodes.size() >= (1u << 25)) {  // node byte offsets are 32-bit in traversal
+        delete s;
+        return fail(DR_ERR_INVALID_SCENE, "Scene's BVH has more nodes than the traversal can address (2^25).");
+    }
     s->stackMax = 7u * s->stats.maxDepth + 2u;
 
     // Sparse volumes: compact brick headers, FP16 distances when every

@@ -41,7 +41,7 @@
 // triangle-only tracers (1 = no constraint); the volume variants of extend
 // and shadow are held to DR_VOL_WAVES, the other volume variants are free.
 #ifndef DR_EXTEND_WAVES
-#define DR_EXTEND_WAVES 1
+#define DR_EXTEND_WAVES 4
 #endif
 #ifndef DR_SHADOW_WAVES
 #define DR_SHADOW_WAVES 1

@@ -56,16 +56,19 @@ struct DrStack {
 struct DrRayPre {
     vec3 o, d;
     vec3 inv;  // 1/d with zero components replaced by +-1e-20
-    // Slab offsets: a plane at x is entered at fma(x, inv, c). cLo serves the
-    // boxes' low planes, cHi their high planes: -o * inv moved by e =
-    // 2^-21 |o * inv| toward the near side for the plane the ray enters first
-    // and toward the far side for the one it leaves by (per axis, from the
-    // sign of inv). The rounding of o * inv and the part of the fma's and
-    // 1/d's relative error that scales with |o| stay below e, so a ray far
-    // from the scene's origin never culls a box the exact test keeps; the
-    // part that scales with the box's own coordinates is covered by the
-    // builder's outward widening of every child box.
-    vec3 cLo, cHi;
+    // Slab offsets: a plane at x is crossed at fma(x, inv, c). cNear serves
+    // the plane of each axis the ray enters by (the low plane for inv > 0,
+    // the high one for inv < 0), cFar the one it leaves by: -o * inv moved by
+    // e = 2^-21 |o * inv| toward the near side and the far side. The rounding
+    // of o * inv and the part of the fma's and 1/d's relative error that
+    // scales with |o| stay below e, so a ray far from the scene's origin
+    // never culls a box the exact test keeps; the part that scales with the
+    // box's own coordinates is covered by the builder's outward widening of
+    // every child box.
+    vec3 cNear, cFar;
+    // Byte offset (0 or 16) of the near plane's float4 within each axis's
+    // {lo, hi} pair of a DrNode4; the far plane's is the other one.
+    uint32_t nearX, nearY, nearZ;
     // Box tests run against max(best t) * (1 + 2^-16) + slack, slack =
     // 2^-14 max|o| / max|d|: a triangle's computed hit distance carries the
     // rounding of o - v0 (about 2^-24 |o|, amplified by the triangle's shape
@@ -89,29 +92,40 @@ __device__ __forceinline__ DrRayPre dr_prep(vec3 o, vec3 d) {
     const float k = 4.76837158e-7f;  // 2^-21
     // (|o * inv| is capped so that an infinite offset never meets its
     // opposite and makes a NaN.)
-    const vec3 e = vec3{copysignf(k * fminf(fabsf(oinv.x), 1e37f), r.inv.x),
-                        copysignf(k * fminf(fabsf(oinv.y), 1e37f), r.inv.y),
-                        copysignf(k * fminf(fabsf(oinv.z), 1e37f), r.inv.z)};
-    r.cLo = vec3{-oinv.x - e.x, -oinv.y - e.y, -oinv.z - e.z};
-    r.cHi = vec3{-oinv.x + e.x, -oinv.y + e.y, -oinv.z + e.z};
+    const vec3 e = vec3{k * fminf(fabsf(oinv.x), 1e37f), k * fminf(fabsf(oinv.y), 1e37f),
+                        k * fminf(fabsf(oinv.z), 1e37f)};
+    r.cNear = vec3{-oinv.x - e.x, -oinv.y - e.y, -oinv.z - e.z};
+    r.cFar = vec3{-oinv.x + e.x, -oinv.y + e.y, -oinv.z + e.z};
+    r.nearX = (__float_as_uint(r.inv.x) >> 27) & 16u;  // sign of inv (never 0)
+    r.nearY = (__float_as_uint(r.inv.y) >> 27) & 16u;
+    r.nearZ = (__float_as_uint(r.inv.z) >> 27) & 16u;
     const float om = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
     const float dm = fmaxf(fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fabsf(d.z)), 1e-30f);
     r.slack = 6.10351562e-5f * om / dm;  // 2^-14
     return r;
 }
 
-// Slab test of one box; returns entry distance or +inf on miss.
-__device__ __forceinline__ float dr_box(const DrRayPre &r, float x0, float y0, float z0, float x1, float y1, float z1,
-                                        float tmax) {
-    float tx0 = __builtin_fmaf(x0, r.inv.x, r.cLo.x);
-    float tx1 = __builtin_fmaf(x1, r.inv.x, r.cHi.x);
-    float ty0 = __builtin_fmaf(y0, r.inv.y, r.cLo.y);
-    float ty1 = __builtin_fmaf(y1, r.inv.y, r.cHi.y);
-    float tz0 = __builtin_fmaf(z0, r.inv.z, r.cLo.z);
-    float tz1 = __builtin_fmaf(z1, r.inv.z, r.cHi.z);
-    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    return tn <= tf ? tn : __builtin_inff();
+static_assert(offsetof(DrNode4, lox) == 0 && offsetof(DrNode4, hix) == 16 && offsetof(DrNode4, loy) == 32 &&
+                  offsetof(DrNode4, hiy) == 48 && offsetof(DrNode4, loz) == 64 && offsetof(DrNode4, hiz) == 80 &&
+                  offsetof(DrNode4, child) == 96 && offsetof(DrNode4, meta) == 112,
+              "node planes as the traversal fetches them");
+
+// Slab test of one box given its planes in the order the ray crosses them
+// per axis: true on a hit, with the entry distance in tn. For a valid box
+// (lo <= hi) fma(lo, inv, c) and fma(hi, inv, c) are ordered by the sign of
+// inv, so these are the min and max of each axis's pair: the same tn and
+// exit distance as a test that orders each pair itself.
+__device__ __forceinline__ bool dr_box(const DrRayPre &r, float nx, float ny, float nz, float fx, float fy, float fz,
+                                       float tmax, float &tn) {
+    const float txn = __builtin_fmaf(nx, r.inv.x, r.cNear.x);
+    const float txf = __builtin_fmaf(fx, r.inv.x, r.cFar.x);
+    const float tyn = __builtin_fmaf(ny, r.inv.y, r.cNear.y);
+    const float tyf = __builtin_fmaf(fy, r.inv.y, r.cFar.y);
+    const float tzn = __builtin_fmaf(nz, r.inv.z, r.cNear.z);
+    const float tzf = __builtin_fmaf(fz, r.inv.z, r.cFar.z);
+    tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, 0.0f));
+    const float tf = fminf(fminf(txf, tyf), fminf(tzf, tmax));
+    return tn <= tf;
 }
 
 // Moeller-Trumbore from the exact vertices (e1 = v1 - v0, e2 = v2 - v0 with
@@ -266,16 +280,33 @@ __device__ __forceinline__ void dr_node_step(const DrSceneDev &S, const DrRayPre
         s.cur = e;
     }
     const uint32_t ni = s.cur;
-    const DrNode4 n = S.nodes[ni];
+    // The node's planes are fetched by a 32-bit byte offset from the node
+    // array (ni < 2^25): each axis's near plane at its lo or hi slot by the
+    // ray's sign, the far plane at the other (offset ^ 16).
+    const char *nb = reinterpret_cast<const char *>(S.nodes);
+    const uint32_t base = ni << 7;
+    const uint32_t ox = base | r.nearX, oy = base | r.nearY, oz = base | r.nearZ;
+    const float4 nX = *reinterpret_cast<const float4 *>(nb + ox);
+    const float4 fX = *reinterpret_cast<const float4 *>(nb + (ox ^ 16u));
+    const float4 nY = *reinterpret_cast<const float4 *>(nb + 32 + oy);
+    const float4 fY = *reinterpret_cast<const float4 *>(nb + 32 + (oy ^ 16u));
+    const float4 nZ = *reinterpret_cast<const float4 *>(nb + 64 + oz);
+    const float4 fZ = *reinterpret_cast<const float4 *>(nb + 64 + (oz ^ 16u));
+    struct {
+        uint4 child;
+        uint2 meta;  // leaf triangle counts, child masks (the rest is unused)
+    } n;
+    n.child = *reinterpret_cast<const uint4 *>(nb + 96 + base);
+    n.meta = *reinterpret_cast<const uint2 *>(nb + 112 + base);
     ++nv;
     const float inf = __builtin_inff();
     tmax = __builtin_fmaf(tmax, 1.0000153f, r.slack);  // 1 + 2^-16
-    float t0 = dr_box(r, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tmax);
-    float t1 = dr_box(r, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tmax);
-    float t2 = dr_box(r, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tmax);
-    float t3 = dr_box(r, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tmax);
-    const uint32_t hit = ((t0 != inf ? 1u : 0u) | (t1 != inf ? 2u : 0u) | (t2 != inf ? 4u : 0u) |
-                          (t3 != inf ? 8u : 0u)) & n.meta.y;
+    float t0, t1, t2, t3;
+    const bool h0 = dr_box(r, nX.x, nY.x, nZ.x, fX.x, fY.x, fZ.x, tmax, t0);
+    const bool h1 = dr_box(r, nX.y, nY.y, nZ.y, fX.y, fY.y, fZ.y, tmax, t1);
+    const bool h2 = dr_box(r, nX.z, nY.z, nZ.z, fX.z, fY.z, fZ.z, tmax, t2);
+    const bool h3 = dr_box(r, nX.w, nY.w, nZ.w, fX.w, fY.w, fZ.w, tmax, t3);
+    const uint32_t hit = ((h0 ? 1u : 0u) | (h1 ? 2u : 0u) | (h2 ? 4u : 0u) | (h3 ? 8u : 0u)) & n.meta.y;
     const uint32_t leafBits = (n.meta.y >> 4) & 15u;
     uint32_t leafMask = hit & leafBits;
     const uint32_t innerMask = hit & ~leafBits;
