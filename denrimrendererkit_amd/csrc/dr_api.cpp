@@ -81,10 +81,11 @@ DrCamera toCam(const dr_gpu_camera &c) {
 
 // Path-state block of a session: 8 float4 arrays of N slots, 4 float4
 // arrays of 3N (shadow records), u32 queues (2 x N ping-pong, 3N shadow,
-// N walk, N resume), counters, stats, the sample base word and the
+// N walk, N resume), counters, claim cursors, stats, the sample base word and the
 // per-lane traversal stack spill.
 struct PathLayout {
-    size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, stats, base, ovf;
+    size_t rayO, rayD, thr, rad, meta, hit, hitN, neeT, shO, shD, shC, shR, q0, q1, sq, sss, res, ctr, claims, stats, base,
+        ovf;
     size_t bytes;
 };
 static PathLayout pathLayout(size_t N, size_t stride, uint32_t stackMax) {
@@ -99,7 +100,8 @@ static PathLayout pathLayout(size_t N, size_t stride, uint32_t stackMax) {
     l.hit = take(N * 16), l.hitN = take(N * 16), l.neeT = take(N * 16);
     l.shO = take(3 * N * 16), l.shD = take(3 * N * 16), l.shC = take(3 * N * 16), l.shR = take(3 * N * 16);
     l.q0 = take(N * 4), l.q1 = take(N * 4), l.sq = take(3 * N * 4), l.sss = take(N * 4), l.res = take(N * 4);
-    l.ctr = take(DR_CTR_WORDS * 4), l.stats = take(DR_CNT_SLOTS * 8), l.base = take(4);
+    l.ctr = take(DR_CTR_WORDS * 4), l.claims = take(DR_CLAIM_WORDS * 4), l.stats = take(DR_CNT_SLOTS * 8);
+    l.base = take(4);
     l.ovf = take(stride * (stackMax > DR_LDS_STACK ? stackMax - DR_LDS_STACK : 0u) * 4);
     l.bytes = bytes;
     return l;
@@ -721,6 +723,7 @@ static dr_status bindSession(dr_session *s, dr_scene *scene, const dr_settings *
     P.sssQueue = reinterpret_cast<uint32_t *>(b + l.sss);
     P.resumeQueue = reinterpret_cast<uint32_t *>(b + l.res);
     P.counters = reinterpret_cast<uint32_t *>(b + l.ctr);
+    P.claims = reinterpret_cast<uint32_t *>(b + l.claims);
     P.stats = reinterpret_cast<unsigned long long *>(b + l.stats);
     s->sampleBase = reinterpret_cast<uint32_t *>(b + l.base);
     P.overflow = reinterpret_cast<uint32_t *>(b + l.ovf);

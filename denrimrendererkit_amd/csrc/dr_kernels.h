@@ -27,20 +27,37 @@
 // shadow-queue lengths per bounce. Zeroed by k_batch_begin at the start of each
 // sample batch, so no counter is reset while a kernel reads it.
 #define DR_CTR_QUEUE 0
-#define DR_CTR_EXTEND (DR_CTR_QUEUE + DR_MAX_BOUNCES + 1)
-#define DR_CTR_SHADOWQ (DR_CTR_EXTEND + DR_MAX_BOUNCES)
-#define DR_CTR_SHADOW (DR_CTR_SHADOWQ + DR_MAX_BOUNCES)
-// Subsurface-walk queue lengths and claim cursors per bounce, and the queue
-// of paths whose walk failed and resume surface shading.
-#define DR_CTR_SSSQ (DR_CTR_SHADOW + DR_MAX_BOUNCES)
-#define DR_CTR_SSS (DR_CTR_SSSQ + DR_MAX_BOUNCES)
-#define DR_CTR_RESUMEQ (DR_CTR_SSS + DR_MAX_BOUNCES)
+#define DR_CTR_SHADOWQ (DR_CTR_QUEUE + DR_MAX_BOUNCES + 1)
+#define DR_CTR_Q(b) (DR_CTR_QUEUE + (b))
+#define DR_CTR_SQ(b) (DR_CTR_SHADOWQ + (b))
+// Subsurface-walk queue lengths per bounce, and the queue of paths whose
+// walk failed and resume surface shading.
+#define DR_CTR_SSSQ (DR_CTR_SHADOWQ + DR_MAX_BOUNCES)
+#define DR_CTR_RESUMEQ (DR_CTR_SSSQ + DR_MAX_BOUNCES)
 // First shadow-queue entry of the bounce left to k_shadow: the triangle-only
 // k_sss traces the entries queued before it ran (those of k_shade) beside
 // its walks and stores their count here; k_shadow takes the rest (those of
 // k_shade<RESUME>). Zero when k_sss does not trace shadow rays.
 #define DR_CTR_SHBEGIN (DR_CTR_RESUMEQ + DR_MAX_BOUNCES)
 #define DR_CTR_WORDS (DR_CTR_SHBEGIN + DR_MAX_BOUNCES)
+
+// Claim cursors of the persistent tracers, per bounce and queue kind: the
+// kind's queue is dealt in chunks of DR_CLAIM_CHUNK items to
+// DR_CLAIM_STRIPES cursors in turn (chunk c to cursor c mod stripes), the
+// cursors DR_CLAIM_STRIDE words apart. A wave claims from its own cursor
+// and moves on to the next when that one's chunks are gone, so the claims
+// of a launch's few thousand waves spread over eight words instead of
+// queueing on one. Zeroed by k_batch_begin for the batch's bounces.
+#define DR_CLAIM_STRIPES 8u
+#define DR_CLAIM_CHUNK 64u
+#define DR_CLAIM_STRIDE 64u
+#define DR_CLAIM_EXTEND 0u
+#define DR_CLAIM_SHADOW 1u
+#define DR_CLAIM_SSS 2u
+#define DR_CLAIM_KINDS 3u
+#define DR_CLAIM_PER_BOUNCE (DR_CLAIM_KINDS * DR_CLAIM_STRIPES * DR_CLAIM_STRIDE)
+#define DR_CLAIM_WORDS (DR_MAX_BOUNCES * DR_CLAIM_PER_BOUNCE)
+#define DR_CLAIM_AT(kind, b) ((b) * DR_CLAIM_PER_BOUNCE + (kind) * DR_CLAIM_STRIPES * DR_CLAIM_STRIDE)
 
 struct DrPaths {
     float4 *rayO;   // origin.xyz, previousBSDFPDF
@@ -60,6 +77,7 @@ struct DrPaths {
     uint32_t *sssQueue;     // paths entering a subsurface walk this bounce
     uint32_t *resumeQueue;  // paths whose walk failed: surface shading resumes
     uint32_t *counters;  // DR_CTR_WORDS
+    uint32_t *claims;    // DR_CLAIM_WORDS
     unsigned long long *stats;  // DR_CNT_SLOTS, accumulated over launches
     uint32_t *overflow;  // traversal stack spill, overflowStride lanes
     uint32_t overflowStride;

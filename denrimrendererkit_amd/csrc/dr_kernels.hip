@@ -216,11 +216,15 @@ __device__ __forceinline__ void generatePath(const DrLaunch &L, const DrCamera &
     P.queue[0][i] = i;
 }
 
-// Start of a sample batch: every counter word zeroed, bounce 0's queue
-// holds all nSlots paths (generated by k_extend(0)).
-__global__ void k_batch_begin(DrPaths P, uint32_t nSlots) {
-    for (uint32_t k = threadIdx.x; k < DR_CTR_WORDS; k += blockDim.x)
-        P.counters[k] = k == DR_CTR_QUEUE + 0 ? nSlots : 0u;
+// Start of a sample batch: every counter word and the claim cursors of the
+// batch's bounces zeroed, bounce 0's queue holds all nSlots paths (generated
+// by k_extend(0)).
+__global__ void k_batch_begin(DrPaths P, uint32_t nSlots, uint32_t claimWords) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n = gridDim.x * blockDim.x;
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < DR_CTR_WORDS; k += blockDim.x)
+            P.counters[k] = k == DR_CTR_Q(0) ? nSlots : 0u;
+    for (uint32_t k = i; k < claimWords; k += n) P.claims[k] = 0u;
 }
 
 // Fills the surface record of a hit (attribute half of intersectTriangle,
@@ -525,15 +529,14 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                                                          DrPaths P, DrOutputs O, uint32_t bounce) {
     const uint32_t *queue = RESUME ? P.resumeQueue : P.queue[bounce & 1u];
     uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
-    const uint32_t count = P.counters[(RESUME ? DR_CTR_RESUMEQ : DR_CTR_QUEUE) + bounce];
+    const uint32_t count = P.counters[RESUME ? DR_CTR_RESUMEQ + bounce : DR_CTR_Q(bounce)];
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= count) return;  // whole waves past the end leave together
     uint32_t cPrimary = 0u, cBounce = 0u;
-    if (!RESUME) {  // each queued path was one extension query (k_extend)
-        const uint64_t live = __ballot(1);
-        if (laneId() == uint32_t(__ffsll((unsigned long long)live) - 1))
-            atomicAdd(&P.stats[DR_CNT_RAY_QUERIES], (unsigned long long)__popcll(live));
-    }
+    // Each queued path was one extension query (k_extend): the launch adds
+    // the queue's length once (one atomic per wave on a single word would
+    // serialize a few hundred thousand waves on it).
+    if (!RESUME && qi == 0u) atomicAdd(&P.stats[DR_CNT_RAY_QUERIES], (unsigned long long)count);
 
     const uint32_t slot = queue[qi];
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
@@ -656,6 +659,18 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                 vec3 view = -rd;
                 // NEE sample generation (sampleDirectLighting, :4407-4585); the
                 // visibility term is applied later by k_shadow/k_resolve.
+                // The three samples (emissive triangle, two environment
+                // directions) are drawn first, in the reference's RNG order,
+                // then the bounce direction; the material's pdf and BRDF are
+                // evaluated afterwards in one loop over the samples and the
+                // bounce direction, so the kernel holds one copy of that code
+                // and a wave runs it once per sample for all its lanes.
+                vec3 nl[3];         // sample directions
+                float npdf[3];      // light / environment pdf
+                vec3 nrad[3];       // emission (light); the environment's is looked up in the loop
+                float ng[3];        // G (light) or cos (environment)
+                float now[3], ndw[3];  // shadow record .w of origin and direction
+                bool nvalid[3] = {false, false, false};
                 if (S.lightCount > 0u) {
                     float r0 = randomFloat(state);
                     uint32_t index = selectLightIndex(S.lights, S.lightCount, r0);
@@ -665,7 +680,6 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                     if (lti < S.triangleCount && selectionPDF > 0.0f) {
                         const DrTriangle &lt = S.triangles[lti];
                         const DrMaterial &lm = S.materials[min(light.materialIndex, S.materialCount - 1u)];
-                        vec3 emission = xyz(lm.s[M_EMISSION]);
                         float area = light.area;
                         if (area > 0.0f) {
                             float ux = randomFloat(state);
@@ -680,17 +694,13 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                             float cosS = fmaxf(0.0f, dot(h.normal, ldir));
                             float cosL = fmaxf(0.0f, dot(xyz(light.normal), -ldir));
                             if (cosS > 0.0f && cosL > 0.0f) {
-                                float lightPDF = selectionPDF * lightSolidAnglePDF(d2, cosL, area);
-                                float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, ldir);
-                                float mis = powerHeuristic(lightPDF, bsdfPDF);
-                                float G = cosS * cosL * area / fmaxf(d2, 1e-6f);
-                                vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, ldir);
-                                vec3 pre = brdf * emission * G * mis;
-                                vec3 so = h.position + h.normal * 0.001f;
-                                P.shO[slot * 3u] = make_float4(so.x, so.y, so.z, dist - 0.002f);
-                                P.shD[slot * 3u] = make_float4(ldir.x, ldir.y, ldir.z, selectionPDF);
-                                P.shC[slot * 3u] = make_float4(pre.x, pre.y, pre.z, 1.0f);
-                                neeMask |= 1u;
+                                nvalid[0] = true;
+                                nl[0] = ldir;
+                                npdf[0] = selectionPDF * lightSolidAnglePDF(d2, cosL, area);
+                                nrad[0] = xyz(lm.s[M_EMISSION]);
+                                ng[0] = cosS * cosL * area / fmaxf(d2, 1e-6f);
+                                now[0] = dist - 0.002f;
+                                ndw[0] = selectionPDF;
                             }
                         }
                     }
@@ -704,25 +714,18 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                     float envPDF;
                     vec3 dir = sampleEnvironmentDirection(S, px, py, cdf, coin, envPDF);
                     float cosS = fmaxf(0.0f, dot(h.normal, dir));
-                    if (envPDF > 0.0f && cosS > 0.0f) {
-                        float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, dir);
-                        float mis = powerHeuristic(envPDF, bsdfPDF);
-                        vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, dir);
-                        vec3 env = sampleEnvironment(S, dir);
-                        vec3 pre = brdf * env * cosS * mis;
-                        vec3 so = h.position + h.normal * 0.001f;
-                        uint32_t k = slot * 3u + 1u + e;
-                        P.shO[k] = make_float4(so.x, so.y, so.z, 1.0e20f);
-                        P.shD[k] = make_float4(dir.x, dir.y, dir.z, envPDF * 2.0f);
-                        P.shC[k] = make_float4(pre.x, pre.y, pre.z, 1.0f);
-                        neeMask |= 2u << e;
-                    }
+                    nvalid[1u + e] = envPDF > 0.0f && cosS > 0.0f;
+                    nl[1u + e] = dir;
+                    npdf[1u + e] = envPDF;
+                    ng[1u + e] = cosS;
+                    now[1u + e] = 1.0e20f;
+                    ndw[1u + e] = envPDF * 2.0f;
                 }
-                if (neeMask != 0u) {
-                    flags |= DR_FLAG_NEE;
-                    P.neeT[slot] = make_float4(T.x, T.y, T.z, 0.0f);
-                }
-                // Lobe sampling (:5423-5513).
+                const vec3 Tnee = T;
+                // Lobe sampling (:5423-5513). The clearcoat and specular lobes
+                // sample the visible normals of one GGX distribution each;
+                // one call serves both with the lobe's roughness and
+                // anisotropy.
                 float roughness = clampf(ms<M_PARAMS>(m).x, 0.02f, 1.0f);
                 float anisotropy = materialSpecularAnisotropy(m);
                 vec3 f0 = materialF0(m);
@@ -733,44 +736,65 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                 float d0 = randomFloat(state);
                 float d1 = randomFloat(state);
                 vec3 diffuseDir = orientHemisphere(cosineHemisphere(d0, d1), h.normal);
-                vec3 next;
+                vec3 next = diffuseDir;
                 float lobe = randomFloat(state);
-                if (lobe < pc) {
+                const bool coat = lobe < pc;
+                bool diffuse = !(coat || lobe < pc + ps);
+                if (!diffuse) {
                     float a = randomFloat(state);
                     float b = randomFloat(state);
-                    vec3 hv = sampleGGXVisibleNormal(a, b, h.normal, h.tangent, h.bitangent, view, ccr, 0.0f);
+                    vec3 hv = sampleGGXVisibleNormal(a, b, h.normal, h.tangent, h.bitangent, view, coat ? ccr : roughness,
+                                                     coat ? 0.0f : anisotropy);
                     next = normalize(reflect(rd, hv));
                     if (dot(next, h.normal) <= 0.0f) {
                         next = diffuseDir;
-                        T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
-                    } else {
-                        float nDotL = fmaxf(dot(h.normal, next), 1e-5f);
-                        float vDotH = fmaxf(dot(view, hv), 1e-5f);
-                        T *= clearcoat * fresnelSchlickThinFilm(vDotH, materialClearcoatF0Color(m), m) *
-                             geometrySmithG1GGX(nDotL, ccr) / fmaxf(pc, 1e-5f);
-                    }
-                } else if (lobe < pc + ps) {
-                    float a = randomFloat(state);
-                    float b = randomFloat(state);
-                    vec3 hv = sampleGGXVisibleNormal(a, b, h.normal, h.tangent, h.bitangent, view, roughness, anisotropy);
-                    next = normalize(reflect(rd, hv));
-                    if (dot(next, h.normal) <= 0.0f) {
-                        next = diffuseDir;
-                        T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
+                        diffuse = true;
                     } else {
                         float vDotH = fmaxf(dot(view, hv), 1e-5f);
-                        vec3 fresnel = fresnelSchlickThinFilm(vDotH, f0, m);
-                        vec3 cca = materialClearcoatAttenuationForDirection(m, h.normal, view, next);
-                        T *= cca * fresnel *
-                             geometrySmithG1GGXAnisotropic(next, h.normal, h.tangent, h.bitangent, roughness, anisotropy) /
-                             fmaxf(ps, 1e-5f);
+                        vec3 fresnel = fresnelSchlickThinFilm(vDotH, coat ? materialClearcoatF0Color(m) : f0, m);
+                        vec3 w;
+                        float G, p;
+                        if (coat) {
+                            w = v3s(clearcoat);
+                            G = geometrySmithG1GGX(fmaxf(dot(h.normal, next), 1e-5f), ccr);
+                            p = pc;
+                        } else {
+                            w = materialClearcoatAttenuationForDirection(m, h.normal, view, next);
+                            G = geometrySmithG1GGXAnisotropic(next, h.normal, h.tangent, h.bitangent, roughness, anisotropy);
+                            p = ps;
+                        }
+                        T *= w * fresnel * G / fmaxf(p, 1e-5f);
                     }
-                } else {
-                    next = diffuseDir;
-                    T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
                 }
-                prevPdf = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, next);
-                ro = h.position + h.normal * 0.001f;
+                if (diffuse) T *= materialDiffuseBounceWeight(m, h.normal, view, next) / fmaxf(pd, 1e-5f);
+                const vec3 so = h.position + h.normal * 0.001f;
+#pragma unroll 1
+                for (uint32_t i = 0; i < 4u; ++i) {
+                    const vec3 l = i == 0u ? nl[0] : (i == 1u ? nl[1] : (i == 2u ? nl[2] : next));
+                    const bool v = i == 0u ? nvalid[0] : (i == 1u ? nvalid[1] : (i == 2u ? nvalid[2] : true));
+                    if (!v) continue;
+                    const float bsdfPDF = materialBSDFPDF(m, h.normal, h.tangent, h.bitangent, view, l);
+                    if (i == 3u) {
+                        prevPdf = bsdfPDF;
+                        break;
+                    }
+                    const float lp = i == 0u ? npdf[0] : (i == 1u ? npdf[1] : npdf[2]);
+                    const float g = i == 0u ? ng[0] : (i == 1u ? ng[1] : ng[2]);
+                    float mis = powerHeuristic(lp, bsdfPDF);
+                    vec3 brdf = evaluateMaterialBRDF(m, h.normal, h.tangent, h.bitangent, view, l);
+                    vec3 rad = i == 0u ? nrad[0] : sampleEnvironment(S, l);
+                    vec3 pre = brdf * rad * g * mis;
+                    const uint32_t k = slot * 3u + i;
+                    P.shO[k] = make_float4(so.x, so.y, so.z, i == 0u ? now[0] : now[1]);
+                    P.shD[k] = make_float4(l.x, l.y, l.z, i == 0u ? ndw[0] : (i == 1u ? ndw[1] : ndw[2]));
+                    P.shC[k] = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                    neeMask |= 1u << i;
+                }
+                if (neeMask != 0u) {
+                    flags |= DR_FLAG_NEE;
+                    P.neeT[slot] = make_float4(Tnee.x, Tnee.y, Tnee.z, 0.0f);
+                }
+                ro = so;
                 rd = next;
                 alive = !terminatePathWithRussianRoulette(bounce, T, state);
             }
@@ -784,14 +808,27 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
     P.rad[slot] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
     P.meta[slot] = make_uint4(meta.x, meta.y, state, flags);
 
-    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_Q(bounce + 1u)]);
     if (alive) queueOut[pos] = slot;
+    // The wave's shadow records take one contiguous run of the shadow queue
+    // (one atomic for all three kinds; an atomic per kind and wave on the one
+    // counter word was most of this kernel's time): the light samples, then
+    // each environment sample's.
+    {
+        const uint64_t m0 = __ballot(neeMask & 1u), m1 = __ballot(neeMask & 2u), m2 = __ballot(neeMask & 4u);
+        const uint32_t c0 = __popcll(m0), c01 = c0 + __popcll(m1), total = c01 + __popcll(m2);
+        const uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+        uint32_t base = 0u;
+        if (total && laneId() == leader) base = atomicAdd(&P.counters[DR_CTR_SQ(bounce)], total);
+        base = __shfl(base, leader);
+        const uint64_t below = (1ull << laneId()) - 1ull;
+        const uint32_t at[3] = {base + uint32_t(__popcll(m0 & below)), base + c0 + uint32_t(__popcll(m1 & below)),
+                                base + c01 + uint32_t(__popcll(m2 & below))};
 #pragma unroll
-    for (uint32_t j = 0; j < 3u; ++j) {
-        bool v = (neeMask >> j) & 1u;
-        uint32_t sp = waveAppend(v, &P.counters[DR_CTR_SHADOWQ + bounce]);
-        if (v) P.shadowQueue[sp] = slot * 3u + j;
-        else P.shC[slot * 3u + j].w = 0.0f;
+        for (uint32_t j = 0; j < 3u; ++j) {
+            if ((neeMask >> j) & 1u) P.shadowQueue[at[j]] = slot * 3u + j;
+            else P.shC[slot * 3u + j].w = 0.0f;
+        }
     }
     if (L.statsEnabled) {
         waveAddStat(P.stats, DR_CNT_PRIMARY, cPrimary);
@@ -904,8 +941,8 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
                                                                          uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t *queue = P.queue[bounce & 1u];
-    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
-    uint32_t *steal = &P.counters[DR_CTR_EXTEND + bounce];
+    const uint32_t count = P.counters[DR_CTR_Q(bounce)];
+    uint32_t *steal = P.claims + DR_CLAIM_AT(DR_CLAIM_EXTEND, bounce);
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0;
@@ -913,7 +950,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
     DrSdfCounters sc{};
     if constexpr (!VOL) {
         const uint32_t shBegin = bounce ? P.counters[DR_CTR_SHBEGIN + bounce - 1u] : 0u;
-        const uint32_t nShadow = bounce ? P.counters[DR_CTR_SHADOWQ + bounce - 1u] - shBegin : 0u;
+        const uint32_t nShadow = bounce ? P.counters[DR_CTR_SQ(bounce - 1u)] - shBegin : 0u;
         const uint32_t *shQueue = P.shadowQueue + shBegin;
         const bool anyShadow = S.opaqueOnly != 0u;
         uint32_t slot = 0, markV = 0u, markT = 0u;
@@ -959,13 +996,13 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_EXTEND_OCC k_extend(DrScene
         nv -= shNodes;
         nt -= shTris;
     } else {
-        while (true) {
-            uint32_t base = 0;
-            if (laneId() == 0) base = atomicAdd(steal, 64u);
-            base = __shfl(base, 0);
-            if (base >= count) break;
-            uint32_t qi = base + laneId();
-            if (qi < count) {
+        DrClaim cl(steal, count);
+        while (cl.more()) {
+            uint32_t base, j, end;
+            cl.claim(64u, base, j, end);
+            const uint32_t c = base + laneId();
+            if (c < end) {
+                const uint32_t qi = DrClaim::entry(c, j);
                 uint32_t slot;
                 vec3 o, d;
                 if (bounce == 0u) {
@@ -1081,7 +1118,7 @@ __device__ __forceinline__ bool walkStep(const DrSceneDev &S, const DrLaunch &L,
     const uint4 meta = P.meta[slot];
     P.thr[slot] = make_float4(it.w.x, it.w.y, it.w.z, alpha);
     P.meta[slot] = make_uint4(meta.x, meta.y, wk.state, meta.w);
-    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+    uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_Q(bounce + 1u)]);
     if (alive) P.queue[(bounce + 1u) & 1u][pos] = slot;
     const bool resume = !exited;
     uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
@@ -1101,14 +1138,14 @@ template <bool VOL, bool STATS>
 __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S, DrLaunch L, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t count = P.counters[DR_CTR_SSSQ + bounce];
-    uint32_t *claim = &P.counters[DR_CTR_SSS + bounce];
+    uint32_t *claim = P.claims + DR_CLAIM_AT(DR_CLAIM_SSS, bounce);
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t cBounce = 0u, cNodes = 0u, cTris = 0u, cSss = 0u;
     uint32_t shQ = 0u, shNodes = 0u, shTris = 0u;
     DrSdfCounters sc{};
     if constexpr (!VOL) {
-        const uint32_t nShadow = P.counters[DR_CTR_SHADOWQ + bounce];
+        const uint32_t nShadow = P.counters[DR_CTR_SQ(bounce)];
         if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[DR_CTR_SHBEGIN + bounce] = nShadow;
         const bool anyShadow = S.opaqueOnly != 0u;
         DrItem it{};
@@ -1146,14 +1183,12 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         cNodes -= shNodes;
         cTris -= shTris;
     }
-    while (VOL) {
-        uint32_t base = 0;
-        if (laneId() == 0) base = atomicAdd(claim, 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        uint32_t qi = base + laneId();
-        bool have = qi < count;
-        uint32_t slot = have ? P.sssQueue[qi] : 0u;
+    for (DrClaim cl(claim, count); VOL && cl.more();) {
+        uint32_t base, j, end;
+        cl.claim(64u, base, j, end);
+        const uint32_t c = base + laneId();
+        const bool have = c < end;
+        uint32_t slot = have ? P.sssQueue[DrClaim::entry(c, j)] : 0u;
         bool alive = false, resume = false;
         if (have) {
             float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot];
@@ -1182,7 +1217,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
             P.thr[slot] = make_float4(T.x, T.y, T.z, t4.w);
             P.meta[slot] = make_uint4(meta.x, meta.y, state, meta.w);
         }
-        uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_QUEUE + bounce + 1u]);
+        uint32_t pos = waveAppend(alive, &P.counters[DR_CTR_Q(bounce + 1u)]);
         if (alive) P.queue[(bounce + 1u) & 1u][pos] = slot;
         uint32_t rpos = waveAppend(resume, &P.counters[DR_CTR_RESUMEQ + bounce]);
         if (resume) P.resumeQueue[rpos] = slot;
@@ -1281,9 +1316,9 @@ template <bool VOL, bool STATS>
 __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrSceneDev S, DrPaths P, uint32_t bounce) {
     __shared__ uint32_t ldsStack[DR_LDS_STACK * DR_TRACE_BLOCK];
     const uint32_t begin = P.counters[DR_CTR_SHBEGIN + bounce];
-    const uint32_t count = P.counters[DR_CTR_SHADOWQ + bounce] - begin;
+    const uint32_t count = P.counters[DR_CTR_SQ(bounce)] - begin;
     const uint32_t *queue = P.shadowQueue + begin;
-    uint32_t *steal = &P.counters[DR_CTR_SHADOW + bounce];
+    uint32_t *steal = P.claims + DR_CLAIM_AT(DR_CLAIM_SHADOW, bounce);
     uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     DrStack stk{dr_lds_ptr(&ldsStack[threadIdx.x]), DR_TRACE_BLOCK, P.overflow + gl, P.overflowStride};
     uint32_t nv = 0, nt = 0, nq = 0;
@@ -1307,14 +1342,12 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SHADOW_OCC k_shadow(DrScene
                                   });
         }
     }
-    while (VOL) {
-        uint32_t base = 0;
-        if (laneId() == 0) base = atomicAdd(steal, 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        uint32_t qi = base + laneId();
-        if (qi >= count) continue;
-        uint32_t k = queue[qi];
+    for (DrClaim cl(steal, count); VOL && cl.more();) {
+        uint32_t base, j, end;
+        cl.claim(64u, base, j, end);
+        const uint32_t c = base + laneId();
+        if (c >= end) continue;
+        uint32_t k = queue[DrClaim::entry(c, j)];
         float4 o4 = P.shO[k], d4 = P.shD[k];
         vec3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
         float maxDistance = o4.w;
@@ -1556,7 +1589,7 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) k_preview(DrSceneDev S, DrLaun
 
 // --------------------------------------------------------------- resolve --
 __global__ void k_resolve(DrLaunch L, DrPaths P, uint32_t bounce) {
-    const uint32_t count = P.counters[DR_CTR_QUEUE + bounce];
+    const uint32_t count = P.counters[DR_CTR_Q(bounce)];
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= count) return;
     uint32_t slot = P.queue[bounce & 1u][qi];
@@ -1770,7 +1803,11 @@ hipError_t launchStage(DrStage stage, uint32_t bounce, const DrSceneDev &S, cons
     const bool vol = S.sdf.volumeCount != 0u;
     switch (stage) {
     case DR_STAGE_RAYGEN:
-        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(bThreads), 0, st, P, nSlots);
+    {
+        const uint32_t words = std::max(1u, L.maxBounces) * DR_CLAIM_PER_BOUNCE;
+        hipLaunchKernelGGL(k_batch_begin, dim3((words + bThreads - 1) / bThreads), dim3(bThreads), 0, st, P, nSlots,
+                           words);
+    }
         break;
     case DR_STAGE_EXTEND:
         if (vol && L.statsEnabled)

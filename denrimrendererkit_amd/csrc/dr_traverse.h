@@ -440,10 +440,54 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
     return found;
 }
 
+// A wave's view of a queue's striped claim cursors (DR_CLAIM_STRIPES
+// cursors DR_CLAIM_STRIDE words apart, zero at the start; cursor j deals
+// the queue's chunks j, j + stripes, ... of DR_CLAIM_CHUNK entries). The
+// wave starts on cursor (wave index mod stripes), moves to the next one when
+// a claim reaches a cursor's end, and is done once it has seen every
+// cursor's end; each entry is handed out exactly once.
+struct DrClaim {
+    uint32_t *cursors;
+    uint32_t fullRounds, rest, stripe, end, left;
+    __device__ __forceinline__ uint32_t stripeCount(uint32_t j) const {
+        const uint32_t lo = j * DR_CLAIM_CHUNK;
+        return fullRounds * DR_CLAIM_CHUNK + (rest > lo ? min(rest - lo, DR_CLAIM_CHUNK) : 0u);
+    }
+    __device__ __forceinline__ DrClaim(uint32_t *c, uint32_t count) : cursors(c) {
+        const uint32_t round = DR_CLAIM_STRIPES * DR_CLAIM_CHUNK;
+        fullRounds = count / round;
+        rest = count - fullRounds * round;
+        stripe = (blockIdx.x * ((blockDim.x + 63u) >> 6) + (threadIdx.x >> 6)) % DR_CLAIM_STRIPES;
+        end = stripeCount(stripe);
+        left = DR_CLAIM_STRIPES;
+    }
+    __device__ __forceinline__ bool more() const { return left != 0u; }
+    // Claims take entries of the current cursor (one atomic, by lane 0; all
+    // lanes of the wave call it): the wave's entries are cursor positions
+    // base + [0, take) below end on cursor j.
+    __device__ __forceinline__ void claim(uint32_t take, uint32_t &base, uint32_t &j, uint32_t &e) {
+        uint32_t b = 0u;
+        if (__lane_id() == 0u) b = atomicAdd(cursors + stripe * DR_CLAIM_STRIDE, take);
+        base = __shfl(b, 0);
+        j = stripe;
+        e = end;
+        if (base + take >= end) {
+            --left;
+            stripe = stripe + 1u == DR_CLAIM_STRIPES ? 0u : stripe + 1u;
+            end = stripeCount(stripe);
+        }
+    }
+    // Queue entry of cursor j's position c.
+    __device__ static __forceinline__ uint32_t entry(uint32_t c, uint32_t j) {
+        return ((c / DR_CLAIM_CHUNK) * DR_CLAIM_STRIPES + j) * DR_CLAIM_CHUNK + c % DR_CLAIM_CHUNK;
+    }
+};
+
 // Persistent tracing of a queue with per-lane refill: a lane whose work
 // item is finished takes the next queue entry as soon as at least
 // DR_REFILL_MIN lanes of the wave are idle (one atomic per refill), so a
-// wave is not held by its longest item. fetch(qi, r, best) sets up the
+// wave is not held by its longest item. claim points to the queue's striped
+// claim cursors (DrClaim). fetch(qi, r, best) sets up the
 // first query of queue entry qi; retire(res, found, r, best) consumes a
 // query's result and either sets up the item's next query in r / best and
 // returns true (a shadow chain step, a subsurface-walk segment), or
@@ -453,8 +497,13 @@ __device__ __forceinline__ bool dr_traverse(const DrSceneDev &S, const DrRayPre 
 // the lane's current query is then an any-hit query (ends at its first hit)
 // even where ANY is false, so closest-hit and any-hit items can share a
 // queue.
+// Refill thresholds: idle lanes a wave waits for before it claims, for the
+// stepwise loop (chained queries) and the while-while loop (extension rays).
 #ifndef DR_REFILL_MIN
 #define DR_REFILL_MIN 16
+#endif
+#ifndef DR_REFILL_MIN_WW
+#define DR_REFILL_MIN_WW 32
 #endif
 #ifndef DR_RETIRE_MIN
 #define DR_RETIRE_MIN 16
@@ -475,6 +524,7 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
     // kernel's duration.
     const uint32_t waves = gridDim.x * ((blockDim.x + 63u) >> 6);
     const uint32_t share = max(4u, (count + waves - 1u) / waves);
+    DrClaim cl(claim, count);
     bool active = false, found = false, more = true;
     DrRayPre r;
     DrHitRec best;
@@ -485,16 +535,16 @@ __device__ __forceinline__ void dr_trace_queue(const DrSceneDev &S, uint32_t cou
         if (more) {
             uint64_t idle = __ballot(!active);
             uint32_t nIdle = __popcll(idle);
-            if (nIdle >= min(DR_REFILL_MIN, share)) {
-                uint32_t take = min(nIdle, share);
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(claim, take);
-                base = __shfl(base, 0);
-                if (base + take >= count) more = false;
+            if (nIdle >= min(STEPWISE ? uint32_t(DR_REFILL_MIN) : uint32_t(DR_REFILL_MIN_WW), share)) {
+                const uint32_t take = min(nIdle, share);
+                uint32_t base, j, end;
+                cl.claim(take, base, j, end);
+                more = cl.more();
                 if (!active) {
-                    uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
-                    uint32_t qi = base + rank;
-                    if (rank < take && qi < count) {
+                    const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+                    const uint32_t c = base + rank;
+                    const uint32_t qi = DrClaim::entry(c, j);
+                    if (rank < take && c < end) {
                         fetch(qi, r, best);
                         dr_trav_init(s);
                         if (S.triangleCount == 0) s.done = true;
