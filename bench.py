@@ -38,6 +38,11 @@ NODE_BYTES, NODE4_BYTES, TRI_BYTES, RAY_BYTES = 32, 128, 36, 48
 # 8 x 8 bisection + 3 x 64 cubic polish/normal samples per hit), 4 B per
 # brick-grid cell visited, 128 B per brick entered.
 SAMPLE_BYTES, CELL_BYTES, BRICK_BYTES = 2, 4, 128
+# k_shade: per shaded path the path state it reads (origin, direction,
+# throughput, radiance, meta, hit: 96 B + its queue entry) and writes (80 B +
+# the next queue entry and the NEE throughput); per shadow record 48 B + its
+# queue entry.
+SHADE_PATH_BYTES, SHADOW_RECORD_BYTES = 200, 52
 SAMPLES_PER_STEP, SAMPLES_PER_HIT = 8, 8 * 8 + 3 * 64
 STAT_RAY_QUERIES = 36
 
@@ -58,6 +63,33 @@ def measured_traffic(workload_name, kernel):
         k = t["workloads"][workload_name]["kernels"][kernel]
         cal = t["copy_calibration"]
         return int((k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]) * 1024)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+# VALU issue peak: a wave64 VALU instruction occupies a 16-lane SIMD for 4
+# cycles; 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock.
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+
+
+def valu_issue(insts, avg_ms):
+    """The dominant kernel's VALU issue rate against the issue peak: the
+    tracers run out of instruction issue slots, not bandwidth."""
+    if not insts or avg_ms <= 0:
+        return None
+    rate = insts / (avg_ms / 1e3) / 1e9
+    return {"insts_per_launch": insts, "achieved_ginst_per_s": round(rate, 1), "peak_ginst_per_s": VALU_PEAK_GINST,
+            "frac": round(rate / VALU_PEAK_GINST, 4)}
+
+
+def measured_valu(workload_name, kernel):
+    """VALU instructions (wave64) per dispatch of `kernel` on this workload
+    from the committed rocprofv3 --pmc summary (SQ_INSTS_VALU over the
+    dispatches of the same batch shape), or None when not profiled."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return int(json.load(f)["workloads"][workload_name]["kernels"][kernel]["valu_insts"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -385,28 +417,31 @@ def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, ise
         nodes, tris, rays, sdf = c[28] + c[31], c[29] + c[32], c[7] + c[30], sdf_ext * 0.0
     elif dom == "extend":
         nodes, tris, rays, sdf = c[3] + c[34], c[4] + c[35], c[0] + c[1] - c[7] + c[33], sdf_ext
-    else:
-        nodes, tris, rays, sdf = c[3], c[4], c[0] + c[1] - c[7], sdf_ext
+    else:  # k_shade: no traversal; path state and shadow records
+        nodes, tris, rays, sdf = 0.0, 0.0, 0.0, sdf_ext * 0.0
     # Counters are per sample (the counting sample); the timed launches of
     # the dominant stage carry samples x those bytes between them.
     launch_share = samples / max(1, st_n[dom])
     sdf_bytes = (SAMPLE_BYTES * (SAMPLES_PER_STEP * sdf[8] + SAMPLES_PER_HIT * sdf[9]) + CELL_BYTES * sdf[2] +
                  BRICK_BYTES * sdf[4])
     tri_bytes = TRI_BYTES * tris + RAY_BYTES * rays
-    bytes_per_launch = (NODE_BYTES * nodes + tri_bytes + sdf_bytes) * launch_share
-    bytes4_per_launch = (NODE4_BYTES * nodes + tri_bytes + sdf_bytes) * launch_share
+    shade_bytes = (SHADE_PATH_BYTES * (c[0] + c[1] - c[7]) + SHADOW_RECORD_BYTES * c[2]) if dom == "shade" else 0.0
+    bytes_per_launch = (NODE_BYTES * nodes + tri_bytes + sdf_bytes + shade_bytes) * launch_share
+    bytes4_per_launch = (NODE4_BYTES * nodes + tri_bytes + sdf_bytes + shade_bytes) * launch_share
     avg_ms = st_ms[dom] / max(1, st_n[dom])
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     kname = {"extend": "k_extend", "shadow": "k_shadow", "shade": "k_shade", "sss": "k_sss"}[dom]
     roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(name, kname),
+                "valu_issue": valu_issue(measured_valu(name, kname), avg_ms),
                 "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bytes_per_launch),
                 "frac_node128": round(bytes4_per_launch / (avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                 "bytes_per_launch_node128": int(bytes4_per_launch),
                 "bytes_note": "32 B per node visit + 36 B per triangle test + 48 B per query (+ SDF: 2 B per "
                               "distance sample, 4 B per grid cell, 128 B per brick); frac_node128 charges the "
-                              "128 B a four-wide node visit reads",
+                              "128 B a four-wide node visit reads; k_shade: 200 B per shaded path + 52 B per "
+                              "shadow record",
                 "stage_ms_per_sample": {k: round(v / samples, 3) for k, v in st_ms.items()}}
     batches = max(1, st_n["raygen"])
     config.update({"spp_per_step": spp, "frame_seed": 0, "bvh_nodes": info["nodes"],

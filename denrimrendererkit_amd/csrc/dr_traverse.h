@@ -500,7 +500,7 @@ struct DrClaim {
 // Refill thresholds: idle lanes a wave waits for before it claims, for the
 // stepwise loop (chained queries) and the while-while loop (extension rays).
 #ifndef DR_REFILL_MIN
-#define DR_REFILL_MIN 16
+#define DR_REFILL_MIN 12
 #endif
 #ifndef DR_REFILL_MIN_WW
 #define DR_REFILL_MIN_WW 32
