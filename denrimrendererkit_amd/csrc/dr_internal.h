@@ -40,6 +40,7 @@ struct dr_scene {
     float4 *triPos = nullptr;
     DrTriangle *triangles = nullptr;
     DrMaterial *materials = nullptr;
+    uint32_t *matKey = nullptr;
     DrLight *lights = nullptr;
     float2 *env = nullptr;
     uint32_t *envGuide = nullptr;

@@ -10,7 +10,7 @@
 #define DR_TRACE_BLOCK 128
 #endif
 #ifndef DR_SHADE_BLOCK
-#define DR_SHADE_BLOCK 128
+#define DR_SHADE_BLOCK 256
 #endif
 #define DR_MAX_BOUNCES 64
 // Size of that scratch: 16 MiB per device.

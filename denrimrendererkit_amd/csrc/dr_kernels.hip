@@ -55,6 +55,15 @@
 #define DR_EXTEND_OCC __attribute__((amdgpu_waves_per_eu(VOL ? DR_VOL_WAVES : DR_EXTEND_WAVES)))
 #define DR_SHADOW_OCC __attribute__((amdgpu_waves_per_eu(VOL ? DR_VOL_WAVES : DR_SHADOW_WAVES)))
 #define DR_SSS_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SSS_WAVES)))
+// Loop shape of k_sss's persistent tracer (walk segments and shadow-chain
+// steps): stepwise.
+#ifndef DR_SSS_STEPWISE
+#define DR_SSS_STEPWISE true
+#endif
+// Sort keys of k_shade's block-local ordering by what a path hit.
+#ifndef DR_SHADE_KEYS
+#define DR_SHADE_KEYS 16u
+#endif
 #ifndef DR_SHADE_WAVES
 #define DR_SHADE_WAVES 1
 #endif
@@ -530,15 +539,55 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
     const uint32_t *queue = RESUME ? P.resumeQueue : P.queue[bounce & 1u];
     uint32_t *queueOut = P.queue[(bounce + 1u) & 1u];
     const uint32_t count = P.counters[RESUME ? DR_CTR_RESUMEQ + bounce : DR_CTR_Q(bounce)];
-    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= count) return;  // whole waves past the end leave together
+    const uint32_t first = blockIdx.x * blockDim.x;
+    if (first >= count) return;  // whole blocks past the end leave together
     uint32_t cPrimary = 0u, cBounce = 0u;
     // Each queued path was one extension query (k_extend): the launch adds
     // the queue's length once (one atomic per wave on a single word would
     // serialize a few hundred thousand waves on it).
-    if (!RESUME && qi == 0u) atomicAdd(&P.stats[DR_CNT_RAY_QUERIES], (unsigned long long)count);
+    if (!RESUME && first + threadIdx.x == 0u) atomicAdd(&P.stats[DR_CNT_RAY_QUERIES], (unsigned long long)count);
 
-    const uint32_t slot = queue[qi];
+    // The block's paths are shaded in the order of a key of what they hit
+    // (miss, volume, or the shading class of the triangle's material),
+    // a counting sort in LDS: after the first bounce the paths of a block
+    // hit many materials, and a wave whose lanes share one takes its
+    // material's branches once instead of once per kind. Each path's
+    // computation is unchanged; only which lane does it.
+    uint32_t slot;
+    if constexpr (RESUME) {
+        if (first + threadIdx.x >= count) return;
+        slot = queue[first + threadIdx.x];
+    } else {
+        __shared__ uint32_t sBins[DR_SHADE_KEYS];
+        __shared__ uint32_t sOrder[DR_SHADE_BLOCK];
+        const uint32_t t = threadIdx.x;
+        const uint32_t n = min(blockDim.x, count - first);
+        if (t < DR_SHADE_KEYS) sBins[t] = 0u;
+        __syncthreads();
+        uint32_t mine = 0u, key = 0u, rank = 0u;
+        if (t < n) {
+            mine = queue[first + t];
+            const uint32_t id = __float_as_uint(P.hit[mine].w);
+            key = id == 0xffffffffu ? 0u
+                  : (VOL && (id & DR_VOL_HIT)) ? DR_SHADE_KEYS - 1u
+                  : 1u + S.matKey[min(S.triangles[id].materialID, S.materialCount - 1u)];
+            rank = atomicAdd(&sBins[key], 1u);
+        }
+        __syncthreads();
+        if (t == 0u) {
+            uint32_t run = 0u;
+            for (uint32_t k = 0; k < DR_SHADE_KEYS; ++k) {
+                const uint32_t c = sBins[k];
+                sBins[k] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        if (t < n) sOrder[sBins[key] + rank] = mine;
+        __syncthreads();
+        if (t >= n) return;
+        slot = sOrder[t];
+    }
     float4 o4 = P.rayO[slot], d4 = P.rayD[slot], t4 = P.thr[slot], l4 = P.rad[slot];
     uint4 meta = P.meta[slot];
     float4 h4 = P.hit[slot];
@@ -1152,7 +1201,7 @@ __global__ void __launch_bounds__(DR_TRACE_BLOCK) DR_SSS_OCC k_sss(DrSceneDev S,
         DrWalk wk{};
         bool walk = false, laneAny = false;
         uint32_t markV = 0u, markT = 0u;
-        dr_trace_queue<false, true>(
+        dr_trace_queue<false, DR_SSS_STEPWISE>(
             S, count + nShadow, claim, stk, cNodes, cTris,
             [&](uint32_t qi, DrRayPre &r, DrHitRec &best) {
                 walk = qi < count;

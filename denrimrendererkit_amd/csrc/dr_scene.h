@@ -119,6 +119,7 @@ struct DrSceneDev {
     const float4 *triPos;        // 3 float4 per triangle in leaf order
     const DrTriangle *triangles; // caller's records, original order
     const DrMaterial *materials;
+    const uint32_t *matKey;      // per material: shading class (which branches its paths take), 0..13
     const DrLight *lights;
     const float2 *env;           // (cdf, pdf) per environment texel
     const uint32_t *envGuide;    // envGuideSize + 1 entries: first texel with cdf >= k / envGuideSize
