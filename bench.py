@@ -141,7 +141,7 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
     served (a) one request at a time on a plain session, (b) by the render
     server (continuous batching, graph replays) at the same spp (identical
     images), admitting in arrival order with 16 requests on the GPU and (b')
-    least work first with 4, (c) by the server (least work first, 4) with the
+    least work first with 2, (c) by the server (least work first, 2) with the
     learned denoiser at the spp of SERVE_DENOISED_SPP. relMSE against 4096-spp renders is measured for up to
     `verify` requests of at most 256^2 in (a) and (c)."""
     import threading
@@ -245,15 +245,15 @@ def serving_bench(renderer, n_requests, n_scenes=12, seed=4, max_bounces=4, load
             raise RuntimeError("served requests failed: %s" % errors)
         return {"images_per_s": round(n_requests / span, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}, got
 
-    # 16 requests on the GPU at once in arrival order; 4 at once, least work
-    # first (short previews are not held behind long renders, and four
-    # streams overlap as well as sixteen).
+    # 16 requests on the GPU at once in arrival order; 2 at once, least work
+    # first (short previews are not held behind long renders, and with each
+    # request batching up to 4 M paths two streams fill the GPU).
     out["served"], _ = served(False)
     out["served"]["admission"] = "arrival, 16 active"
-    out["served_least_work"], _ = served(False, "least_work", 4)
-    out["served_least_work"]["admission"] = "least work first, 4 active"
-    out["served_denoised"], den_images = served(True, "least_work", 4)
-    out["served_denoised"]["admission"] = "least work first, 4 active"
+    out["served_least_work"], _ = served(False, "least_work", 2)
+    out["served_least_work"]["admission"] = "least work first, 2 active"
+    out["served_denoised"], den_images = served(True, "least_work", 2)
+    out["served_denoised"]["admission"] = "least work first, 2 active"
     # relMSE of the verified requests against 4096-spp renders (another frame seed).
     rel_plain, rel_den = [], []
     for k in check:
@@ -382,8 +382,8 @@ def measure(renderer, name, steps, warmup, spp_per_step=None, env_mode=None, ise
     sdf_ext, sdf_sh = c[8:18], c[18:28]
     counted_queries = c[0] + c[1] + c[2]
 
-    # samples_per_launch left to the library (0): about 32 M path slots per
-    # batch, so a 64-spp 1080p frame runs as 4 batches of 16 samples.
+    # samples_per_launch left to the library (0): about 64 M path slots per
+    # batch, so a 64-spp 1080p frame runs as 2 batches of 32 samples.
     sess = renderer.make_session(gscene, settings(False), camera=scene.camera, samples_per_launch=0)
     sess.encode_next_sample(warmup * spp)
     sess.synchronize()

@@ -108,15 +108,16 @@ static PathLayout pathLayout(size_t N, size_t stride, uint32_t stackMax) {
 }
 
 // Samples of one pixel in flight together when the caller leaves it open:
-// about 32 M path slots per batch (348 B each, 11.7 GB of path state), at
+// about 64 M path slots per batch (348 B each, 23 GB of path state), at
 // most 64 samples. The paths still alive after a few bounces are too few to
 // fill the GPU on their own, so a batch of one sample spends most of its
 // time in thin late-bounce launches whose length is set by their slowest
 // chain; with k samples in flight each launch carries k times the paths
 // (1080p layered terrain, 64 spp: 562 Mrays/s at k = 1, 1118 at 8, 1217 at
-// 16).
+// 16 with the tracers of the time; later 1598 at 16, 1689 at 32, 1727 at
+// 64).
 uint32_t drAutoSamplesPerLaunch(uint64_t pixels) {
-    return uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (32u << 20) / std::max<uint64_t>(pixels, 1))));
+    return uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (64u << 20) / std::max<uint64_t>(pixels, 1))));
 }
 
 static void dropGraphs(dr_session *s) {

@@ -109,7 +109,7 @@ std::recursive_mutex &drDeviceMutex();
 // Sets the thread's dr_last_error() message and returns s.
 dr_status drFail(dr_status s, const std::string &msg);
 // Samples in flight per batch of a session that leaves samples_per_launch
-// open (0): about 32 M path slots, 1 to 64 samples.
+// open (0): about 64 M path slots, 1 to 64 samples.
 uint32_t drAutoSamplesPerLaunch(uint64_t pixels);
 // Device bytes of the U-Net's activations for one width x height image.
 size_t drUnetWorkBytes(uint32_t width, uint32_t height);

@@ -116,7 +116,7 @@ typedef struct {
     uint32_t collects_stats;          /* collectsSDFTraversalStats: query / node / triangle counters */
     uint32_t frame_seed;              /* 0x1234ABCD in the reference (RenderSession.swift:917) */
     uint32_t samples_per_launch;      /* samples of one pixel in flight together; 0 = auto: a session
-                                       * keeps about 32 M paths in flight (at most 64 samples), a
+                                       * keeps about 64 M paths in flight (at most 64 samples), a
                                        * server request about 4 M (at most its sample count) */
     /* DenoiseSettings (Denoising/Denoiser.swift:20-75): the beauty output is
      * denoised on read when `denoiser` is not DR_DENOISER_NONE (which also
