@@ -70,7 +70,7 @@ struct DrPaths {
     float4 *neeT;   // throughput at the NEE vertex
     float4 *shO;    // 3 per slot: shadow origin.xyz, max distance
     float4 *shD;    // 3 per slot: direction.xyz, pdf denominator
-    float4 *shC;    // 3 per slot: unshadowed contribution.xyz, valid
+    float4 *shC;    // 3 per slot: unshadowed contribution.xyz, pdf denominator (0: no record)
     float4 *shR;    // 3 per slot: transmittance.xyz, visible
     uint32_t *queue[2];
     uint32_t *shadowQueue;

@@ -836,7 +836,9 @@ __global__ void __launch_bounds__(DR_SHADE_BLOCK) DR_SHADE_OCC k_shade(DrSceneDe
                     const uint32_t k = slot * 3u + i;
                     P.shO[k] = make_float4(so.x, so.y, so.z, i == 0u ? now[0] : now[1]);
                     P.shD[k] = make_float4(l.x, l.y, l.z, i == 0u ? ndw[0] : (i == 1u ? ndw[1] : ndw[2]));
-                    P.shC[k] = make_float4(pre.x, pre.y, pre.z, 1.0f);
+                    // .w: the record's pdf denominator (> 0), so k_resolve
+                    // reads one float4 per record for both.
+                    P.shC[k] = make_float4(pre.x, pre.y, pre.z, i == 0u ? ndw[0] : (i == 1u ? ndw[1] : ndw[2]));
                     neeMask |= 1u << i;
                 }
                 if (neeMask != 0u) {
@@ -1649,12 +1651,11 @@ __global__ void k_resolve(DrLaunch L, DrPaths P, uint32_t bounce) {
     vec3 direct = v3s(0.0f);
     for (uint32_t j = 0; j < 3u; ++j) {
         uint32_t k = slot * 3u + j;
-        float4 c = P.shC[k];
+        float4 c = P.shC[k];  // .w: pdf denominator, 0 for no record
         if (c.w == 0.0f) continue;
         float4 r = P.shR[k];
         if (r.w == 0.0f) continue;
-        float denom = P.shD[k].w;
-        direct += v3(c.x, c.y, c.z) * v3(r.x, r.y, r.z) / denom;
+        direct += v3(c.x, c.y, c.z) * v3(r.x, r.y, r.z) / c.w;
     }
     float4 t = P.neeT[slot];
     float4 l = P.rad[slot];

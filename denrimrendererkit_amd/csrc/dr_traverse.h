@@ -506,7 +506,7 @@ struct DrClaim {
 #define DR_REFILL_MIN_WW 32
 #endif
 #ifndef DR_RETIRE_MIN
-#define DR_RETIRE_MIN 16
+#define DR_RETIRE_MIN 24
 #endif
 // STEPWISE selects the loop shape: false = while-while (node phase until
 // every lane holds a leaf, then the leaf phase; best for the extension
