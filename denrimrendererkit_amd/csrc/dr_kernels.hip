@@ -38,16 +38,21 @@
 #define DR_FLAG_NEE 2u
 
 // Minimum waves per SIMD the register allocator is held to for the
-// triangle-only tracers (1 = no constraint); the volume variants of extend
+// triangle-only kernels (1 = no constraint); the volume variants of extend
 // and shadow are held to DR_VOL_WAVES, the other volume variants are free.
+// With 32 samples in flight the triangle tracers and k_shade gain from
+// waves over registers: extension and walk/shadow tracing at 5 waves
+// (27 and 16 VGPRs spilled) and k_shade at 4 (10 spilled) took the 1080p
+// terrain from 1724 to 1855 Mrays/s; 6 waves on the walk/shadow tracer
+// (more spills) lost 20%, and the volume tracers stay at 3.
 #ifndef DR_EXTEND_WAVES
-#define DR_EXTEND_WAVES 4
+#define DR_EXTEND_WAVES 5
 #endif
 #ifndef DR_SHADOW_WAVES
 #define DR_SHADOW_WAVES 1
 #endif
 #ifndef DR_SSS_WAVES
-#define DR_SSS_WAVES 3
+#define DR_SSS_WAVES 5
 #endif
 #ifndef DR_VOL_WAVES
 #define DR_VOL_WAVES 3
@@ -65,7 +70,7 @@
 #define DR_SHADE_KEYS 16u
 #endif
 #ifndef DR_SHADE_WAVES
-#define DR_SHADE_WAVES 1
+#define DR_SHADE_WAVES 4
 #endif
 #define DR_SHADE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADE_WAVES)))
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
