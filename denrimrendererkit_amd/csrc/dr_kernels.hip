@@ -72,7 +72,10 @@
 #ifndef DR_SHADE_WAVES
 #define DR_SHADE_WAVES 4
 #endif
-#define DR_SHADE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? 1 : DR_SHADE_WAVES)))
+#ifndef DR_VOL_SHADE_WAVES
+#define DR_VOL_SHADE_WAVES 1
+#endif
+#define DR_SHADE_OCC __attribute__((amdgpu_waves_per_eu(VOL ? DR_VOL_SHADE_WAVES : DR_SHADE_WAVES)))
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
 // Sample index of the batch's first sample: a launch argument, or a device
